@@ -1,0 +1,138 @@
+"""Benchmark of the discrete-Sibson tile engine on one MI355X.
+
+Workload (one step = one tile through the whole device pipeline, inputs
+already resident in HBM): a single 4096 x 4096 tile of 0.5 m cells at
+scale 1 with 4M clustered points (2k Matern parents, 30 % water, 1 % exact
+duplicates), the plain one-tile-per-launch form.
+
+Prints ONE JSON line with cells/s (value), points/s, the tile latency
+percentiles, the roofline of the dominant kernel (k_tiles, HIP events on
+its own stream) and the CPU oracle timed on this host's cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(tile, budget_s: float = 12.0):
+    """The CPU oracle (site-major raster + query gather, OpenMP) on the same
+    tile, repeated until about budget_s of work; returns (cells/s, threads,
+    seconds per tile, oracle output of the last run)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    threads = len(os.sched_getaffinity(0))
+    threads = min(threads, 16)
+    reps, t_total, ref = 0, 0.0, None
+    while reps < 1 or (t_total < budget_s and reps < 5):
+        t0 = time.perf_counter()
+        ref = O.run(tile.params, tile.X, tile.Y, tile.Z, nthreads=threads)
+        t_total += time.perf_counter() - t0
+        reps += 1
+    per = t_total / reps
+    return tile.cells / per, threads, per, reps, ref
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=int, default=4096)
+    ap.add_argument("--points", type=int, default=4_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); tiles are "
+              "independent, so more GPUs would each run their own share", file=sys.stderr)
+        sys.exit(2)
+
+    import terranni_amd as T
+    import terranni_amd.synth as S
+
+    t0 = time.perf_counter()
+    tile = S.workload_single_4m(seed=1, n=args.points, grid=args.grid)
+    log(f"generated {tile.n} points, {tile.cells} cells in {time.perf_counter() - t0:.1f}s")
+    fr = T.frame(tile.params)
+
+    ctx = T.Context(0)
+    dX, dY, dZ = ctx.upload(tile.X), ctx.upload(tile.Y), ctx.upload(tile.Z)
+    dH = ctx.alloc((tile.params.nrows, tile.params.ncols), np.float32)
+
+    for _ in range(args.warmup):
+        ctx.tile_run_dev(tile.params, dX, dY, dZ, tile.n, dH)
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        ctx.tile_run_dev(tile.params, dX, dY, dZ, tile.n, dH)  # ends in a stream sync
+        lat.append(time.perf_counter() - s0)
+    elapsed = time.perf_counter() - t_start
+    launches, kern_ms = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    h = dH.to_host()
+
+    cells_s = tile.cells * args.steps / elapsed
+    pts_s = tile.n * args.steps / elapsed
+    k_ms = kern_ms / max(launches, 1)
+    alg_bytes = 12 * tile.n + 4 * tile.cells  # points in (X, Y, Z int32) + FP32 heights out
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": "DEM cells/s (single 4096^2 tile, 4M clustered points, plain form)",
+        "value": cells_s,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded Matern-cluster LiDAR, LAS int32 at 0.01 m)",
+        "config": {"workload": "single tile 4096x4096 cells at 0.5 m, scale 1, 4M clustered points",
+                   "points": tile.n, "cells": tile.cells, "pixels": fr["W"] * fr["H"],
+                   "form": "plain"},
+        "points_per_s": pts_s,
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                       "p99": float(np.percentile(lat, 99) * 1e3)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_tiles", "kernel_ms": k_ms, "alg_bytes": alg_bytes},
+    }
+    if not args.no_cpu:
+        c_cells, threads, per, reps, ref = cpu_baseline(tile)
+        out["cpu_baseline"] = {"value": c_cells, "unit": "cells/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"the same tile, full, {reps} run(s), {per:.2f} s each"}
+        valid = ref["h64"] != T.NODATA
+        rel = np.abs(h[valid] - ref["h64"][valid]) / np.maximum(np.abs(ref["h64"][valid]), 1e-300)
+        out["accuracy"] = {"max_rel_err": float(rel.max()),
+                           "rms_rel_err": float(np.sqrt(np.mean(rel ** 2))),
+                           "nodata_match": bool(np.array_equal(valid, h != T.NODATA)),
+                           "stolen_pairs": ref["S"], "candidate_pairs": ref["K"]}
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

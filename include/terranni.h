@@ -1,0 +1,178 @@
+/*
+ * terranni.h -- C ABI of the MI355X (gfx950) discrete-Sibson tile engine.
+ *
+ * This is the thin device boundary that replaces TerraNNI's internal
+ * cuda_nni.h interface (cuda_nni.h:29-42: init, setup_run, cudaSetup,
+ * bufferAnalysis, completeAnalysis, cleanupCUDA, cudaSwitchVoronoiRB,
+ * cudaRegisterAllBuffers) together with the OpenGL passes of nni.cpp that
+ * produce the discrete Voronoi diagram and the query stencil.  The public C++
+ * API of the reference (nni.h:32-99, namespace nni) is provided on top of
+ * this ABI by include/nni.h.
+ *
+ * Plain C: plain pointers and sizes, no C++ or framework types.  Every call
+ * returns a tnni_status; nothing calls exit() or abort().  Host buffers stay
+ * owned by the caller; device arenas are owned by the context.
+ *
+ * Exact semantics (shared with the CPU oracle under oracle/):
+ *   - point (X,Y) is de-scaled in FP64, x = fma(X, sx, ox) - origin_x, then
+ *     rounded to the tile-local fixed-point unit 1/upm metre;
+ *   - pixel (k,l) of the W x H raster has its centre at x0 + k*ps, y0 + l*ps
+ *     (the glOrtho frame of nni.cpp:656-660); query (i,j) sits at
+ *     (i*cs, j*cs) (nni.cpp:488);
+ *   - a site covers a pixel if it lies in its region (planes: square of
+ *     half-side r/sqrt(2), nni.cpp:183; cones: disk of radius r);
+ *   - label(pixel) = the covering site with the smallest integer squared
+ *     distance, ties to the lowest input index;
+ *   - query q steals pixel x if q's region covers x and
+ *     |x - q|^2 <= |x - label(x)|^2 (the depth test of nni.cpp:794-820);
+ *   - height(q) = mean height of the stolen pixels' sites, -9999 if none
+ *     (cuda_nni.cu:491-503).
+ */
+#ifndef TERRANNI_H
+#define TERRANNI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TNNI_ABI_VERSION 1
+
+typedef enum tnni_status {
+    TNNI_OK = 0,
+    TNNI_PENDING = 1,        /* tnni_poll: request still running            */
+    TNNI_E_INVALID = -1,     /* bad argument                                 */
+    TNNI_E_GEOMETRY = -2,    /* tile geometry not representable exactly      */
+    TNNI_E_NODEVICE = -3,    /* no HIP device / library built without one    */
+    TNNI_E_HIP = -4,         /* a HIP runtime call failed                    */
+    TNNI_E_NOMEM = -5,       /* device allocation failed                     */
+    TNNI_E_BUDGET = -6,      /* request larger than the context's HBM budget */
+    TNNI_E_SHUTDOWN = -7     /* context is being destroyed                   */
+} tnni_status;
+
+enum { TNNI_ALG_PLANES = 0, TNNI_ALG_CONES = 1 }; /* nni::Settings::algorithm */
+
+enum {
+    TNNI_ACC_FIXED64 = 0,   /* int64 sum of the LAS Z integers (default)     */
+    TNNI_ACC_FP32_2SUM = 1, /* FP32 two-sum of the Z integers                */
+    TNNI_ACC_S0 = 2         /* the reference's zMultiplier int32 arithmetic  */
+};
+
+typedef struct tnni_ctx tnni_ctx;
+
+typedef struct tnni_ctx_opts {
+    int device;               /* HIP device ordinal                         */
+    int n_slots;              /* served form: batches in flight (0 = 2)     */
+    uint64_t hbm_budget;      /* bytes the context may allocate (0 = 80% of
+                                 free device memory)                        */
+    int max_batch_tiles;      /* served form: tiles per batch (0 = 256)     */
+    int batch_window_us;      /* served form: wait for more arrivals (0)    */
+} tnni_ctx_opts;
+
+/* One work package (nni::WorkPackage nni.h:66-70 + nni::Settings nni.h:72-81)
+ * plus the LAS scale/offset of its points. */
+typedef struct tnni_tile_desc {
+    double origin_x, origin_y;  /* tile origin in metres (FP64)            */
+    double las_scale[3];        /* X, Y, Z scale                           */
+    double las_offset[3];       /* X, Y, Z offset                          */
+    float cell_size;            /* Settings::cell_size                     */
+    int scale;                  /* Settings::scale (pixels per cell side)  */
+    int nrows, ncols;           /* cells in the tile                       */
+    float site_radius;          /* Settings::site_radius                   */
+    float query_radius;         /* Settings::query_radius (as float)       */
+    int algorithm;              /* TNNI_ALG_*                              */
+    int accum_mode;             /* TNNI_ACC_*                              */
+    int units_per_m;            /* fixed-point unit, 0 = auto (100 first)  */
+    int pixel_offset;           /* -1 = the formula of nni.cpp:883         */
+    int point_min;              /* Settings::pointMin   (TNNI_ACC_S0 only) */
+    int multiplier;             /* Settings::multiplier (TNNI_ACC_S0 only) */
+    int bucket_units;           /* bucket side in units, 0 = tuned         */
+    int tile_cells;             /* cell-tile side per workgroup, 0 = auto  */
+} tnni_tile_desc;
+
+/* Derived geometry of a tile (all lengths in fixed-point units). */
+typedef struct tnni_frame_info {
+    int upm;                    /* units per metre                         */
+    int ps_u, cs_u;             /* pixel and cell size                     */
+    int po;                     /* border pixels on each side              */
+    int W, H;                   /* raster size in pixels                   */
+    int x0, y0;                 /* centre of pixel (0,0)                   */
+    int Rs, Rq;                 /* square half-sides (planes)              */
+    int64_t Rs2, Rq2;           /* squared disk radii (cones)              */
+    int Rsb, Rqb;               /* L-inf extent of the regions in use      */
+    int spacing_factor;         /* GL query rounds of the reference        */
+    int algorithm;
+} tnni_frame_info;
+
+/* ---- context ---------------------------------------------------------- */
+/* Replaces cuda_nni::init (cuda_nni.cu:641-669): picks the device, creates
+ * the stream and arenas.  No GL context. */
+tnni_status tnni_ctx_create(const tnni_ctx_opts *opts, tnni_ctx **out);
+/* Replaces nni::cleanup / cuda_nni::cleanupCUDA (nni.cpp:563-566,
+ * cuda_nni.cu:624-632).  Waits for outstanding requests. */
+void tnni_ctx_destroy(tnni_ctx *ctx);
+/* Last error message of this thread (or of ctx when ctx != NULL). */
+const char *tnni_last_error(const tnni_ctx *ctx);
+int tnni_abi_version(void);
+
+/* ---- geometry and admission ------------------------------------------ */
+/* The pixel frame of performNNI (nni.cpp:879-886) and setupRun's glOrtho
+ * (nni.cpp:656-660), in fixed-point units.  Host only. */
+tnni_status tnni_frame(const tnni_tile_desc *desc, tnni_frame_info *out);
+/* Device bytes one tile of n points needs (used for admission). */
+tnni_status tnni_footprint(const tnni_tile_desc *desc, uint64_t n_points,
+                           int with_labels, uint64_t *bytes);
+
+/* ---- plain form: one tile per call, host buffers ---------------------- */
+/* The whole of nni::performNNI's device work for one 2-D time slice
+ * (nni.cpp:870-944): site pass, query pass, accumulate, divide.
+ * X, Y, Z: n LAS int32 coordinates.  out_h: nrows*ncols float heights,
+ * row-major (row j = y index).  out_labels (nullable): W*H int32 labels,
+ * -1 = no covering site.  out_d2 (nullable): W*H uint32 squared distance
+ * of the label, 0xFFFFFFFF where none.  Synchronous. */
+tnni_status tnni_tile_run(tnni_ctx *ctx, const tnni_tile_desc *desc,
+                          const int32_t *X, const int32_t *Y, const int32_t *Z,
+                          uint64_t n, float *out_h, int32_t *out_labels,
+                          uint32_t *out_d2);
+
+/* ---- device-resident buffers ------------------------------------------ */
+tnni_status tnni_dev_alloc(tnni_ctx *ctx, uint64_t bytes, void **dptr);
+tnni_status tnni_dev_free(tnni_ctx *ctx, void *dptr);
+tnni_status tnni_h2d(tnni_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+tnni_status tnni_d2h(tnni_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
+/* Plain form on device pointers: one launch sequence for one tile on the
+ * context stream, then a stream synchronisation. */
+tnni_status tnni_tile_run_dev(tnni_ctx *ctx, const tnni_tile_desc *desc,
+                              const int32_t *dX, const int32_t *dY,
+                              const int32_t *dZ, uint64_t n, float *d_out_h,
+                              int32_t *d_out_labels, uint32_t *d_out_d2);
+
+/* ---- batched form ----------------------------------------------------- */
+/* n_tiles uneven tiles in one pipeline: one bucket sort over all points and
+ * one persistent launch over a device work queue of (tile, cell-block)
+ * items, replayed from a captured hipGraph when the batch shape repeats.
+ * Arrays are indexed by tile; pointers are device pointers; d_out_labels and
+ * d_out_d2 may be NULL (or hold NULL entries). */
+tnni_status tnni_batch_run_dev(tnni_ctx *ctx, int n_tiles,
+                               const tnni_tile_desc *descs,
+                               const int32_t *const *dX,
+                               const int32_t *const *dY,
+                               const int32_t *const *dZ, const uint64_t *n,
+                               float *const *d_out_h,
+                               int32_t *const *d_out_labels,
+                               uint32_t *const *d_out_d2);
+
+/* ---- profiling -------------------------------------------------------- */
+/* When enabled, HIP events bracket every launch of the main tile kernel on
+ * the stream it runs on. */
+tnni_status tnni_profile_enable(tnni_ctx *ctx, int enable);
+tnni_status tnni_profile_read(tnni_ctx *ctx, uint64_t *launches,
+                              double *total_ms, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

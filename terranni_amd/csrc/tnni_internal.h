@@ -1,0 +1,134 @@
+// Internal types shared by the gfx950 kernels and the host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "terranni.h"
+
+namespace tnni {
+
+constexpr int kBlock = 256;            // threads per workgroup (4 wave64)
+constexpr int kWaves = kBlock / 64;
+constexpr int kChunkPts = 4096;        // points per chunk of the sort passes
+constexpr uint32_t kNoD2 = 0xFFFFFFFFu;
+constexpr int kScanBlocks = 512;       // fixed grid of the device-wide scan
+constexpr int kScanItems = 4;          // items per thread per scan step
+
+// Everything a kernel needs to know about one tile of a batch.  Lengths are
+// in the tile-local fixed-point unit (1/upm metre).
+struct TileDev {
+    const int32_t *X, *Y, *Z;   // LAS int32 coordinates (device)
+    int64_t n;                  // points
+    int64_t pt_base;            // first point of this tile in the batch
+    double sx, sy, sz, ox, oy, oz;
+    double origin_x, origin_y;
+    double lo_x, hi_x, lo_y, hi_y;  // window of sites that can cover a pixel
+    int upm, ps_u, cs_u, po, W, H, x0, y0;
+    int Rs, Rq, Rsb, Rqb;
+    uint32_t Rs2, Rq2;
+    int algorithm, mode, nrows, ncols, scale;
+    int point_min, multiplier;
+    int gx0, gy0, bs, nbx, nby; // bucket grid: origin, side, counts
+    int64_t bk_base;            // first bucket of this tile in the batch
+    int tc, ntx, nty;           // cell-tile side and tiles along x / y
+    int pad0;
+    int64_t item_base;          // first work item of this tile
+    float *out_h;
+    int32_t *out_labels;
+    uint32_t *out_d2;
+};
+
+// A run of points of one tile, the unit of work of the sort passes.
+struct Chunk {
+    int tile;
+    int pad;
+    int64_t begin, end;
+};
+
+struct BatchDev {
+    int n_tiles;
+    int n_chunks;
+    uint32_t n_scan;       // buckets of the whole batch + 1
+    uint32_t work_counter; // next work item of the persistent kernel
+    int64_t n_items;
+    uint32_t scan_chunk;   // elements per scan block
+    uint32_t pad;
+};
+
+__host__ __device__ inline int floordiv_i(int a, int b)  // b > 0
+{
+    int q = a / b;
+    if ((a % b) != 0 && a < 0) --q;
+    return q;
+}
+
+// The cells of one work item and the pixels it labels.
+struct Region {
+    int i0, i1, j0, j1;      // cells [i0,i1) x [j0,j1)
+    int rk0, rk1, rl0, rl1;  // pixels labelled into LDS (inclusive)
+    int ok0, ok1, ol0, ol1;  // pixels whose label this item writes out
+};
+
+// Pixel k lies in the query region of cell i iff k - i*scale is in
+// [qa, qb]: |x0 + (k - i*scale)*ps| <= Rqb.
+__host__ __device__ inline int query_lo(const TileDev &t) { return -floordiv_i(t.Rqb + t.x0, t.ps_u); }
+__host__ __device__ inline int query_hi(const TileDev &t) { return floordiv_i(t.Rqb - t.x0, t.ps_u); }
+
+__host__ __device__ inline Region region_of(const TileDev &t, int ti, int tj)
+{
+    Region g;
+    g.i0 = ti * t.tc;
+    g.i1 = g.i0 + t.tc < t.ncols ? g.i0 + t.tc : t.ncols;
+    g.j0 = tj * t.tc;
+    g.j1 = g.j0 + t.tc < t.nrows ? g.j0 + t.tc : t.nrows;
+    const int qa = query_lo(t), qb = query_hi(t);
+    int kg0 = g.i0 * t.scale + qa, kg1 = (g.i1 - 1) * t.scale + qb;
+    int lg0 = g.j0 * t.scale + qa, lg1 = (g.j1 - 1) * t.scale + qb;
+    kg0 = kg0 < 0 ? 0 : kg0;
+    lg0 = lg0 < 0 ? 0 : lg0;
+    kg1 = kg1 > t.W - 1 ? t.W - 1 : kg1;
+    lg1 = lg1 > t.H - 1 ? t.H - 1 : lg1;
+    // every pixel of the raster is owned by exactly one item: the cell
+    // columns' own pixels, plus the border for the first and last items
+    g.ok0 = (g.i0 == 0) ? 0 : t.po + g.i0 * t.scale;
+    g.ok1 = (g.i1 == t.ncols) ? t.W - 1 : t.po + g.i1 * t.scale - 1;
+    g.ol0 = (g.j0 == 0) ? 0 : t.po + g.j0 * t.scale;
+    g.ol1 = (g.j1 == t.nrows) ? t.H - 1 : t.po + g.j1 * t.scale - 1;
+    g.rk0 = kg0 < g.ok0 ? kg0 : g.ok0;
+    g.rk1 = kg1 > g.ok1 ? kg1 : g.ok1;
+    g.rl0 = lg0 < g.ol0 ? lg0 : g.ol0;
+    g.rl1 = lg1 > g.ol1 ? lg1 : g.ol1;
+    return g;
+}
+
+// Host-side launch helpers (tnni_kernels.hip).
+struct LaunchCfg {
+    int fill_grid;      // blocks of the zero pass
+    int sort_grid;      // blocks of the count/scatter passes
+    int main_grid;      // blocks of the persistent tile kernel
+    size_t main_lds;    // dynamic LDS bytes of the tile kernel
+    int algorithm;      // TNNI_ALG_* of every tile in the batch
+    int mode;           // TNNI_ACC_* of every tile in the batch
+};
+
+struct DevBuffers {
+    BatchDev *batch;
+    const TileDev *tiles;
+    const Chunk *chunks;
+    uint32_t *counts;   // n_scan entries, scanned in place into offsets
+    uint32_t *rank;     // one per point
+    uint32_t *partials; // kScanBlocks
+    int4 *sorted;       // one per point
+};
+
+// Enqueue the whole pipeline on `stream`.  ev_main0/ev_main1 (nullable)
+// bracket the tile kernel.
+hipError_t enqueue_pipeline(const DevBuffers &b, const LaunchCfg &cfg,
+                            hipStream_t stream, hipEvent_t ev_main0,
+                            hipEvent_t ev_main1);
+
+// Largest workgroups per CU of the tile kernel at the given LDS size.
+int tile_kernel_occupancy(size_t lds_bytes);
+
+}  // namespace tnni

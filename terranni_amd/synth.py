@@ -1,0 +1,221 @@
+"""Seeded synthetic LiDAR tiles for the benchmark and the parity tests.
+
+The reference ships no data and its generator (data-generator.py) is
+unseeded Python 2 text output, so the workloads are defined here:
+
+  uniform   points uniform over the tile (+ a border), as in the 100k-point
+            1000 x 1000 m workload;
+  clustered Matern cluster process: parents uniform over land, children
+            uniform in discs of radius U[5, 40] m, 'water' discs covering
+            30 % of the area hold no returns, 1 % of the points are exact xy
+            duplicates of others with a new height (ties).
+Heights everywhere: sum of 4 sinusoids (amplitude 5-50 m, wavelength
+50-500 m) + N(0, 0.1 m) noise.  Coordinates and heights are LAS int32 at
+0.01 m with a UTM-like offset, so the engine's FP64 de-scale is exercised.
+
+Everything is numpy with np.random.default_rng(seed): the same seed gives
+the same tile on the same machine.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import TileParams
+
+LAS_SCALE = (0.01, 0.01, 0.01)
+UTM_OFFSET = (500000.0, 4200000.0, 0.0)
+
+
+@dataclass
+class Tile:
+    params: TileParams
+    X: np.ndarray
+    Y: np.ndarray
+    Z: np.ndarray
+    kind: str = "uniform"
+    seed: int = 0
+
+    @property
+    def n(self) -> int:
+        return int(self.X.shape[0])
+
+    @property
+    def cells(self) -> int:
+        return self.params.nrows * self.params.ncols
+
+
+class HeightField:
+    """z(x, y) = sum_k A_k sin(2 pi (x cos t_k + y sin t_k) / L_k + p_k)."""
+
+    def __init__(self, rng: np.random.Generator):
+        self.A = rng.uniform(5.0, 50.0, 4)
+        self.L = rng.uniform(50.0, 500.0, 4)
+        self.T = rng.uniform(0.0, 2 * np.pi, 4)
+        self.P = rng.uniform(0.0, 2 * np.pi, 4)
+
+    def __call__(self, x: np.ndarray, y: np.ndarray) -> np.ndarray:
+        z = np.zeros_like(x, dtype=np.float64)
+        for k in range(4):
+            z += self.A[k] * np.sin(2 * np.pi * (x * np.cos(self.T[k]) + y * np.sin(self.T[k]))
+                                    / self.L[k] + self.P[k])
+        return z
+
+
+def _to_las(x, y, z, origin):
+    """Tile-local metres -> LAS int32 with offset UTM_OFFSET; the tile's
+    origin sits at UTM_OFFSET + origin."""
+    ox, oy = origin
+    X = np.rint((x + ox) / LAS_SCALE[0]).astype(np.int64)
+    Y = np.rint((y + oy) / LAS_SCALE[1]).astype(np.int64)
+    Z = np.rint(z / LAS_SCALE[2]).astype(np.int64)
+    assert np.abs(X).max(initial=0) < 2**31 and np.abs(Y).max(initial=0) < 2**31
+    return X.astype(np.int32), Y.astype(np.int32), Z.astype(np.int32)
+
+
+def _params(nrows, ncols, cell_size, scale, origin, **kw) -> TileParams:
+    return TileParams(nrows=nrows, ncols=ncols, cell_size=cell_size, scale=scale,
+                      origin_x=UTM_OFFSET[0] + origin[0], origin_y=UTM_OFFSET[1] + origin[1],
+                      las_scale=LAS_SCALE, las_offset=UTM_OFFSET, **kw)
+
+
+def _water_mask(rng, ext, frac, res=2.0):
+    """Union of random discs covering `frac` of [0, ext)^2 on a res-metre raster."""
+    g = max(1, int(np.ceil(ext / res)))
+    mask = np.zeros((g, g), dtype=bool)
+    if frac <= 0:
+        return mask, res
+    yy, xx = np.mgrid[0:g, 0:g]
+    cx_all = (xx + 0.5) * res
+    cy_all = (yy + 0.5) * res
+    while mask.mean() < frac:
+        r = rng.uniform(0.03, 0.12) * ext
+        cx, cy = rng.uniform(0, ext, 2)
+        mask |= (cx_all - cx) ** 2 + (cy_all - cy) ** 2 <= r * r
+    return mask, res
+
+
+def _in_water(mask, res, x, y):
+    g = mask.shape[0]
+    i = np.clip((x / res).astype(np.int64), 0, g - 1)
+    j = np.clip((y / res).astype(np.int64), 0, g - 1)
+    return mask[j, i]
+
+
+def uniform_points(rng, n, lo, hi):
+    x = rng.uniform(lo, hi, n)
+    y = rng.uniform(lo, hi, n)
+    return x, y
+
+
+def clustered_points(rng, n, ext, n_parents, water=0.30, dup=0.01, border=0.0):
+    """Matern cluster process over [-border, ext + border)^2 with water."""
+    mask, res = _water_mask(rng, ext, water)
+    lo, hi = -border, ext + border
+    # parents uniform over land
+    px = np.empty(0)
+    py = np.empty(0)
+    while px.size < n_parents:
+        cx = rng.uniform(lo, hi, 2 * n_parents)
+        cy = rng.uniform(lo, hi, 2 * n_parents)
+        land = ~_in_water(mask, res, cx, cy)
+        px = np.concatenate([px, cx[land]])
+        py = np.concatenate([py, cy[land]])
+    px, py = px[:n_parents], py[:n_parents]
+    pr = rng.uniform(5.0, 40.0, n_parents)
+    n_main = n - int(round(dup * n))
+    xs, ys = [], []
+    have = 0
+    while have < n_main:
+        m = int((n_main - have) * 1.5) + 16
+        k = rng.integers(0, n_parents, m)
+        rr = pr[k] * np.sqrt(rng.uniform(0, 1, m))
+        th = rng.uniform(0, 2 * np.pi, m)
+        x = px[k] + rr * np.cos(th)
+        y = py[k] + rr * np.sin(th)
+        ok = (x >= lo) & (x < hi) & (y >= lo) & (y < hi) & ~_in_water(mask, res, x, y)
+        xs.append(x[ok])
+        ys.append(y[ok])
+        have += int(ok.sum())
+    x = np.concatenate(xs)[:n_main]
+    y = np.concatenate(ys)[:n_main]
+    nd = n - n_main
+    if nd > 0 and n_main > 0:
+        src = rng.integers(0, n_main, nd)
+        x = np.concatenate([x, x[src]])
+        y = np.concatenate([y, y[src]])
+    perm = rng.permutation(x.shape[0])
+    return x[perm], y[perm]
+
+
+def make_tile(seed: int, n: int, grid: int, cell_size: float = 0.5, scale: int = 1,
+              kind: str = "uniform", origin=(0.0, 0.0), border: float | None = None,
+              n_parents: int | None = None, **kw) -> Tile:
+    """A square tile of grid x grid cells with n points (border points included)."""
+    rng = np.random.default_rng(seed)
+    ext = grid * cell_size
+    if border is None:
+        border = 5 * cell_size + 5.0  # the driver's work-package border, terranni.cpp:659
+    field = HeightField(rng)
+    if kind == "uniform":
+        x, y = uniform_points(rng, n, -border, ext + border)
+    elif kind == "clustered":
+        if n_parents is None:
+            n_parents = max(4, int(round(2000 * (ext / 2048.0) ** 2)))
+        x, y = clustered_points(rng, n, ext, n_parents, border=border)
+    else:
+        raise ValueError(kind)
+    z = field(x, y) + rng.normal(0.0, 0.1, x.shape[0])
+    X, Y, Z = _to_las(x, y, z, origin)
+    p = _params(grid, grid, cell_size, scale, origin, **kw)
+    return Tile(p, X, Y, Z, kind=kind, seed=seed)
+
+
+def workload_uniform_100k(seed: int = 0, n: int = 100_000, **kw) -> Tile:
+    """100k points uniform in 1000 x 1000 m, 512 x 512 cells of 2 m, scale 5."""
+    rng = np.random.default_rng(seed)
+    field = HeightField(rng)
+    x, y = uniform_points(rng, n, 0.0, 1000.0)
+    z = field(x, y) + rng.normal(0.0, 0.1, n)
+    X, Y, Z = _to_las(x, y, z, (0.0, 0.0))
+    p = _params(512, 512, 2.0, 5, (0.0, 0.0), **kw)
+    return Tile(p, X, Y, Z, kind="uniform", seed=seed)
+
+
+def workload_single_4m(seed: int = 1, n: int = 4_000_000, grid: int = 4096, **kw) -> Tile:
+    """4M clustered points (2k parents, 30 % water, 1 % duplicates),
+    4096 x 4096 cells of 0.5 m at scale 1."""
+    rng = np.random.default_rng(seed)
+    ext = grid * 0.5
+    field = HeightField(rng)
+    n_parents = max(4, int(round(2000 * (grid / 4096.0) ** 2)))
+    x, y = clustered_points(rng, n, ext, n_parents, border=0.0)
+    z = field(x, y) + rng.normal(0.0, 0.1, x.shape[0])
+    X, Y, Z = _to_las(x, y, z, (0.0, 0.0))
+    p = _params(grid, grid, 0.5, 1, (0.0, 0.0), **kw)
+    return Tile(p, X, Y, Z, kind="clustered", seed=seed)
+
+
+def workload_batch_manifest(seed: int = 3, n_tiles: int = 256, median: float = 2e5,
+                            sigma: float = 1.0, nmin: float = 1e4, nmax: float = 2e6,
+                            grids=(256, 512, 1024, 2048)):
+    """(seed, n, grid, kind) of each uneven tile of the batched workload."""
+    rng = np.random.default_rng(seed)
+    ns = np.clip(rng.lognormal(np.log(median), sigma, n_tiles), nmin, nmax).astype(np.int64)
+    gs = rng.choice(np.asarray(grids), n_tiles)
+    kinds = np.where(rng.uniform(0, 1, n_tiles) < 0.5, "uniform", "clustered")
+    seeds = rng.integers(0, 2**31 - 1, n_tiles)
+    return [dict(seed=int(s), n=int(n), grid=int(g), kind=str(k))
+            for s, n, g, k in zip(seeds, ns, gs, kinds)]
+
+
+def workload_batch(seed: int = 3, n_tiles: int = 256, scale_points: float = 1.0, **kw):
+    """The batched uneven-tile workload: list of Tiles."""
+    man = workload_batch_manifest(seed, n_tiles, **kw)
+    tiles = []
+    for k, m in enumerate(man):
+        n = max(1, int(m["n"] * scale_points))
+        tiles.append(make_tile(m["seed"], n, m["grid"], kind=m["kind"],
+                               origin=(float(k % 16) * 1100.0, float(k // 16) * 1100.0)))
+    return tiles

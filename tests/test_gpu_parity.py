@@ -1,0 +1,100 @@
+"""GPU parity: labels bit-exact, heights bit-exact to the oracle's engine
+formula and within 1e-5 relative of its FP64 mean, on seeded tiles."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import terranni_amd as T
+import terranni_amd.synth as S
+from helpers import check_tile
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("seed,n,grid,cs,scale,kind", [
+    (1, 500, 32, 0.5, 1, "uniform"),
+    (2, 3000, 64, 0.5, 1, "clustered"),
+    (3, 2000, 40, 2.0, 5, "uniform"),
+    (4, 800, 33, 1.0, 2, "uniform"),
+    (5, 20000, 128, 0.5, 1, "clustered"),
+    (6, 5000, 50, 2.0, 3, "clustered"),
+])
+def test_small_tiles(ctx, seed, n, grid, cs, scale, kind):
+    _say("small tile", seed, n, grid, cs, scale, kind)
+    t = S.make_tile(seed, n, grid, cell_size=cs, scale=scale, kind=kind)
+    check_tile(ctx, t.params, t.X, t.Y, t.Z)
+
+
+@pytest.mark.parametrize("mode", [T.ACC_FIXED64, T.ACC_FP32_2SUM, T.ACC_S0])
+@pytest.mark.parametrize("alg", [T.ALG_PLANES, T.ALG_CONES])
+def test_modes(ctx, mode, alg):
+    _say("modes", mode, alg)
+    t = S.make_tile(11, 6000, 96, cell_size=0.5, scale=1, kind="clustered")
+    p = dataclasses.replace(t.params, accum_mode=mode, algorithm=alg, point_min=-300,
+                            multiplier=10000)
+    check_tile(ctx, p, t.X, t.Y, t.Z, s0=(mode == T.ACC_S0))
+
+
+def test_uniform_100k_512(ctx):
+    _say("uniform 100k, 512^2 at 2 m, scale 5")
+    t = S.workload_uniform_100k()
+    check_tile(ctx, t.params, t.X, t.Y, t.Z)
+
+
+@pytest.mark.parametrize("bucket", [0, 7, 50, 333, 2000])
+def test_bucket_size_does_not_change_results(ctx, bucket):
+    _say("bucket", bucket)
+    t = S.make_tile(21, 8000, 80, cell_size=0.5, scale=1, kind="clustered")
+    p = dataclasses.replace(t.params, bucket_units=bucket)
+    check_tile(ctx, p, t.X, t.Y, t.Z)
+
+
+@pytest.mark.parametrize("tc", [1, 3, 16, 64])
+def test_cell_tile_size_does_not_change_results(ctx, tc):
+    _say("tile cells", tc)
+    t = S.make_tile(22, 4000, 70, cell_size=2.0, scale=1, kind="uniform")
+    p = dataclasses.replace(t.params, tile_cells=tc)
+    check_tile(ctx, p, t.X, t.Y, t.Z)
+
+
+def test_empty_tile(ctx):
+    _say("empty tile")
+    t = S.make_tile(1, 10, 16)
+    e = np.zeros(0, np.int32)
+    h, lab, d2 = ctx.tile_run(t.params, e, e, e, labels=True)
+    assert (h == T.NODATA).all()
+    assert (lab == -1).all() and (d2 == T.NO_D2).all()
+
+
+def test_points_outside_tile(ctx):
+    _say("points far outside")
+    t = S.make_tile(1, 1000, 16)
+    X = t.X + 10**7
+    h, lab, _ = ctx.tile_run(t.params, X, t.Y, t.Z, labels=True)
+    assert (h == T.NODATA).all() and (lab == -1).all()
+
+
+def test_single_site(ctx):
+    _say("single site")
+    t = S.make_tile(1, 1, 32, cell_size=0.5, scale=1)
+    h, ref = check_tile(ctx, t.params, t.X, t.Y, t.Z)
+    valid = h != T.NODATA
+    assert valid.any()
+    z = t.Z[0] * 0.01
+    assert np.allclose(h[valid], z, rtol=1e-6)
+
+
+def test_exact_duplicates_take_lowest_index(ctx):
+    _say("duplicates")
+    t = S.make_tile(3, 400, 24, cell_size=0.5, scale=1)
+    X = np.concatenate([t.X, t.X[::-1]])
+    Y = np.concatenate([t.Y, t.Y[::-1]])
+    Z = np.concatenate([t.Z, t.Z[::-1] + 100])
+    h, lab, _ = ctx.tile_run(t.params, X, Y, Z, labels=True)
+    assert lab.max() < t.n  # the originals (lower indices) always win
+    check_tile(ctx, t.params, X, Y, Z)
