@@ -88,7 +88,7 @@ typedef struct tnni_tile_desc {
     int pixel_offset;           /* -1 = the formula of nni.cpp:883         */
     int point_min;              /* Settings::pointMin   (TNNI_ACC_S0 only) */
     int multiplier;             /* Settings::multiplier (TNNI_ACC_S0 only) */
-    int bucket_units;           /* bucket side in units, 0 = tuned         */
+    int bucket_px;              /* bucket side in pixels, 0 = tuned        */
     int tile_cells;             /* cell-tile side per workgroup, 0 = auto  */
 } tnni_tile_desc;
 

@@ -50,7 +50,7 @@ class TileDesc(ctypes.Structure):
         ("algorithm", ctypes.c_int), ("accum_mode", ctypes.c_int),
         ("units_per_m", ctypes.c_int), ("pixel_offset", ctypes.c_int),
         ("point_min", ctypes.c_int), ("multiplier", ctypes.c_int),
-        ("bucket_units", ctypes.c_int), ("tile_cells", ctypes.c_int),
+        ("bucket_px", ctypes.c_int), ("tile_cells", ctypes.c_int),
     ]
 
 
@@ -95,7 +95,7 @@ class TileParams:
     pixel_offset: int = -1
     point_min: int = 0
     multiplier: int = 1
-    bucket_units: int = 0
+    bucket_px: int = 0
     tile_cells: int = 0
 
     def qr(self) -> float:
@@ -120,7 +120,7 @@ class TileParams:
         d.pixel_offset = self.pixel_offset
         d.point_min = self.point_min
         d.multiplier = self.multiplier
-        d.bucket_units = self.bucket_units
+        d.bucket_px = self.bucket_px
         d.tile_cells = self.tile_cells
         return d
 

@@ -158,29 +158,34 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
     t.hi_x = (double)f.x0 + (double)(f.W - 1) * f.ps_u + f.Rsb;
     t.lo_y = (double)f.y0 - f.Rsb;
     t.hi_y = (double)f.y0 + (double)(f.H - 1) * f.ps_u + f.Rsb;
-    // bucket grid over that window
-    const int64_t ex = (int64_t)(f.W - 1) * f.ps_u + 2 * (int64_t)f.Rsb + 1;
-    const int64_t ey = (int64_t)(f.H - 1) * f.ps_u + 2 * (int64_t)f.Rsb + 1;
-    int64_t bs = d.bucket_units;
-    if (bs <= 0) {
-        // about three sites per bucket at the tile's mean density
-        const double area = (double)ex * (double)ey;
-        bs = n ? (int64_t)std::ceil(std::sqrt(3.0 * area / (double)n)) : std::max(ex, ey);
-        bs = std::max<int64_t>(bs, std::max(1, f.ps_u / 4));
+    // Bucket grid over that window: square buckets of bm x bm pixels, edges
+    // on pixel edges, so pixel k sits in bucket column bK + k / bm.  bm is
+    // set for about one site per bucket at the tile's mean density.
+    int bm = d.bucket_px;
+    if (bm <= 0) {
+        const double per_px = (double)n / ((double)f.W * (double)f.H);
+        bm = per_px > 0 ? (int)std::lround(std::sqrt(1.0 / per_px)) : 64;
     }
-    bs = std::max<int64_t>(bs, 1);
-    bs = std::min<int64_t>(bs, std::max(ex, ey));
-    int64_t nbx = (ex + bs - 1) / bs, nby = (ey + bs - 1) / bs;
-    while (nbx * nby > (int64_t)1 << 28) {  // cap the bucket table
-        bs *= 2;
-        nbx = (ex + bs - 1) / bs;
-        nby = (ey + bs - 1) / bs;
+    bm = std::max(1, std::min(bm, f.Rsb / f.ps_u + 1));
+    const int64_t bs = (int64_t)bm * f.ps_u;
+    const int64_t half = f.ps_u / 2;
+    const int64_t K = f.Rsb > half ? (f.Rsb - half + bs - 1) / bs : 0;
+    const int64_t gx0 = (int64_t)f.x0 - half - K * bs;
+    const int64_t last_x = (int64_t)f.x0 + (int64_t)(f.W - 1) * f.ps_u + f.Rsb;
+    const int64_t last_y = (int64_t)f.y0 + (int64_t)(f.H - 1) * f.ps_u + f.Rsb;
+    const int64_t nbx = (last_x - gx0) / bs + 1, nby = (last_y - gx0) / bs + 1;
+    if (nbx * nby > (int64_t)1 << 30) {
+        err = "bucket table too large: raise bucket_px";
+        return TNNI_E_GEOMETRY;
     }
-    t.gx0 = f.x0 - f.Rsb;
-    t.gy0 = f.y0 - f.Rsb;
+    t.gx0 = (int)gx0;
+    t.gy0 = (int)gx0;
     t.bs = (int)bs;
     t.nbx = (int)nbx;
     t.nby = (int)nby;
+    t.bm = bm;
+    t.bK = (int)K;
+    t.ecap = (int)std::min<int64_t>(250, f.Rsb / bs + 2);
     // cell tiles: the largest that fits the LDS budget; the plain form
     // shrinks them so that a lone small tile still spreads over the CUs
     auto lds_for = [&t, &d](int tc) {
@@ -243,7 +248,7 @@ struct tnni_ctx {
     uint64_t allocated = 0;
 
     // pipeline buffers
-    DevBuf tiles, chunks, batch, counts, rank, partials, sorted;
+    DevBuf tiles, chunks, batch, counts, rank, partials, sorted, hdist, edist;
     // staging for the host-pointer API
     DevBuf sx, sy, sz, sh, sl, sd;
     // pinned host mirrors of the descriptors
@@ -321,7 +326,7 @@ tnni_status run_prepared(tnni_ctx *ctx, std::vector<Prepared> &tiles, bool sync)
 {
     if (tiles.empty()) return TNNI_OK;
     const int alg = tiles[0].t.algorithm, mode = tiles[0].t.mode;
-    int64_t pts = 0, bks = 0, items = 0;
+    int64_t pts = 0, bks = 0, items = 0, rows = 0;
     size_t lds = 0;
     std::vector<Chunk> chunks;
     for (size_t k = 0; k < tiles.size(); ++k) {
@@ -331,6 +336,8 @@ tnni_status run_prepared(tnni_ctx *ctx, std::vector<Prepared> &tiles, bool sync)
         t.pt_base = pts;
         t.bk_base = bks;
         t.item_base = items;
+        t.row_base = rows;
+        rows += t.nby;
         for (int64_t b = 0; b < t.n; b += kChunkPts) {
             Chunk c;
             c.tile = (int)k;
@@ -358,6 +365,8 @@ tnni_status run_prepared(tnni_ctx *ctx, std::vector<Prepared> &tiles, bool sync)
     if ((st = ensure(ctx, ctx->rank, (size_t)pts * 4)) != TNNI_OK) return st;
     if ((st = ensure(ctx, ctx->partials, kScanBlocks * 4)) != TNNI_OK) return st;
     if ((st = ensure(ctx, ctx->sorted, (size_t)pts * 16)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, ctx->hdist, (size_t)bks)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, ctx->edist, (size_t)bks)) != TNNI_OK) return st;
 
     BatchDev bd;
     std::memset(&bd, 0, sizeof bd);
@@ -365,6 +374,7 @@ tnni_status run_prepared(tnni_ctx *ctx, std::vector<Prepared> &tiles, bool sync)
     bd.n_chunks = (int)chunks.size();
     bd.n_scan = (uint32_t)(bks + 1);
     bd.n_items = items;
+    bd.n_rows = rows;
     bd.scan_chunk = (uint32_t)((bd.n_scan + kScanBlocks - 1) / kScanBlocks);
 
     // descriptors through one pinned staging block
@@ -391,6 +401,7 @@ tnni_status run_prepared(tnni_ctx *ctx, std::vector<Prepared> &tiles, bool sync)
     LaunchCfg cfg;
     cfg.fill_grid = (int)std::min<int64_t>(1024, (bks + 1 + kBlock - 1) / kBlock);
     cfg.sort_grid = (int)std::min<size_t>(4096, chunks.size());
+    cfg.row_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, rows));
     const int occ = tile_kernel_occupancy(lds);
     cfg.main_grid = (int)std::max<int64_t>(1, std::min<int64_t>(items, (int64_t)ctx->n_cu * occ));
     cfg.main_lds = lds;
@@ -404,6 +415,8 @@ tnni_status run_prepared(tnni_ctx *ctx, std::vector<Prepared> &tiles, bool sync)
     db.rank = (uint32_t *)ctx->rank.p;
     db.partials = (uint32_t *)ctx->partials.p;
     db.sorted = (int4 *)ctx->sorted.p;
+    db.hdist = (uint8_t *)ctx->hdist.p;
+    db.edist = (uint8_t *)ctx->edist.p;
     HIP_TRY(ctx, enqueue_pipeline(db, cfg, ctx->stream, ctx->prof ? ctx->ev0 : nullptr,
                                   ctx->prof ? ctx->ev1 : nullptr));
     if (sync || ctx->prof) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -451,7 +464,7 @@ tnni_status tnni_footprint(const tnni_tile_desc *desc, uint64_t n_points, int wi
     uint64_t b = 0;
     b += n_points * 12;                         // X, Y, Z
     b += n_points * (4 + 16);                   // rank + site records
-    b += ((uint64_t)t.nbx * t.nby + 1) * 4;     // bucket offsets
+    b += ((uint64_t)t.nbx * t.nby + 1) * 6;     // bucket offsets + distance maps
     b += (uint64_t)t.nrows * t.ncols * 4;       // heights
     if (with_labels) b += (uint64_t)t.W * t.H * 8;  // labels + d2
     b += sizeof(TileDev) + ((n_points + kChunkPts - 1) / kChunkPts + 1) * sizeof(Chunk);
@@ -505,7 +518,7 @@ void tnni_ctx_destroy(tnni_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->tiles, &ctx->chunks, &ctx->batch, &ctx->counts, &ctx->rank,
-                      &ctx->partials, &ctx->sorted, &ctx->sx, &ctx->sy, &ctx->sz,
+                      &ctx->partials, &ctx->sorted, &ctx->hdist, &ctx->edist, &ctx->sx, &ctx->sy, &ctx->sz,
                       &ctx->sh, &ctx->sl, &ctx->sd};
     for (DevBuf *b : bufs) release(*b);
     for (void *p : ctx->user_allocs) (void)hipFree(p);

@@ -29,8 +29,14 @@ struct TileDev {
     uint32_t Rs2, Rq2;
     int algorithm, mode, nrows, ncols, scale;
     int point_min, multiplier;
-    int gx0, gy0, bs, nbx, nby; // bucket grid: origin, side, counts
+    // bucket grid: square buckets of bm x bm pixels (side bs units) aligned
+    // to pixel edges; pixel k lies in bucket column bK + k / bm
+    int gx0, gy0, bs, nbx, nby;
+    int bm, bK;
+    int ecap;                   // cap of the empty-distance map
+    int pad1;
     int64_t bk_base;            // first bucket of this tile in the batch
+    int64_t row_base;           // first bucket row of this tile in the batch
     int tc, ntx, nty;           // cell-tile side and tiles along x / y
     int pad0;
     int64_t item_base;          // first work item of this tile
@@ -52,6 +58,7 @@ struct BatchDev {
     uint32_t n_scan;       // buckets of the whole batch + 1
     uint32_t work_counter; // next work item of the persistent kernel
     int64_t n_items;
+    int64_t n_rows;        // bucket rows of the whole batch
     uint32_t scan_chunk;   // elements per scan block
     uint32_t pad;
 };
@@ -107,6 +114,7 @@ struct LaunchCfg {
     int fill_grid;      // blocks of the zero pass
     int sort_grid;      // blocks of the count/scatter passes
     int main_grid;      // blocks of the persistent tile kernel
+    int row_grid;       // blocks of the empty-distance passes
     size_t main_lds;    // dynamic LDS bytes of the tile kernel
     int algorithm;      // TNNI_ALG_* of every tile in the batch
     int mode;           // TNNI_ACC_* of every tile in the batch
@@ -120,6 +128,8 @@ struct DevBuffers {
     uint32_t *rank;     // one per point
     uint32_t *partials; // kScanBlocks
     int4 *sorted;       // one per point
+    uint8_t *hdist;     // per bucket: distance to a non-empty bucket in its row
+    uint8_t *edist;     // per bucket: Chebyshev distance to a non-empty bucket
 };
 
 // Enqueue the whole pipeline on `stream`.  ev_main0/ev_main1 (nullable)

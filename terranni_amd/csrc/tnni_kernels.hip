@@ -175,6 +175,70 @@ k_scan_down(const BatchDev *b, uint32_t *counts, const uint32_t *partials)
     }
 }
 
+// ---- empty-distance map ----------------------------------------------------
+// edist[b] = min(ecap, Chebyshev distance in buckets from b to the nearest
+// bucket holding a site), computed separably: first along rows (hdist), then
+// along columns.
+
+__device__ __forceinline__ int tile_of_row(const TileDev *tiles, int n_tiles, int64_t row)
+{
+    int lo = 0, hi = n_tiles - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tiles[mid].row_base <= row) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_edist_rows(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
+             uint8_t *__restrict__ hdist)
+{
+    const int64_t nr = b->n_rows;
+    const int n_tiles = b->n_tiles;
+    for (int64_t item = blockIdx.x; item < nr; item += gridDim.x) {
+        const TileDev &t = tiles[tile_of_row(tiles, n_tiles, item)];
+        const int64_t rb = t.bk_base + (item - t.row_base) * t.nbx;
+        const uint32_t *o = offs + rb;
+        const int nbx = t.nbx, cap = t.ecap;
+        for (int x = threadIdx.x; x < nbx; x += kBlock) {
+            int d = cap;
+            for (int q = 0; q < cap; ++q) {
+                const bool l = x - q >= 0 && o[x - q + 1] > o[x - q];
+                const bool r = x + q < nbx && o[x + q + 1] > o[x + q];
+                if (l || r) {
+                    d = q;
+                    break;
+                }
+            }
+            hdist[rb + x] = (uint8_t)d;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_edist_cols(const BatchDev *b, const TileDev *tiles, const uint8_t *__restrict__ hdist,
+             uint8_t *__restrict__ edist)
+{
+    const int64_t nr = b->n_rows;
+    const int n_tiles = b->n_tiles;
+    for (int64_t item = blockIdx.x; item < nr; item += gridDim.x) {
+        const TileDev &t = tiles[tile_of_row(tiles, n_tiles, item)];
+        const int row = (int)(item - t.row_base);
+        const int nbx = t.nbx, nby = t.nby;
+        const uint8_t *h = hdist + t.bk_base;
+        for (int x = threadIdx.x; x < nbx; x += kBlock) {
+            int e = t.ecap;
+            for (int q = 0; q < e; ++q) {
+                if (row - q >= 0) e = min(e, max(q, (int)h[(int64_t)(row - q) * nbx + x]));
+                if (row + q < nby) e = min(e, max(q, (int)h[(int64_t)(row + q) * nbx + x]));
+            }
+            edist[t.bk_base + (int64_t)row * nbx + x] = (uint8_t)e;
+        }
+    }
+}
+
 // ---- the tile kernel -------------------------------------------------------
 
 // Visit the sites stored at [s, e) for the pixel centre (px, py).
@@ -200,23 +264,36 @@ __device__ __forceinline__ void scan_range(uint32_t s, uint32_t e, const int4 *_
     }
 }
 
-// Nearest covering site of one pixel centre: ring search over the bucket
-// grid, stopping when every unvisited site is provably farther than the
-// best so far (strictly, so equal distances still reach the lower index)
-// or lies outside the site region.
+// Nearest covering site of pixel (k, l): ring search over the bucket grid.
+// Rings closer than the bucket's empty distance hold no site and are
+// skipped; the search stops when every unvisited site is provably farther
+// than the best so far (strictly, so an equal distance still reaches the
+// lower index) or lies outside the site region.
 template <int ALG>
 __device__ __forceinline__ void label_pixel(const TileDev &t, const uint32_t *__restrict__ offs,
-                                            const int4 *__restrict__ sorted, int px, int py,
+                                            const uint8_t *__restrict__ edist,
+                                            const int4 *__restrict__ sorted, int k, int l,
                                             uint64_t &best, int &bval)
 {
     best = ~0ull;
     bval = 0;
+    const int px = t.x0 + k * t.ps_u, py = t.y0 + l * t.ps_u;
     const int bs = t.bs, nbx = t.nbx, nby = t.nby;
-    const int bx = (px - t.gx0) / bs;
-    const int by = (py - t.gy0) / bs;
+    const int bx = t.bK + k / t.bm;
+    const int by = t.bK + l / t.bm;
     const uint32_t *base = offs + t.bk_base;
     const int R = (ALG == TNNI_ALG_PLANES) ? t.Rs : t.Rsb;
-    for (int r = 0;; ++r) {
+    // distances from the pixel centre to the edges of its own bucket
+    const int el = px - (t.gx0 + bx * bs) + 1, er = t.gx0 + (bx + 1) * bs - px;
+    const int eb = py - (t.gy0 + by * bs) + 1, et = t.gy0 + (by + 1) * bs - py;
+    const int e0 = min(min(el, er), min(eb, et));
+    for (int r = edist[t.bk_base + (int64_t)by * nbx + bx];; ++r) {
+        if (r > 0) {
+            // every site outside the visited (2r-1)^2 square is at least gap away
+            const int gap = e0 + (r - 1) * bs;
+            if (gap > t.Rsb) break;
+            if ((uint64_t)gap * (uint64_t)gap > (best >> 32)) break;
+        }
         for (int dy = -r; dy <= r; ++dy) {
             const int row = by + dy;
             if (row < 0 || row >= nby) continue;
@@ -229,13 +306,6 @@ __device__ __forceinline__ void label_pixel(const TileDev &t, const uint32_t *__
                 if (bx + r < nbx) scan_range<ALG>(rs[bx + r], rs[bx + r + 1], sorted, px, py, R, t.Rs2, best, bval);
             }
         }
-        const int gl = px - (t.gx0 + (bx - r) * bs) + 1;
-        const int gr = t.gx0 + (bx + r + 1) * bs - px;
-        const int gb = py - (t.gy0 + (by - r) * bs) + 1;
-        const int gt = t.gy0 + (by + r + 1) * bs - py;
-        const int gap = min(min(gl, gr), min(gb, gt));
-        if (gap > t.Rsb) break;
-        if ((uint64_t)gap * (uint64_t)gap > (best >> 32)) break;
     }
 }
 
@@ -294,6 +364,7 @@ struct Acc {
 template <int ALG, int MODE>
 __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, uint32_t *smem,
                                               const uint32_t *__restrict__ offs,
+                                              const uint8_t *__restrict__ edist,
                                               const int4 *__restrict__ sorted)
 {
     const Region g = region_of(t, ti, tj);
@@ -308,10 +379,9 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
         const int k = g.rk0 + (blk % nbx8) * 8 + (lane & 7);
         const int l = g.rl0 + (blk / nbx8) * 8 + (lane >> 3);
         if (k > g.rk1 || l > g.rl1) continue;
-        const int px = t.x0 + k * t.ps_u, py = t.y0 + l * t.ps_u;
         uint64_t best;
         int bval;
-        label_pixel<ALG>(t, offs, sorted, px, py, best, bval);
+        label_pixel<ALG>(t, offs, edist, sorted, k, l, best, bval);
         const uint32_t d2 = (uint32_t)(best >> 32);
         s_d2[(l - g.rl0) * RW + (k - g.rk0)] = d2;
         s_val[(l - g.rl0) * RW + (k - g.rk0)] = bval;
@@ -351,7 +421,7 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
 template <int ALG, int MODE>
 __global__ void __launch_bounds__(kBlock)
 k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
-        const int4 *__restrict__ sorted)
+        const uint8_t *__restrict__ edist, const int4 *__restrict__ sorted)
 {
     extern __shared__ uint32_t smem[];
     __shared__ uint32_t s_item;
@@ -372,7 +442,7 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
         const TileDev &t = tiles[lo];
         const int64_t local = item - t.item_base;
         const int tj = (int)(local / t.ntx), ti = (int)(local % t.ntx);
-        process_block<ALG, MODE>(t, ti, tj, smem, offs, sorted);
+        process_block<ALG, MODE>(t, ti, tj, smem, offs, edist, sorted);
         __syncthreads();
     }
 }
@@ -381,7 +451,7 @@ template <int ALG, int MODE>
 hipError_t launch_tiles(const DevBuffers &d, const LaunchCfg &cfg, hipStream_t s)
 {
     hipLaunchKernelGGL((k_tiles<ALG, MODE>), dim3(cfg.main_grid), dim3(kBlock), cfg.main_lds, s,
-                       d.batch, d.tiles, d.counts, d.sorted);
+                       d.batch, d.tiles, d.counts, d.edist, d.sorted);
     return hipGetLastError();
 }
 
@@ -401,6 +471,10 @@ hipError_t enqueue_pipeline(const DevBuffers &d, const LaunchCfg &cfg, hipStream
                        d.partials);
     hipLaunchKernelGGL(k_scatter, dim3(cfg.sort_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
                        d.chunks, d.counts, d.rank, d.sorted);
+    hipLaunchKernelGGL(k_edist_rows, dim3(cfg.row_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
+                       d.counts, d.hdist);
+    hipLaunchKernelGGL(k_edist_cols, dim3(cfg.row_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
+                       d.hdist, d.edist);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
     const int sel = cfg.algorithm * 3 + cfg.mode;

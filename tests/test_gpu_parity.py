@@ -46,11 +46,11 @@ def test_uniform_100k_512(ctx):
     check_tile(ctx, t.params, t.X, t.Y, t.Z)
 
 
-@pytest.mark.parametrize("bucket", [0, 7, 50, 333, 2000])
+@pytest.mark.parametrize("bucket", [0, 1, 2, 3, 5, 8])
 def test_bucket_size_does_not_change_results(ctx, bucket):
     _say("bucket", bucket)
     t = S.make_tile(21, 8000, 80, cell_size=0.5, scale=1, kind="clustered")
-    p = dataclasses.replace(t.params, bucket_units=bucket)
+    p = dataclasses.replace(t.params, bucket_px=bucket)
     check_tile(ctx, p, t.X, t.Y, t.Z)
 
 
@@ -98,3 +98,11 @@ def test_exact_duplicates_take_lowest_index(ctx):
     h, lab, _ = ctx.tile_run(t.params, X, Y, Z, labels=True)
     assert lab.max() < t.n  # the originals (lower indices) always win
     check_tile(ctx, t.params, X, Y, Z)
+
+
+def test_single_4m_tile_labels_and_heights(ctx):
+    """The 4096 x 4096 tile with 4M clustered points, labels emitted."""
+    _say("4M-point 4096^2 tile")
+    t = S.workload_single_4m()
+    h, ref = check_tile(ctx, t.params, t.X, t.Y, t.Z)
+    _say("valid cells", int((h != T.NODATA).sum()), "stolen pairs", ref["S"])
