@@ -49,6 +49,57 @@ def cpu_baseline(tile, budget_s: float = 12.0):
     return tile.cells / per, threads, per, reps, ref
 
 
+def bench_batch(T, S, ctx, reps: int = 3):
+    """The 256 uneven tiles (log-normal point counts, grids 256^2..2048^2,
+    half uniform, half clustered): the plain form (one tile per launch
+    sequence, synchronised) against the batched form (one bucket sort + one
+    persistent launch over all tiles, replayed from a captured graph)."""
+    t0 = time.perf_counter()
+    tiles = S.workload_batch()
+    log(f"batched workload: {len(tiles)} tiles, {sum(t.n for t in tiles)} points, "
+        f"{sum(t.cells for t in tiles)} cells, generated in {time.perf_counter() - t0:.1f}s")
+    dev = [(ctx.upload(t.X), ctx.upload(t.Y), ctx.upload(t.Z),
+            ctx.alloc((t.params.nrows, t.params.ncols), np.float32)) for t in tiles]
+    cells = sum(t.cells for t in tiles)
+    pts = sum(t.n for t in tiles)
+    # plain: one tile per launch sequence, in manifest order
+    for t, d in zip(tiles[:8], dev[:8]):
+        ctx.tile_run_dev(t.params, d[0], d[1], d[2], t.n, d[3])
+    lat = []
+    s0 = time.perf_counter()
+    for t, d in zip(tiles, dev):
+        a = time.perf_counter()
+        ctx.tile_run_dev(t.params, d[0], d[1], d[2], t.n, d[3])
+        lat.append(time.perf_counter() - a)
+    plain_s = time.perf_counter() - s0
+    plain_out = [d[3].to_host() for d in dev]
+    # batched: all tiles in one graph-replayed pipeline
+    args = ([t.params for t in tiles], [d[0] for d in dev], [d[1] for d in dev],
+            [d[2] for d in dev], [t.n for t in tiles], [d[3] for d in dev])
+    ctx.batch_run_dev(*args)
+    s0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.batch_run_dev(*args)
+    batch_s = (time.perf_counter() - s0) / reps
+    same = all(np.array_equal(po.view(np.uint32), d[3].to_host().view(np.uint32))
+               for po, d in zip(plain_out, dev))
+    for d in dev:
+        for a in d:
+            a.free()
+    return {
+        "workload": "256 uneven tiles, log-normal points (median 200k, 10k-2M), grids "
+                    "256^2-2048^2 at 0.5 m, half uniform / half clustered",
+        "tiles": len(tiles), "points": pts, "cells": cells,
+        "plain_cells_per_s": cells / plain_s, "plain_points_per_s": pts / plain_s,
+        "plain_tile_latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                                  "p99": float(np.percentile(lat, 99) * 1e3)},
+        "batched_cells_per_s": cells / batch_s, "batched_points_per_s": pts / batch_s,
+        "batched_ms": batch_s * 1e3,
+        "speedup": plain_s / batch_s,
+        "batched_equals_plain": bool(same),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +108,7 @@ def main():
     ap.add_argument("--grid", type=int, default=4096)
     ap.add_argument("--points", type=int, default=4_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); tiles are "
@@ -119,6 +171,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_tiles", "kernel_ms": k_ms, "alg_bytes": alg_bytes},
     }
+    if not args.no_batch:
+        for d in (dX, dY, dZ, dH):
+            d.free()
+        out["batched_vs_plain"] = bench_batch(T, S, ctx)
     if not args.no_cpu:
         c_cells, threads, per, reps, ref = cpu_baseline(tile)
         out["cpu_baseline"] = {"value": c_cells, "unit": "cells/s", "cores": threads,

@@ -165,6 +165,33 @@ tnni_status tnni_batch_run_dev(tnni_ctx *ctx, int n_tiles,
                                int32_t *const *d_out_labels,
                                uint32_t *const *d_out_d2);
 
+/* ---- served form: continuous batching ---------------------------------- */
+/* Queue one tile (host buffers, which must stay valid until tnni_wait
+ * returns).  A scheduler thread admits queued tiles as batch slots and HBM
+ * free up (admission by tnni_footprint against the context budget), runs
+ * each admitted batch as one graph-replayed pipeline on its own stream, so
+ * that batches overlap on the device, and copies results back.  The result
+ * of a tile does not depend on which batch it ran in.  Fails at once with
+ * TNNI_E_BUDGET if the tile alone exceeds the budget. */
+tnni_status tnni_submit(tnni_ctx *ctx, const tnni_tile_desc *desc,
+                        const int32_t *X, const int32_t *Y, const int32_t *Z,
+                        uint64_t n, float *out_h, int32_t *out_labels,
+                        uint32_t *out_d2, uint64_t *ticket);
+/* TNNI_OK when the result is in host memory, TNNI_PENDING while queued or
+ * running, or the request's error. */
+tnni_status tnni_poll(tnni_ctx *ctx, uint64_t ticket);
+/* Block until done; returns the request's status and releases the ticket.
+ * latency_s (nullable): submit -> results in host memory. */
+tnni_status tnni_wait(tnni_ctx *ctx, uint64_t ticket, double *latency_s);
+
+typedef struct tnni_serve_stats {
+    uint64_t submitted, completed, batches;
+    uint64_t max_inflight_tiles;   /* most tiles on the device at once   */
+    uint64_t max_inflight_batches; /* most batches on the device at once */
+    uint64_t inflight_bytes;       /* footprint admitted right now       */
+} tnni_serve_stats;
+tnni_status tnni_serve_stats_read(tnni_ctx *ctx, tnni_serve_stats *out);
+
 /* ---- profiling -------------------------------------------------------- */
 /* When enabled, HIP events bracket every launch of the main tile kernel on
  * the stream it runs on. */

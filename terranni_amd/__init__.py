@@ -76,6 +76,15 @@ class CtxOpts(ctypes.Structure):
     ]
 
 
+class ServeStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "submitted", "completed", "batches", "max_inflight_tiles", "max_inflight_batches",
+        "inflight_bytes")]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 @dataclass
 class TileParams:
     """One work package: nni::Settings + nni::WorkPackage + LAS scale/offset."""
@@ -159,9 +168,15 @@ def lib():
     L.tnni_profile_enable.argtypes = [P, ctypes.c_int]
     L.tnni_profile_read.argtypes = [P, ctypes.POINTER(ctypes.c_uint64),
                                     ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    L.tnni_submit.argtypes = [P, ctypes.POINTER(TileDesc), P, P, P, ctypes.c_uint64, P, P, P,
+                              ctypes.POINTER(ctypes.c_uint64)]
+    L.tnni_poll.argtypes = [P, ctypes.c_uint64]
+    L.tnni_wait.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+    L.tnni_serve_stats_read.argtypes = [P, ctypes.POINTER(ServeStats)]
     for name in ("tnni_frame", "tnni_footprint", "tnni_ctx_create", "tnni_tile_run",
                  "tnni_tile_run_dev", "tnni_batch_run_dev", "tnni_dev_alloc", "tnni_dev_free",
-                 "tnni_h2d", "tnni_d2h", "tnni_profile_enable", "tnni_profile_read"):
+                 "tnni_h2d", "tnni_d2h", "tnni_profile_enable", "tnni_profile_read",
+                 "tnni_submit", "tnni_poll", "tnni_wait", "tnni_serve_stats_read"):
         getattr(L, name).restype = ctypes.c_int
     del i32p
     _lib = L
@@ -299,6 +314,28 @@ class Context:
             arr(d_labs) if d_labs is not None else None,
             arr(d_d2s) if d_d2s is not None else None))
 
+    # ---- served form ----
+    def submit(self, p: TileParams, X, Y, Z, labels: bool = False):
+        """Queue a tile; returns a Pending whose arrays are filled once
+        wait() returns.  The input arrays are kept alive until then."""
+        X = np.ascontiguousarray(X, dtype=np.int32)
+        Y = np.ascontiguousarray(Y, dtype=np.int32)
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        fr = frame(p)
+        h = np.empty((p.nrows, p.ncols), dtype=np.float32)
+        lab = np.empty((fr["H"], fr["W"]), dtype=np.int32) if labels else None
+        d2 = np.empty((fr["H"], fr["W"]), dtype=np.uint32) if labels else None
+        d = p.to_desc()
+        t = ctypes.c_uint64()
+        self._chk(lib().tnni_submit(self.h, ctypes.byref(d), _ptr(X), _ptr(Y), _ptr(Z), X.shape[0],
+                                    _ptr(h), _ptr(lab), _ptr(d2), ctypes.byref(t)))
+        return Pending(self, t.value, (X, Y, Z), h, lab, d2)
+
+    def serve_stats(self) -> dict:
+        s = ServeStats()
+        self._chk(lib().tnni_serve_stats_read(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
     # ---- profiling ----
     def profile(self, enable: bool):
         self._chk(lib().tnni_profile_enable(self.h, int(enable)))
@@ -308,3 +345,31 @@ class Context:
         ms = ctypes.c_double()
         self._chk(lib().tnni_profile_read(self.h, ctypes.byref(n), ctypes.byref(ms), int(reset)))
         return n.value, ms.value
+
+
+@dataclass
+class Pending:
+    """A submitted tile (tnni_submit ticket)."""
+    ctx: Context
+    ticket: int
+    inputs: tuple
+    h: np.ndarray
+    labels: np.ndarray | None
+    d2: np.ndarray | None
+    latency_s: float | None = None
+
+    def done(self) -> bool:
+        st = lib().tnni_poll(self.ctx.h, self.ticket)
+        if st == 1:
+            return False
+        if st != 0:
+            self.ctx._chk(st)
+        return True
+
+    def wait(self):
+        lat = ctypes.c_double()
+        st = lib().tnni_wait(self.ctx.h, self.ticket, ctypes.byref(lat))
+        self.latency_s = lat.value
+        self.inputs = ()
+        self.ctx._chk(st)
+        return self.h, self.labels, self.d2

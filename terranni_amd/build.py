@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["tnni_kernels.hip", "tnni_engine.hip"]
+HIP_SOURCES = ["tnni_kernels.hip", "tnni_geometry.hip", "tnni_engine.hip"]
 # -ffp-contract=off: every FP64 de-scale step rounds as written, so the
 # device and the CPU oracle agree bit for bit.  The packed-fp32 feature is
 # switched off for all device code.
@@ -40,7 +40,7 @@ def build_lib(verbose: bool = False) -> str:
     tmp = out + ".tmp"
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
     _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-          *srcs, "-o", tmp], verbose)
+          *srcs, "-o", tmp, "-lpthread"], verbose)
     os.replace(tmp, out)
     return out
 

@@ -1,11 +1,20 @@
-// Host side of the tile engine: context, geometry, arenas, the plain and
-// batched forms, and the C ABI of include/terranni.h.
+// Host side of the tile engine: context, device arenas, the plain, batched
+// and served forms, and the C ABI of include/terranni.h.
+//
+// An Arena is one stream with its own pipeline buffers.  The context owns a
+// main arena for the synchronous calls and, once tiles are submitted, a few
+// slot arenas used by the serving scheduler so that batches overlap.
 #include <algorithm>
-#include <cmath>
-#include <cstdio>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "tnni_internal.h"
@@ -16,224 +25,73 @@ namespace {
 
 thread_local std::string g_thread_err;
 
-// ---- geometry ------------------------------------------------------------
-
-int64_t isqrt64(int64_t v)
+double now_s()
 {
-    if (v <= 0) return 0;
-    int64_t r = (int64_t)std::sqrt((double)v);
-    while (r * r > v) --r;
-    while ((r + 1) * (r + 1) <= v) ++r;
-    return r;
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
-
-// The pixel frame of nni::performNNI (nni.cpp:879-886) and setupRun's
-// glOrtho (nni.cpp:656-660), put on an integer grid of 1/upm metre.
-tnni_status compute_frame(const tnni_tile_desc &d, tnni_frame_info &f, std::string &err)
-{
-    std::memset(&f, 0, sizeof f);
-    if (d.algorithm != TNNI_ALG_PLANES && d.algorithm != TNNI_ALG_CONES) {
-        err = "algorithm must be 0 (planes) or 1 (cones)";
-        return TNNI_E_INVALID;
-    }
-    if (d.scale < 1 || d.nrows < 1 || d.ncols < 1 || !(d.cell_size > 0.0f) ||
-        !(d.site_radius > 0.0f) || !(d.query_radius > 0.0f)) {
-        err = "scale, nrows, ncols, cell_size, site_radius and query_radius must be positive";
-        return TNNI_E_INVALID;
-    }
-    const double cs = (double)d.cell_size;
-    const double qr = (double)d.query_radius;
-    const double sr = (double)d.site_radius;
-    const double ps = cs / d.scale;  // nni.cpp:881
-    int po = d.pixel_offset >= 0 ? d.pixel_offset
-                                 : (int)std::ceil((qr - std::floor(d.scale / 2.0) * ps) / ps) + 1;  // nni.cpp:883
-    if (po < 0) po = 0;
-    const int64_t W = (int64_t)d.ncols * d.scale + 2 * (int64_t)po;  // nni.cpp:885-886
-    const int64_t H = (int64_t)d.nrows * d.scale + 2 * (int64_t)po;
-    if (W > (1 << 20) || H > (1 << 20)) {
-        err = "raster larger than 2^20 pixels on a side";
-        return TNNI_E_GEOMETRY;
-    }
-    f.po = po;
-    f.W = (int)W;
-    f.H = (int)H;
-    f.algorithm = d.algorithm;
-    f.spacing_factor = (int)std::ceil((qr * 2 + 0.001) / (cs * 5));  // nni.cpp:879
-    // the smallest unit of 1/(100 k) metre that puts pixel centres, queries
-    // and cell edges on integers
-    for (int t = 1; t <= 200; ++t) {
-        int upm = 100 * t;
-        if (d.units_per_m > 0) {
-            if (t > 1) break;
-            upm = d.units_per_m;
-        }
-        const double c = cs * upm;
-        const int64_t ci = std::llround(c);
-        if (ci <= 0 || std::fabs(c - (double)ci) > 1e-3 || ci % d.scale) continue;
-        const int64_t ps_u = ci / d.scale;
-        const int64_t x0n = (int64_t)(1 - 2 * po - d.scale) * ps_u;  // 2 * centre of pixel 0
-        if (x0n % 2) continue;
-        const int64_t Rs = (int64_t)std::floor(sr * upm / std::sqrt(2.0));  // nni.cpp:183
-        const int64_t Rq = (int64_t)std::floor(qr * upm / std::sqrt(2.0));
-        const double srd = sr * upm, qrd = qr * upm;
-        const int64_t Rs2 = (int64_t)std::floor(srd * srd);
-        const int64_t Rq2 = (int64_t)std::floor(qrd * qrd);
-        const int64_t Rsb = d.algorithm == TNNI_ALG_PLANES ? Rs : isqrt64(Rs2);
-        const int64_t Rqb = d.algorithm == TNNI_ALG_PLANES ? Rq : isqrt64(Rq2);
-        if (d.algorithm == TNNI_ALG_PLANES ? (2 * Rs * Rs >= (int64_t)kNoD2) : (Rs2 >= (int64_t)kNoD2))
-            continue;
-        const int64_t ext = std::max(W, H) * ps_u + 2 * Rsb + 2 * Rqb;
-        if (ext >= ((int64_t)1 << 30)) continue;
-        f.upm = upm;
-        f.ps_u = (int)ps_u;
-        f.cs_u = (int)ci;
-        f.x0 = f.y0 = (int)(x0n / 2);
-        f.Rs = (int)Rs;
-        f.Rq = (int)Rq;
-        f.Rs2 = Rs2;
-        f.Rq2 = Rq2;
-        f.Rsb = (int)Rsb;
-        f.Rqb = (int)Rqb;
-        return TNNI_OK;
-    }
-    err = "cell size / scale / radii cannot be put on an exact fixed-point grid";
-    return TNNI_E_GEOMETRY;
-}
-
-constexpr size_t kLdsBudget = 60 * 1024;  // two workgroups per CU, under the 64 KB per-group limit
-
-// Fill everything of a TileDev that depends on the descriptor and the
-// point count (not on the batch layout).  lds_out: bytes of LDS it needs.
-tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool batched, TileDev &t,
-                         size_t &lds_out, std::string &err)
-{
-    tnni_frame_info f;
-    tnni_status st = compute_frame(d, f, err);
-    if (st != TNNI_OK) return st;
-    if (d.accum_mode < TNNI_ACC_FIXED64 || d.accum_mode > TNNI_ACC_S0) {
-        err = "accum_mode must be 0, 1 or 2";
-        return TNNI_E_INVALID;
-    }
-    if (d.accum_mode == TNNI_ACC_S0 && d.multiplier < 1) {
-        err = "TNNI_ACC_S0 needs multiplier >= 1";
-        return TNNI_E_INVALID;
-    }
-    if (n > 0x7FFFFFFFull) {
-        err = "more than 2^31-1 points in one tile";
-        return TNNI_E_INVALID;
-    }
-    std::memset(&t, 0, sizeof t);
-    t.n = (int64_t)n;
-    t.sx = d.las_scale[0];
-    t.sy = d.las_scale[1];
-    t.sz = d.las_scale[2];
-    t.ox = d.las_offset[0];
-    t.oy = d.las_offset[1];
-    t.oz = d.las_offset[2];
-    t.origin_x = d.origin_x;
-    t.origin_y = d.origin_y;
-    t.upm = f.upm;
-    t.ps_u = f.ps_u;
-    t.cs_u = f.cs_u;
-    t.po = f.po;
-    t.W = f.W;
-    t.H = f.H;
-    t.x0 = f.x0;
-    t.y0 = f.y0;
-    t.Rs = f.Rs;
-    t.Rq = f.Rq;
-    t.Rsb = f.Rsb;
-    t.Rqb = f.Rqb;
-    t.Rs2 = (uint32_t)std::min<int64_t>(f.Rs2, 0xFFFFFFFFll);
-    t.Rq2 = (uint32_t)std::min<int64_t>(f.Rq2, 0xFFFFFFFFll);
-    t.algorithm = d.algorithm;
-    t.mode = d.accum_mode;
-    t.nrows = d.nrows;
-    t.ncols = d.ncols;
-    t.scale = d.scale;
-    t.point_min = d.point_min;
-    t.multiplier = d.multiplier;
-    // sites further than the site region from every pixel centre are dropped
-    t.lo_x = (double)f.x0 - f.Rsb;
-    t.hi_x = (double)f.x0 + (double)(f.W - 1) * f.ps_u + f.Rsb;
-    t.lo_y = (double)f.y0 - f.Rsb;
-    t.hi_y = (double)f.y0 + (double)(f.H - 1) * f.ps_u + f.Rsb;
-    // Bucket grid over that window: square buckets of bm x bm pixels, edges
-    // on pixel edges, so pixel k sits in bucket column bK + k / bm.  bm is
-    // set for about one site per bucket at the tile's mean density.
-    int bm = d.bucket_px;
-    if (bm <= 0) {
-        const double per_px = (double)n / ((double)f.W * (double)f.H);
-        bm = per_px > 0 ? (int)std::lround(std::sqrt(1.0 / per_px)) : 64;
-    }
-    bm = std::max(1, std::min(bm, f.Rsb / f.ps_u + 1));
-    const int64_t bs = (int64_t)bm * f.ps_u;
-    const int64_t half = f.ps_u / 2;
-    const int64_t K = f.Rsb > half ? (f.Rsb - half + bs - 1) / bs : 0;
-    const int64_t gx0 = (int64_t)f.x0 - half - K * bs;
-    const int64_t last_x = (int64_t)f.x0 + (int64_t)(f.W - 1) * f.ps_u + f.Rsb;
-    const int64_t last_y = (int64_t)f.y0 + (int64_t)(f.H - 1) * f.ps_u + f.Rsb;
-    const int64_t nbx = (last_x - gx0) / bs + 1, nby = (last_y - gx0) / bs + 1;
-    if (nbx * nby > (int64_t)1 << 30) {
-        err = "bucket table too large: raise bucket_px";
-        return TNNI_E_GEOMETRY;
-    }
-    t.gx0 = (int)gx0;
-    t.gy0 = (int)gx0;
-    t.bs = (int)bs;
-    t.nbx = (int)nbx;
-    t.nby = (int)nby;
-    t.bm = bm;
-    t.bK = (int)K;
-    t.ecap = (int)std::min<int64_t>(250, f.Rsb / bs + 2);
-    // cell tiles: the largest that fits the LDS budget; the plain form
-    // shrinks them so that a lone small tile still spreads over the CUs
-    auto lds_for = [&t, &d](int tc) {
-        t.tc = tc;
-        t.ntx = (d.ncols + tc - 1) / tc;
-        t.nty = (d.nrows + tc - 1) / tc;
-        int rw = 0, rh = 0;
-        for (int ti = 0; ti < t.ntx; ++ti) {
-            const Region g = region_of(t, ti, 0);
-            rw = std::max(rw, g.rk1 - g.rk0 + 1);
-        }
-        for (int tj = 0; tj < t.nty; ++tj) {
-            const Region g = region_of(t, 0, tj);
-            rh = std::max(rh, g.rl1 - g.rl0 + 1);
-        }
-        return (size_t)rw * rh * 8;
-    };
-    size_t lds = 0;
-    if (d.tile_cells > 0) {
-        lds = lds_for(d.tile_cells);
-        if (lds > kLdsBudget) {
-            err = "tile_cells too large for the LDS budget";
-            return TNNI_E_GEOMETRY;
-        }
-    } else {
-        static const int kTc[] = {64, 48, 32, 24, 16, 12, 8, 6, 4, 3, 2, 1};
-        int chosen = -1;
-        for (int tc : kTc) {
-            if (lds_for(tc) > kLdsBudget) continue;
-            chosen = tc;
-            if (!batched && tc > 16 && (int64_t)t.ntx * t.nty < 2ll * n_cu) continue;
-            break;
-        }
-        if (chosen < 0) {
-            err = "no cell-tile size fits the LDS budget for this geometry";
-            return TNNI_E_GEOMETRY;
-        }
-        lds = lds_for(chosen);
-    }
-    lds_out = lds;
-    return TNNI_OK;
-}
-
-// ---- device arena --------------------------------------------------------
 
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+};
+
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+// Everything a captured pipeline graph depends on.
+struct GraphKey {
+    const void *ptrs[9];
+    size_t lds;
+    int alg, mode, main_grid;
+    bool operator==(const GraphKey &o) const
+    {
+        return std::memcmp(ptrs, o.ptrs, sizeof ptrs) == 0 && lds == o.lds && alg == o.alg &&
+               mode == o.mode && main_grid == o.main_grid;
+    }
+};
+
+struct Arena {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    DevBuf tiles, chunks, batch, counts, rank, partials, sorted, hdist, edist;
+    DevBuf sx, sy, sz, sh, sl, sd;  // device staging of host-pointer calls
+    HostBuf h_desc;                 // pinned descriptors
+    HostBuf h_in, h_out;            // pinned staging of the served form
+    hipGraphExec_t gexec = nullptr;
+    GraphKey gkey{};
+    uint64_t graph_launches = 0;
+};
+
+// Fixed launch grids of graph-captured pipelines: the kernels read every
+// size from device memory and stride over their work.
+constexpr int kGraphFillGrid = 1024;
+constexpr int kGraphSortGrid = 2048;
+constexpr int kGraphRowGrid = 2048;
+
+struct Request {
+    uint64_t id = 0;
+    const int32_t *X = nullptr, *Y = nullptr, *Z = nullptr;
+    uint64_t n = 0;
+    float *out_h = nullptr;
+    int32_t *out_labels = nullptr;
+    uint32_t *out_d2 = nullptr;
+    TileDev t{};
+    size_t lds = 0;
+    uint64_t footprint = 0;
+    double t_submit = 0, t_done = 0;
+    tnni_status status = TNNI_PENDING;
+    // placement inside its batch's staging
+    size_t in_off = 0, out_h_off = 0, out_l_off = 0, out_d_off = 0;
+};
+
+struct Slot {
+    Arena a;
+    bool busy = false;
+    std::vector<std::shared_ptr<Request>> reqs;
+    uint64_t bytes = 0;
 };
 
 }  // namespace
@@ -241,19 +99,12 @@ struct DevBuf {
 struct tnni_ctx {
     int device = 0;
     int n_cu = 0;
-    hipStream_t stream = nullptr;
-    std::mutex mu;
+    std::mutex mu;  // the synchronous calls and the main arena
     std::string err;
     uint64_t budget = 0;
+    std::mutex mem_mu;
     uint64_t allocated = 0;
-
-    // pipeline buffers
-    DevBuf tiles, chunks, batch, counts, rank, partials, sorted, hdist, edist;
-    // staging for the host-pointer API
-    DevBuf sx, sy, sz, sh, sl, sd;
-    // pinned host mirrors of the descriptors
-    void *h_desc = nullptr;
-    size_t h_desc_cap = 0;
+    Arena main;
 
     bool prof = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -261,6 +112,24 @@ struct tnni_ctx {
     double prof_ms = 0.0;
 
     std::vector<void *> user_allocs;
+
+    // served form
+    int n_slots = 2;
+    int max_batch_tiles = 256;
+    int batch_window_us = 0;
+    std::mutex smu;
+    std::condition_variable scv;   // scheduler wake-up
+    std::condition_variable dcv;   // request completion
+    std::deque<std::shared_ptr<Request>> pending;
+    std::unordered_map<uint64_t, std::shared_ptr<Request>> reqs;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::thread sched;
+    bool sched_started = false;
+    bool shutdown = false;
+    uint64_t next_id = 1;
+    uint64_t serve_budget = 0;
+    uint64_t inflight_bytes = 0;
+    tnni_serve_stats stats{};
 };
 
 namespace {
@@ -278,9 +147,9 @@ tnni_status hip_fail(tnni_ctx *ctx, hipError_t e, const char *what)
     return fail(ctx, e == hipErrorOutOfMemory ? TNNI_E_NOMEM : TNNI_E_HIP, m);
 }
 
-#define HIP_TRY(ctx, expr)                                 \
-    do {                                                   \
-        hipError_t e_ = (expr);                            \
+#define HIP_TRY(ctx, expr)                                     \
+    do {                                                       \
+        hipError_t e_ = (expr);                                \
         if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
     } while (0)
 
@@ -289,6 +158,7 @@ tnni_status ensure(tnni_ctx *ctx, DevBuf &b, size_t bytes)
     if (bytes == 0) bytes = 256;
     if (b.cap >= bytes) return TNNI_OK;
     size_t want = bytes + bytes / 4;
+    std::lock_guard<std::mutex> lk(ctx->mem_mu);
     if (b.p) {
         hipError_t e = hipFree(b.p);
         ctx->allocated -= b.cap;
@@ -307,11 +177,54 @@ tnni_status ensure(tnni_ctx *ctx, DevBuf &b, size_t bytes)
     return TNNI_OK;
 }
 
-void release(DevBuf &b)
+tnni_status ensure_host(tnni_ctx *ctx, HostBuf &b, size_t bytes)
 {
-    if (b.p) (void)hipFree(b.p);
+    if (b.cap >= bytes && b.p) return TNNI_OK;
+    if (b.p) (void)hipHostFree(b.p);
     b.p = nullptr;
     b.cap = 0;
+    const size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
+    HIP_TRY(ctx, hipHostMalloc(&b.p, want, hipHostMallocDefault));
+    b.cap = want;
+    return TNNI_OK;
+}
+
+void release(tnni_ctx *ctx, DevBuf &b)
+{
+    if (b.p) {
+        (void)hipFree(b.p);
+        std::lock_guard<std::mutex> lk(ctx->mem_mu);
+        ctx->allocated -= b.cap;
+    }
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+tnni_status arena_init(tnni_ctx *ctx, Arena &a)
+{
+    HIP_TRY(ctx, hipStreamCreateWithFlags(&a.stream, hipStreamNonBlocking));
+    HIP_TRY(ctx, hipEventCreateWithFlags(&a.done, hipEventDisableTiming));
+    return TNNI_OK;
+}
+
+void arena_free(tnni_ctx *ctx, Arena &a)
+{
+    if (a.stream) (void)hipStreamSynchronize(a.stream);
+    DevBuf *bufs[] = {&a.tiles, &a.chunks, &a.batch, &a.counts, &a.rank, &a.partials, &a.sorted,
+                      &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd};
+    for (DevBuf *b : bufs) release(ctx, *b);
+    HostBuf *hb[] = {&a.h_desc, &a.h_in, &a.h_out};
+    for (HostBuf *b : hb) {
+        if (b->p) (void)hipHostFree(b->p);
+        b->p = nullptr;
+        b->cap = 0;
+    }
+    if (a.gexec) (void)hipGraphExecDestroy(a.gexec);
+    a.gexec = nullptr;
+    if (a.done) (void)hipEventDestroy(a.done);
+    if (a.stream) (void)hipStreamDestroy(a.stream);
+    a.done = nullptr;
+    a.stream = nullptr;
 }
 
 // One prepared tile of a batch, before the batch layout is assigned.
@@ -320,9 +233,13 @@ struct Prepared {
     size_t lds;
 };
 
-// Lay the batch out (bases, chunks, items), upload the descriptors and run
-// the pipeline.  Tiles keep their order; the caller orders them.
-tnni_status run_prepared(tnni_ctx *ctx, std::vector<Prepared> &tiles, bool sync)
+enum class Launch { Direct, Graph };
+
+// Lay the batch out (bases, chunks, items), upload the descriptors and
+// enqueue the pipeline on the arena's stream.  Tiles keep their order; the
+// caller orders them.  sync: wait for the stream before returning.
+tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, bool sync,
+                         Launch how)
 {
     if (tiles.empty()) return TNNI_OK;
     const int alg = tiles[0].t.algorithm, mode = tiles[0].t.mode;
@@ -337,7 +254,6 @@ tnni_status run_prepared(tnni_ctx *ctx, std::vector<Prepared> &tiles, bool sync)
         t.bk_base = bks;
         t.item_base = items;
         t.row_base = rows;
-        rows += t.nby;
         for (int64_t b = 0; b < t.n; b += kChunkPts) {
             Chunk c;
             c.tile = (int)k;
@@ -348,25 +264,23 @@ tnni_status run_prepared(tnni_ctx *ctx, std::vector<Prepared> &tiles, bool sync)
         }
         pts += t.n;
         bks += (int64_t)t.nbx * t.nby;
+        rows += t.nby;
         items += (int64_t)t.ntx * t.nty;
         lds = std::max(lds, tiles[k].lds);
     }
     if (bks + 1 > 0xFFFFFFFFll || pts > 0xFFFFFFFFll)
         return fail(ctx, TNNI_E_INVALID, "batch exceeds 2^32 buckets or points");
-    if (chunks.empty()) {
-        Chunk c{0, 0, 0, 0};
-        chunks.push_back(c);
-    }
+    if (chunks.empty()) chunks.push_back(Chunk{0, 0, 0, 0});
     tnni_status st;
-    if ((st = ensure(ctx, ctx->tiles, tiles.size() * sizeof(TileDev))) != TNNI_OK) return st;
-    if ((st = ensure(ctx, ctx->chunks, chunks.size() * sizeof(Chunk))) != TNNI_OK) return st;
-    if ((st = ensure(ctx, ctx->batch, sizeof(BatchDev))) != TNNI_OK) return st;
-    if ((st = ensure(ctx, ctx->counts, (size_t)(bks + 1) * 4)) != TNNI_OK) return st;
-    if ((st = ensure(ctx, ctx->rank, (size_t)pts * 4)) != TNNI_OK) return st;
-    if ((st = ensure(ctx, ctx->partials, kScanBlocks * 4)) != TNNI_OK) return st;
-    if ((st = ensure(ctx, ctx->sorted, (size_t)pts * 16)) != TNNI_OK) return st;
-    if ((st = ensure(ctx, ctx->hdist, (size_t)bks)) != TNNI_OK) return st;
-    if ((st = ensure(ctx, ctx->edist, (size_t)bks)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.tiles, tiles.size() * sizeof(TileDev))) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.chunks, chunks.size() * sizeof(Chunk))) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.batch, sizeof(BatchDev))) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.counts, (size_t)(bks + 1) * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.rank, (size_t)pts * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.partials, kScanBlocks * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.sorted, (size_t)pts * 16)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.hdist, (size_t)bks)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.edist, (size_t)bks)) != TNNI_OK) return st;
 
     BatchDev bd;
     std::memset(&bd, 0, sizeof bd);
@@ -377,55 +291,316 @@ tnni_status run_prepared(tnni_ctx *ctx, std::vector<Prepared> &tiles, bool sync)
     bd.n_rows = rows;
     bd.scan_chunk = (uint32_t)((bd.n_scan + kScanBlocks - 1) / kScanBlocks);
 
-    // descriptors through one pinned staging block
+    // descriptors through one pinned staging block; it may still feed the
+    // copies of the previous batch on this stream
     const size_t sz_t = tiles.size() * sizeof(TileDev);
     const size_t sz_c = chunks.size() * sizeof(Chunk);
-    const size_t need = sz_t + sz_c + sizeof(BatchDev);
-    if (ctx->h_desc_cap < need) {
-        if (ctx->h_desc) (void)hipHostFree(ctx->h_desc);
-        ctx->h_desc = nullptr;
-        ctx->h_desc_cap = 0;
-        HIP_TRY(ctx, hipHostMalloc(&ctx->h_desc, need * 2, hipHostMallocDefault));
-        ctx->h_desc_cap = need * 2;
-    }
-    // the staging block may still feed a copy of the previous batch
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    char *h = (char *)ctx->h_desc;
+    HIP_TRY(ctx, hipStreamSynchronize(a.stream));
+    if ((st = ensure_host(ctx, a.h_desc, sz_t + sz_c + sizeof(BatchDev))) != TNNI_OK) return st;
+    char *h = (char *)a.h_desc.p;
     for (size_t k = 0; k < tiles.size(); ++k) std::memcpy(h + k * sizeof(TileDev), &tiles[k].t, sizeof(TileDev));
     std::memcpy(h + sz_t, chunks.data(), sz_c);
     std::memcpy(h + sz_t + sz_c, &bd, sizeof bd);
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->tiles.p, h, sz_t, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->chunks.p, h + sz_t, sz_c, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->batch.p, h + sz_t + sz_c, sizeof bd, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(a.tiles.p, h, sz_t, hipMemcpyHostToDevice, a.stream));
+    HIP_TRY(ctx, hipMemcpyAsync(a.chunks.p, h + sz_t, sz_c, hipMemcpyHostToDevice, a.stream));
+    HIP_TRY(ctx, hipMemcpyAsync(a.batch.p, h + sz_t + sz_c, sizeof bd, hipMemcpyHostToDevice, a.stream));
 
     LaunchCfg cfg;
-    cfg.fill_grid = (int)std::min<int64_t>(1024, (bks + 1 + kBlock - 1) / kBlock);
-    cfg.sort_grid = (int)std::min<size_t>(4096, chunks.size());
-    cfg.row_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, rows));
     const int occ = tile_kernel_occupancy(lds);
-    cfg.main_grid = (int)std::max<int64_t>(1, std::min<int64_t>(items, (int64_t)ctx->n_cu * occ));
     cfg.main_lds = lds;
     cfg.algorithm = alg;
     cfg.mode = mode;
     DevBuffers db;
-    db.batch = (BatchDev *)ctx->batch.p;
-    db.tiles = (const TileDev *)ctx->tiles.p;
-    db.chunks = (const Chunk *)ctx->chunks.p;
-    db.counts = (uint32_t *)ctx->counts.p;
-    db.rank = (uint32_t *)ctx->rank.p;
-    db.partials = (uint32_t *)ctx->partials.p;
-    db.sorted = (int4 *)ctx->sorted.p;
-    db.hdist = (uint8_t *)ctx->hdist.p;
-    db.edist = (uint8_t *)ctx->edist.p;
-    HIP_TRY(ctx, enqueue_pipeline(db, cfg, ctx->stream, ctx->prof ? ctx->ev0 : nullptr,
-                                  ctx->prof ? ctx->ev1 : nullptr));
-    if (sync || ctx->prof) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (ctx->prof) {
+    db.batch = (BatchDev *)a.batch.p;
+    db.tiles = (const TileDev *)a.tiles.p;
+    db.chunks = (const Chunk *)a.chunks.p;
+    db.counts = (uint32_t *)a.counts.p;
+    db.rank = (uint32_t *)a.rank.p;
+    db.partials = (uint32_t *)a.partials.p;
+    db.sorted = (int4 *)a.sorted.p;
+    db.hdist = (uint8_t *)a.hdist.p;
+    db.edist = (uint8_t *)a.edist.p;
+    const bool prof = ctx->prof && how == Launch::Direct;
+    if (how == Launch::Direct) {
+        cfg.fill_grid = (int)std::min<int64_t>(1024, (bks + 1 + kBlock - 1) / kBlock);
+        cfg.sort_grid = (int)std::min<size_t>(4096, chunks.size());
+        cfg.row_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, rows));
+        cfg.main_grid = (int)std::max<int64_t>(1, std::min<int64_t>(items, (int64_t)ctx->n_cu * occ));
+        HIP_TRY(ctx, enqueue_pipeline(db, cfg, a.stream, prof ? ctx->ev0 : nullptr,
+                                      prof ? ctx->ev1 : nullptr));
+    } else {
+        cfg.fill_grid = kGraphFillGrid;
+        cfg.sort_grid = kGraphSortGrid;
+        cfg.row_grid = kGraphRowGrid;
+        cfg.main_grid = ctx->n_cu * occ;
+        GraphKey key;
+        const void *p[9] = {a.batch.p, a.tiles.p, a.chunks.p, a.counts.p, a.rank.p,
+                            a.partials.p, a.sorted.p, a.hdist.p, a.edist.p};
+        std::memcpy(key.ptrs, p, sizeof p);
+        key.lds = lds;
+        key.alg = alg;
+        key.mode = mode;
+        key.main_grid = cfg.main_grid;
+        if (!a.gexec || !(key == a.gkey)) {
+            if (a.gexec) HIP_TRY(ctx, hipGraphExecDestroy(a.gexec));
+            a.gexec = nullptr;
+            hipGraph_t g = nullptr;
+            HIP_TRY(ctx, hipStreamBeginCapture(a.stream, hipStreamCaptureModeThreadLocal));
+            hipError_t ce = enqueue_pipeline(db, cfg, a.stream, nullptr, nullptr);
+            hipError_t ee = hipStreamEndCapture(a.stream, &g);
+            if (ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return hip_fail(ctx, ce, "pipeline capture");
+            }
+            if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
+            hipError_t ie = hipGraphInstantiate(&a.gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                a.gexec = nullptr;
+                return hip_fail(ctx, ie, "hipGraphInstantiate");
+            }
+            a.gkey = key;
+        }
+        HIP_TRY(ctx, hipGraphLaunch(a.gexec, a.stream));
+        a.graph_launches++;
+    }
+    if (sync || prof) HIP_TRY(ctx, hipStreamSynchronize(a.stream));
+    if (prof) {
         float ms = 0.f;
         HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ctx->prof_ms += ms;
         ctx->prof_launches++;
     }
+    return TNNI_OK;
+}
+
+// Heaviest tiles first: the persistent kernel drains big tiles early and
+// fills the tail with small ones.
+double tile_cost(const TileDev &t) { return (double)t.W * t.H + 8.0 * (double)t.n; }
+
+// ---- served form ---------------------------------------------------------
+
+uint64_t request_bytes(const TileDev &t, bool labels, bool d2)
+{
+    const uint64_t cells = (uint64_t)t.nrows * t.ncols, pix = (uint64_t)t.W * t.H;
+    uint64_t b = (uint64_t)t.n * (12 + 4 + 16) + ((uint64_t)t.nbx * t.nby + 1) * 6 + cells * 4;
+    if (labels) b += pix * 4;
+    if (d2) b += pix * 4;
+    return b + sizeof(TileDev) + ((uint64_t)t.n / kChunkPts + 1) * sizeof(Chunk);
+}
+
+void finish_request(tnni_ctx *ctx, const std::shared_ptr<Request> &r, tnni_status st)
+{
+    r->status = st;
+    r->t_done = now_s();
+    ctx->stats.completed++;
+}
+
+// Drop the cached device buffers of an idle arena (its graph goes with them).
+void arena_trim(tnni_ctx *ctx, Arena &a)
+{
+    DevBuf *bufs[] = {&a.tiles, &a.chunks, &a.batch, &a.counts, &a.rank, &a.partials, &a.sorted,
+                      &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd};
+    for (DevBuf *b : bufs) release(ctx, *b);
+    if (a.gexec) (void)hipGraphExecDestroy(a.gexec);
+    a.gexec = nullptr;
+}
+
+// Launch one admitted batch on slot s (scheduler thread, smu not held; only
+// this thread changes the slots' arenas and busy flags).
+tnni_status launch_batch(tnni_ctx *ctx, Slot &s)
+{
+    Arena &a = s.a;
+    uint64_t allocated;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mem_mu);
+        allocated = ctx->allocated;
+    }
+    if (allocated + s.bytes + s.bytes / 4 > ctx->budget)
+        for (auto &q : ctx->slots)
+            if (q.get() != &s && !q->busy) arena_trim(ctx, q->a);
+    size_t n_in = 0, n_h = 0, n_l = 0, n_d = 0;
+    for (auto &r : s.reqs) {
+        r->in_off = n_in;
+        n_in += r->n;
+        r->out_h_off = n_h;
+        n_h += (size_t)r->t.nrows * r->t.ncols;
+        r->out_l_off = n_l;
+        if (r->out_labels) n_l += (size_t)r->t.W * r->t.H;
+        r->out_d_off = n_d;
+        if (r->out_d2) n_d += (size_t)r->t.W * r->t.H;
+    }
+    tnni_status st;
+    if ((st = ensure(ctx, a.sx, n_in * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.sy, n_in * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.sz, n_in * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.sh, n_h * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.sl, n_l * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.sd, n_d * 4)) != TNNI_OK) return st;
+    HIP_TRY(ctx, hipStreamSynchronize(a.stream));
+    if ((st = ensure_host(ctx, a.h_in, n_in * 12)) != TNNI_OK) return st;
+    if ((st = ensure_host(ctx, a.h_out, (n_h + n_l + n_d) * 4)) != TNNI_OK) return st;
+    int32_t *hx = (int32_t *)a.h_in.p, *hy = hx + n_in, *hz = hy + n_in;
+    std::vector<Prepared> prep(s.reqs.size());
+    for (size_t k = 0; k < s.reqs.size(); ++k) {
+        Request &r = *s.reqs[k];
+        if (r.n) {
+            std::memcpy(hx + r.in_off, r.X, r.n * 4);
+            std::memcpy(hy + r.in_off, r.Y, r.n * 4);
+            std::memcpy(hz + r.in_off, r.Z, r.n * 4);
+        }
+        prep[k].t = r.t;
+        prep[k].lds = r.lds;
+        prep[k].t.X = (const int32_t *)a.sx.p + r.in_off;
+        prep[k].t.Y = (const int32_t *)a.sy.p + r.in_off;
+        prep[k].t.Z = (const int32_t *)a.sz.p + r.in_off;
+        prep[k].t.out_h = (float *)a.sh.p + r.out_h_off;
+        prep[k].t.out_labels = r.out_labels ? (int32_t *)a.sl.p + r.out_l_off : nullptr;
+        prep[k].t.out_d2 = r.out_d2 ? (uint32_t *)a.sd.p + r.out_d_off : nullptr;
+    }
+    if (n_in) {
+        HIP_TRY(ctx, hipMemcpyAsync(a.sx.p, hx, n_in * 4, hipMemcpyHostToDevice, a.stream));
+        HIP_TRY(ctx, hipMemcpyAsync(a.sy.p, hy, n_in * 4, hipMemcpyHostToDevice, a.stream));
+        HIP_TRY(ctx, hipMemcpyAsync(a.sz.p, hz, n_in * 4, hipMemcpyHostToDevice, a.stream));
+    }
+    if ((st = run_prepared(ctx, a, prep, false, Launch::Graph)) != TNNI_OK) return st;
+    float *oh = (float *)a.h_out.p;
+    int32_t *ol = (int32_t *)(oh + n_h);
+    uint32_t *od = (uint32_t *)(ol + n_l);
+    HIP_TRY(ctx, hipMemcpyAsync(oh, a.sh.p, n_h * 4, hipMemcpyDeviceToHost, a.stream));
+    if (n_l) HIP_TRY(ctx, hipMemcpyAsync(ol, a.sl.p, n_l * 4, hipMemcpyDeviceToHost, a.stream));
+    if (n_d) HIP_TRY(ctx, hipMemcpyAsync(od, a.sd.p, n_d * 4, hipMemcpyDeviceToHost, a.stream));
+    HIP_TRY(ctx, hipEventRecord(a.done, a.stream));
+    return TNNI_OK;
+}
+
+// Copy a finished batch's results out of the slot's pinned staging.
+void complete_batch(Slot &s)
+{
+    size_t n_h = 0, n_l = 0;
+    for (auto &r : s.reqs) {
+        n_h += (size_t)r->t.nrows * r->t.ncols;
+        if (r->out_labels) n_l += (size_t)r->t.W * r->t.H;
+    }
+    const float *oh = (const float *)s.a.h_out.p;
+    const int32_t *ol = (const int32_t *)(oh + n_h);
+    const uint32_t *od = (const uint32_t *)(ol + n_l);
+    for (auto &r : s.reqs) {
+        const size_t cells = (size_t)r->t.nrows * r->t.ncols, pix = (size_t)r->t.W * r->t.H;
+        std::memcpy(r->out_h, oh + r->out_h_off, cells * 4);
+        if (r->out_labels) std::memcpy(r->out_labels, ol + r->out_l_off, pix * 4);
+        if (r->out_d2) std::memcpy(r->out_d2, od + r->out_d_off, pix * 4);
+    }
+}
+
+void scheduler(tnni_ctx *ctx)
+{
+    (void)hipSetDevice(ctx->device);
+    std::unique_lock<std::mutex> lk(ctx->smu);
+    for (;;) {
+        // 1. retire finished batches
+        bool any_busy = false;
+        for (auto &sp : ctx->slots) {
+            Slot &s = *sp;
+            if (!s.busy) continue;
+            const hipError_t q = hipEventQuery(s.a.done);
+            if (q == hipErrorNotReady) {
+                any_busy = true;
+                continue;
+            }
+            lk.unlock();
+            if (q == hipSuccess) complete_batch(s);
+            lk.lock();
+            for (auto &r : s.reqs) finish_request(ctx, r, q == hipSuccess ? TNNI_OK : TNNI_E_HIP);
+            ctx->inflight_bytes -= s.bytes;
+            s.reqs.clear();
+            s.bytes = 0;
+            s.busy = false;
+            ctx->dcv.notify_all();
+        }
+        if (ctx->shutdown && !any_busy) break;
+        // 2. admit queued tiles into free slots: oldest first, one
+        //    algorithm/mode per batch, within the HBM budget
+        bool launched = false;
+        for (auto &sp : ctx->slots) {
+            Slot &s = *sp;
+            if (s.busy || ctx->pending.empty() || ctx->shutdown) continue;
+            if (ctx->batch_window_us > 0 && !any_busy &&
+                (int)ctx->pending.size() < ctx->max_batch_tiles &&
+                now_s() - ctx->pending.front()->t_submit < ctx->batch_window_us * 1e-6)
+                break;
+            const int alg = ctx->pending.front()->t.algorithm, mode = ctx->pending.front()->t.mode;
+            while (!ctx->pending.empty() && (int)s.reqs.size() < ctx->max_batch_tiles) {
+                auto &r = ctx->pending.front();
+                if (r->t.algorithm != alg || r->t.mode != mode) break;
+                const bool over = ctx->inflight_bytes + s.bytes + r->footprint > ctx->serve_budget;
+                // over budget: wait until running batches free memory
+                if (over && (!s.reqs.empty() || any_busy)) break;
+                s.bytes += r->footprint;
+                s.reqs.push_back(r);
+                ctx->pending.pop_front();
+            }
+            if (s.reqs.empty()) break;
+            std::stable_sort(s.reqs.begin(), s.reqs.end(),
+                             [](const std::shared_ptr<Request> &x, const std::shared_ptr<Request> &y) {
+                                 return tile_cost(x->t) > tile_cost(y->t);
+                             });
+            ctx->inflight_bytes += s.bytes;
+            s.busy = true;
+            lk.unlock();
+            const tnni_status st = launch_batch(ctx, s);
+            lk.lock();
+            if (st != TNNI_OK) {
+                for (auto &r : s.reqs) finish_request(ctx, r, st);
+                ctx->inflight_bytes -= s.bytes;
+                s.reqs.clear();
+                s.bytes = 0;
+                s.busy = false;
+                ctx->dcv.notify_all();
+                continue;
+            }
+            ctx->stats.batches++;
+            uint64_t tiles = 0, batches = 0;
+            for (auto &q : ctx->slots)
+                if (q->busy) {
+                    tiles += q->reqs.size();
+                    batches++;
+                }
+            ctx->stats.max_inflight_tiles = std::max(ctx->stats.max_inflight_tiles, tiles);
+            ctx->stats.max_inflight_batches = std::max(ctx->stats.max_inflight_batches, batches);
+            launched = true;
+            any_busy = true;
+        }
+        if (launched) continue;
+        if (any_busy || !ctx->pending.empty()) {
+            // batches in flight (poll their events) or a batch window open
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            lk.lock();
+        } else {
+            ctx->scv.wait(lk, [ctx] { return ctx->shutdown || !ctx->pending.empty(); });
+        }
+    }
+    // shutdown: fail what never ran
+    for (auto &r : ctx->pending) finish_request(ctx, r, TNNI_E_SHUTDOWN);
+    ctx->pending.clear();
+    ctx->dcv.notify_all();
+}
+
+tnni_status start_scheduler(tnni_ctx *ctx)
+{
+    if (ctx->sched_started) return TNNI_OK;
+    for (int k = 0; k < ctx->n_slots; ++k) {
+        auto s = std::make_unique<Slot>();
+        tnni_status st = arena_init(ctx, s->a);
+        ctx->slots.push_back(std::move(s));
+        if (st != TNNI_OK) return st;
+    }
+    // admitted footprints stay below 3/4 of the budget: arenas grow with
+    // 25 % headroom and the descriptors and graphs need a little more
+    ctx->serve_budget = ctx->budget / 4 * 3;
+    ctx->sched = std::thread(scheduler, ctx);
+    ctx->sched_started = true;
     return TNNI_OK;
 }
 
@@ -461,14 +636,7 @@ tnni_status tnni_footprint(const tnni_tile_desc *desc, uint64_t n_points, int wi
     std::string err;
     tnni_status st = prepare_tile(*desc, n_points, 256, true, t, lds, err);
     if (st != TNNI_OK) return fail(nullptr, st, err);
-    uint64_t b = 0;
-    b += n_points * 12;                         // X, Y, Z
-    b += n_points * (4 + 16);                   // rank + site records
-    b += ((uint64_t)t.nbx * t.nby + 1) * 6;     // bucket offsets + distance maps
-    b += (uint64_t)t.nrows * t.ncols * 4;       // heights
-    if (with_labels) b += (uint64_t)t.W * t.H * 8;  // labels + d2
-    b += sizeof(TileDev) + ((n_points + kChunkPts - 1) / kChunkPts + 1) * sizeof(Chunk);
-    *bytes = b;
+    *bytes = request_bytes(t, with_labels != 0, with_labels != 0);
     return TNNI_OK;
 }
 
@@ -485,6 +653,9 @@ tnni_status tnni_ctx_create(const tnni_ctx_opts *opts, tnni_ctx **out)
     if (o.device < 0 || o.device >= ndev) return fail(nullptr, TNNI_E_INVALID, "device ordinal out of range");
     tnni_ctx *ctx = new tnni_ctx();
     ctx->device = o.device;
+    ctx->n_slots = o.n_slots > 0 ? o.n_slots : 2;
+    ctx->max_batch_tiles = o.max_batch_tiles > 0 ? o.max_batch_tiles : 256;
+    ctx->batch_window_us = std::max(0, o.batch_window_us);
     hipError_t e;
     if ((e = hipSetDevice(o.device)) != hipSuccess) {
         delete ctx;
@@ -502,9 +673,11 @@ tnni_status tnni_ctx_create(const tnni_ctx_opts *opts, tnni_ctx **out)
         return hip_fail(nullptr, e, "hipMemGetInfo");
     }
     ctx->budget = o.hbm_budget ? o.hbm_budget : (uint64_t)(0.8 * (double)fr);
-    if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipEventCreate(&ctx->ev0)) != hipSuccess || (e = hipEventCreate(&ctx->ev1)) != hipSuccess) {
-        tnni_status st = hip_fail(nullptr, e, "stream/event creation");
+    tnni_status st = arena_init(ctx, ctx->main);
+    if (st == TNNI_OK && ((e = hipEventCreate(&ctx->ev0)) != hipSuccess ||
+                          (e = hipEventCreate(&ctx->ev1)) != hipSuccess))
+        st = hip_fail(nullptr, e, "hipEventCreate");
+    if (st != TNNI_OK) {
         tnni_ctx_destroy(ctx);
         return st;
     }
@@ -516,16 +689,19 @@ void tnni_ctx_destroy(tnni_ctx *ctx)
 {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->tiles, &ctx->chunks, &ctx->batch, &ctx->counts, &ctx->rank,
-                      &ctx->partials, &ctx->sorted, &ctx->hdist, &ctx->edist, &ctx->sx, &ctx->sy, &ctx->sz,
-                      &ctx->sh, &ctx->sl, &ctx->sd};
-    for (DevBuf *b : bufs) release(*b);
+    if (ctx->sched_started) {
+        {
+            std::lock_guard<std::mutex> lk(ctx->smu);
+            ctx->shutdown = true;
+        }
+        ctx->scv.notify_all();
+        ctx->sched.join();
+    }
+    for (auto &s : ctx->slots) arena_free(ctx, s->a);
+    arena_free(ctx, ctx->main);
     for (void *p : ctx->user_allocs) (void)hipFree(p);
-    if (ctx->h_desc) (void)hipHostFree(ctx->h_desc);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -548,7 +724,7 @@ tnni_status tnni_dev_free(tnni_ctx *ctx, void *dptr)
     auto it = std::find(ctx->user_allocs.begin(), ctx->user_allocs.end(), dptr);
     if (it == ctx->user_allocs.end()) return fail(ctx, TNNI_E_INVALID, "pointer not from tnni_dev_alloc");
     ctx->user_allocs.erase(it);
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->main.stream));
     HIP_TRY(ctx, hipFree(dptr));
     return TNNI_OK;
 }
@@ -558,8 +734,8 @@ tnni_status tnni_h2d(tnni_ctx *ctx, void *dst, const void *src, uint64_t bytes)
     if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(ctx, TNNI_E_INVALID, "null argument");
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->main.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->main.stream));
     return TNNI_OK;
 }
 
@@ -568,8 +744,8 @@ tnni_status tnni_d2h(tnni_ctx *ctx, void *dst, const void *src, uint64_t bytes)
     if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(ctx, TNNI_E_INVALID, "null argument");
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->main.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->main.stream));
     return TNNI_OK;
 }
 
@@ -591,7 +767,7 @@ tnni_status tnni_tile_run_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, const i
     v[0].t.out_h = d_out_h;
     v[0].t.out_labels = d_out_labels;
     v[0].t.out_d2 = d_out_d2;
-    return run_prepared(ctx, v, true);
+    return run_prepared(ctx, ctx->main, v, true, Launch::Direct);
 }
 
 tnni_status tnni_tile_run(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *X,
@@ -606,31 +782,32 @@ tnni_status tnni_tile_run(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32
     std::string err;
     tnni_status st = prepare_tile(*desc, n, ctx->n_cu, false, v[0].t, v[0].lds, err);
     if (st != TNNI_OK) return fail(ctx, st, err);
+    Arena &a = ctx->main;
     TileDev &t = v[0].t;
     const size_t cells = (size_t)t.nrows * t.ncols, pix = (size_t)t.W * t.H;
-    if ((st = ensure(ctx, ctx->sx, n * 4)) != TNNI_OK) return st;
-    if ((st = ensure(ctx, ctx->sy, n * 4)) != TNNI_OK) return st;
-    if ((st = ensure(ctx, ctx->sz, n * 4)) != TNNI_OK) return st;
-    if ((st = ensure(ctx, ctx->sh, cells * 4)) != TNNI_OK) return st;
-    if (out_labels && (st = ensure(ctx, ctx->sl, pix * 4)) != TNNI_OK) return st;
-    if (out_d2 && (st = ensure(ctx, ctx->sd, pix * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.sx, n * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.sy, n * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.sz, n * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.sh, cells * 4)) != TNNI_OK) return st;
+    if (out_labels && (st = ensure(ctx, a.sl, pix * 4)) != TNNI_OK) return st;
+    if (out_d2 && (st = ensure(ctx, a.sd, pix * 4)) != TNNI_OK) return st;
     if (n) {
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->sx.p, X, n * 4, hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->sy.p, Y, n * 4, hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->sz.p, Z, n * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(a.sx.p, X, n * 4, hipMemcpyHostToDevice, a.stream));
+        HIP_TRY(ctx, hipMemcpyAsync(a.sy.p, Y, n * 4, hipMemcpyHostToDevice, a.stream));
+        HIP_TRY(ctx, hipMemcpyAsync(a.sz.p, Z, n * 4, hipMemcpyHostToDevice, a.stream));
     }
-    t.X = (const int32_t *)ctx->sx.p;
-    t.Y = (const int32_t *)ctx->sy.p;
-    t.Z = (const int32_t *)ctx->sz.p;
-    t.out_h = (float *)ctx->sh.p;
-    t.out_labels = out_labels ? (int32_t *)ctx->sl.p : nullptr;
-    t.out_d2 = out_d2 ? (uint32_t *)ctx->sd.p : nullptr;
-    if ((st = run_prepared(ctx, v, false)) != TNNI_OK) return st;
-    HIP_TRY(ctx, hipMemcpyAsync(out_h, ctx->sh.p, cells * 4, hipMemcpyDeviceToHost, ctx->stream));
+    t.X = (const int32_t *)a.sx.p;
+    t.Y = (const int32_t *)a.sy.p;
+    t.Z = (const int32_t *)a.sz.p;
+    t.out_h = (float *)a.sh.p;
+    t.out_labels = out_labels ? (int32_t *)a.sl.p : nullptr;
+    t.out_d2 = out_d2 ? (uint32_t *)a.sd.p : nullptr;
+    if ((st = run_prepared(ctx, a, v, false, Launch::Direct)) != TNNI_OK) return st;
+    HIP_TRY(ctx, hipMemcpyAsync(out_h, a.sh.p, cells * 4, hipMemcpyDeviceToHost, a.stream));
     if (out_labels)
-        HIP_TRY(ctx, hipMemcpyAsync(out_labels, ctx->sl.p, pix * 4, hipMemcpyDeviceToHost, ctx->stream));
-    if (out_d2) HIP_TRY(ctx, hipMemcpyAsync(out_d2, ctx->sd.p, pix * 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(out_labels, a.sl.p, pix * 4, hipMemcpyDeviceToHost, a.stream));
+    if (out_d2) HIP_TRY(ctx, hipMemcpyAsync(out_d2, a.sd.p, pix * 4, hipMemcpyDeviceToHost, a.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(a.stream));
     return TNNI_OK;
 }
 
@@ -658,14 +835,74 @@ tnni_status tnni_batch_run_dev(tnni_ctx *ctx, int n_tiles, const tnni_tile_desc 
         v[k].t.out_labels = d_out_labels ? d_out_labels[k] : nullptr;
         v[k].t.out_d2 = d_out_d2 ? d_out_d2[k] : nullptr;
     }
-    // largest work first: the persistent kernel drains big tiles early and
-    // fills the tail with small ones
-    std::stable_sort(v.begin(), v.end(), [](const Prepared &a, const Prepared &b) {
-        const double ca = (double)a.t.W * a.t.H + 8.0 * (double)a.t.n;
-        const double cb = (double)b.t.W * b.t.H + 8.0 * (double)b.t.n;
-        return ca > cb;
-    });
-    return run_prepared(ctx, v, true);
+    std::stable_sort(v.begin(), v.end(),
+                     [](const Prepared &a, const Prepared &b) { return tile_cost(a.t) > tile_cost(b.t); });
+    return run_prepared(ctx, ctx->main, v, true, ctx->prof ? Launch::Direct : Launch::Graph);
+}
+
+tnni_status tnni_submit(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *X,
+                        const int32_t *Y, const int32_t *Z, uint64_t n, float *out_h,
+                        int32_t *out_labels, uint32_t *out_d2, uint64_t *ticket)
+{
+    if (!ctx || !desc || !out_h || !ticket || (n && (!X || !Y || !Z)))
+        return fail(ctx, TNNI_E_INVALID, "null argument");
+    auto r = std::make_shared<Request>();
+    std::string err;
+    tnni_status st = prepare_tile(*desc, n, ctx->n_cu, true, r->t, r->lds, err);
+    if (st != TNNI_OK) return fail(ctx, st, err);
+    r->X = X;
+    r->Y = Y;
+    r->Z = Z;
+    r->n = n;
+    r->out_h = out_h;
+    r->out_labels = out_labels;
+    r->out_d2 = out_d2;
+    r->footprint = request_bytes(r->t, out_labels != nullptr, out_d2 != nullptr);
+    std::lock_guard<std::mutex> lk(ctx->smu);
+    if (ctx->shutdown) return fail(ctx, TNNI_E_SHUTDOWN, "context is shutting down");
+    if ((st = start_scheduler(ctx)) != TNNI_OK) return st;
+    if (r->footprint > ctx->serve_budget)
+        return fail(ctx, TNNI_E_BUDGET, "tile footprint exceeds the context's HBM budget");
+    r->id = ctx->next_id++;
+    r->t_submit = now_s();
+    ctx->reqs[r->id] = r;
+    ctx->pending.push_back(r);
+    ctx->stats.submitted++;
+    ctx->scv.notify_all();
+    *ticket = r->id;
+    return TNNI_OK;
+}
+
+tnni_status tnni_poll(tnni_ctx *ctx, uint64_t ticket)
+{
+    if (!ctx) return fail(ctx, TNNI_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->smu);
+    auto it = ctx->reqs.find(ticket);
+    if (it == ctx->reqs.end()) return fail(ctx, TNNI_E_INVALID, "unknown ticket");
+    return it->second->status;
+}
+
+tnni_status tnni_wait(tnni_ctx *ctx, uint64_t ticket, double *latency_s)
+{
+    if (!ctx) return fail(ctx, TNNI_E_INVALID, "null argument");
+    std::unique_lock<std::mutex> lk(ctx->smu);
+    auto it = ctx->reqs.find(ticket);
+    if (it == ctx->reqs.end()) return fail(ctx, TNNI_E_INVALID, "unknown ticket");
+    std::shared_ptr<Request> r = it->second;
+    ctx->dcv.wait(lk, [&r] { return r->status != TNNI_PENDING; });
+    ctx->reqs.erase(ticket);
+    if (latency_s) *latency_s = r->t_done - r->t_submit;
+    if (r->status != TNNI_OK) ctx->err = "request failed";
+    return r->status;
+}
+
+tnni_status tnni_serve_stats_read(tnni_ctx *ctx, tnni_serve_stats *out)
+{
+    if (!ctx || !out) return fail(ctx, TNNI_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->smu);
+    *out = ctx->stats;
+    out->inflight_bytes = ctx->inflight_bytes;
+    return TNNI_OK;
 }
 
 tnni_status tnni_profile_enable(tnni_ctx *ctx, int enable)

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "terranni.h"
 
 namespace tnni {
@@ -140,5 +142,16 @@ hipError_t enqueue_pipeline(const DevBuffers &b, const LaunchCfg &cfg,
 
 // Largest workgroups per CU of the tile kernel at the given LDS size.
 int tile_kernel_occupancy(size_t lds_bytes);
+
+// ---- host geometry (tnni_geometry.hip) ----
+constexpr size_t kLdsBudget = 60 * 1024;  // two workgroups per CU, under the 64 KB per-group limit
+
+tnni_status compute_frame(const tnni_tile_desc &d, tnni_frame_info &f, std::string &err);
+
+// Fill everything of a TileDev that depends on the descriptor and the point
+// count (not on the batch layout).  batched: the tile shares the GPU with
+// others, so its cell tiles need not be shrunk to fill the CUs.
+tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool batched, TileDev &t,
+                         size_t &lds_out, std::string &err);
 
 }  // namespace tnni
