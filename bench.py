@@ -108,7 +108,12 @@ def main():
     ap.add_argument("--grid", type=int, default=4096)
     ap.add_argument("--points", type=int, default=4_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile-cells", type=int, default=0, help="cell-tile side (0 = auto)")
+    ap.add_argument("--bucket-px", type=int, default=0, help="bucket side in pixels (0 = auto)")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (calibrates "
+                         "the FETCH_SIZE / WRITE_SIZE counters against known traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); tiles are "
@@ -119,7 +124,8 @@ def main():
     import terranni_amd.synth as S
 
     t0 = time.perf_counter()
-    tile = S.workload_single_4m(seed=1, n=args.points, grid=args.grid)
+    tile = S.workload_single_4m(seed=1, n=args.points, grid=args.grid,
+                                tile_cells=args.tile_cells, bucket_px=args.bucket_px)
     log(f"generated {tile.n} points, {tile.cells} cells in {time.perf_counter() - t0:.1f}s")
     fr = T.frame(tile.params)
 
@@ -141,6 +147,12 @@ def main():
     launches, kern_ms = ctx.profile_read(reset=True)
     ctx.profile(False)
     h = dH.to_host()
+    if args.copy_probe_mb:
+        nb = args.copy_probe_mb << 20
+        a, c = ctx.alloc((nb,), np.uint8), ctx.alloc((nb,), np.uint8)
+        ctx.copy(c, a)
+        a.free()
+        c.free()
 
     cells_s = tile.cells * args.steps / elapsed
     pts_s = tile.n * args.steps / elapsed

@@ -142,6 +142,7 @@ tnni_status tnni_dev_alloc(tnni_ctx *ctx, uint64_t bytes, void **dptr);
 tnni_status tnni_dev_free(tnni_ctx *ctx, void *dptr);
 tnni_status tnni_h2d(tnni_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 tnni_status tnni_d2h(tnni_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+tnni_status tnni_d2d(tnni_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 
 /* Plain form on device pointers: one launch sequence for one tile on the
  * context stream, then a stream synchronisation. */

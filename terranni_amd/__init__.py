@@ -165,6 +165,7 @@ def lib():
     L.tnni_dev_free.argtypes = [P, P]
     L.tnni_h2d.argtypes = [P, P, P, ctypes.c_uint64]
     L.tnni_d2h.argtypes = [P, P, P, ctypes.c_uint64]
+    L.tnni_d2d.argtypes = [P, P, P, ctypes.c_uint64]
     L.tnni_profile_enable.argtypes = [P, ctypes.c_int]
     L.tnni_profile_read.argtypes = [P, ctypes.POINTER(ctypes.c_uint64),
                                     ctypes.POINTER(ctypes.c_double), ctypes.c_int]
@@ -175,7 +176,7 @@ def lib():
     L.tnni_serve_stats_read.argtypes = [P, ctypes.POINTER(ServeStats)]
     for name in ("tnni_frame", "tnni_footprint", "tnni_ctx_create", "tnni_tile_run",
                  "tnni_tile_run_dev", "tnni_batch_run_dev", "tnni_dev_alloc", "tnni_dev_free",
-                 "tnni_h2d", "tnni_d2h", "tnni_profile_enable", "tnni_profile_read",
+                 "tnni_h2d", "tnni_d2h", "tnni_d2d", "tnni_profile_enable", "tnni_profile_read",
                  "tnni_submit", "tnni_poll", "tnni_wait", "tnni_serve_stats_read"):
         getattr(L, name).restype = ctypes.c_int
     del i32p
@@ -293,6 +294,10 @@ class Context:
     def _d2h(self, out: np.ndarray, d: DevArray):
         if d.nbytes:
             self._chk(lib().tnni_d2h(self.h, _ptr(out), ctypes.c_void_p(d.ptr), d.nbytes))
+
+    def copy(self, dst: DevArray, src: DevArray, nbytes: int | None = None):
+        n = min(dst.nbytes, src.nbytes) if nbytes is None else nbytes
+        self._chk(lib().tnni_d2d(self.h, ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), n))
 
     def _free(self, ptr):
         self._chk(lib().tnni_dev_free(self.h, ctypes.c_void_p(ptr)))

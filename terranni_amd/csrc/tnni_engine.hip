@@ -749,6 +749,16 @@ tnni_status tnni_d2h(tnni_ctx *ctx, void *dst, const void *src, uint64_t bytes)
     return TNNI_OK;
 }
 
+tnni_status tnni_d2d(tnni_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(ctx, TNNI_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->main.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->main.stream));
+    return TNNI_OK;
+}
+
 tnni_status tnni_tile_run_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *dX,
                               const int32_t *dY, const int32_t *dZ, uint64_t n, float *d_out_h,
                               int32_t *d_out_labels, uint32_t *d_out_d2)

@@ -178,6 +178,11 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
     t.bm = bm;
     t.bK = (int)K;
     t.ecap = (int)std::min<int64_t>(250, f.Rsb / bs + 2);
+    t.bm_magic = bm == 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)bm - 1) / (uint64_t)bm);
+    t.inv_bs = 1.0 / (double)bs;
+    // A site of ring 0 or 1 around a pixel's bucket is within 2*bs of the
+    // pixel on both axes (strictly less than 2*bs).
+    t.ring1_covered = d.algorithm == TNNI_ALG_PLANES ? (2 * bs <= f.Rs) : (8 * bs * bs <= f.Rs2);
     // cell tiles: the largest that fits the LDS budget; the plain form
     // shrinks them so that a lone small tile still spreads over the CUs
     auto lds_for = [&t, &d](int tc) {
@@ -193,7 +198,7 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
             const Region g = region_of(t, 0, tj);
             rh = std::max(rh, g.rl1 - g.rl0 + 1);
         }
-        return (size_t)rw * rh * 8;
+        return (size_t)rw * rh * 8 + tile_kernel_stage_bytes();
     };
     size_t lds = 0;
     if (d.tile_cells > 0) {
@@ -203,10 +208,14 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
             return TNNI_E_GEOMETRY;
         }
     } else {
-        static const int kTc[] = {64, 48, 32, 24, 16, 12, 8, 6, 4, 3, 2, 1};
+        // the largest cell tile that leaves room for four workgroups per CU
+        // (16 waves), else the largest that fits at all
+        static const int kTc[] = {64, 48, 40, 32, 24, 16, 12, 8, 6, 4, 3, 2, 1};
         int chosen = -1;
+        size_t budget = kLdsBudget4;
+        for (int pass = 0; pass < 2 && chosen < 0; ++pass, budget = kLdsBudget)
         for (int tc : kTc) {
-            if (lds_for(tc) > kLdsBudget) continue;
+            if (lds_for(tc) > budget) continue;
             chosen = tc;
             if (!batched && tc > 16 && (int64_t)t.ntx * t.nty < 2ll * n_cu) continue;
             break;

@@ -36,7 +36,10 @@ struct TileDev {
     int gx0, gy0, bs, nbx, nby;
     int bm, bK;
     int ecap;                   // cap of the empty-distance map
+    uint32_t bm_magic;          // ceil(2^32 / bm), 0 for bm == 1
+    int ring1_covered;          // every site of rings 0-1 covers the pixel
     int pad1;
+    double inv_bs;              // 1 / bs
     int64_t bk_base;            // first bucket of this tile in the batch
     int64_t row_base;           // first bucket row of this tile in the batch
     int tc, ntx, nty;           // cell-tile side and tiles along x / y
@@ -142,9 +145,15 @@ hipError_t enqueue_pipeline(const DevBuffers &b, const LaunchCfg &cfg,
 
 // Largest workgroups per CU of the tile kernel at the given LDS size.
 int tile_kernel_occupancy(size_t lds_bytes);
+// LDS of the per-wave site stages of the tile kernel (in front of the
+// label tile).
+size_t tile_kernel_stage_bytes();
 
 // ---- host geometry (tnni_geometry.hip) ----
-constexpr size_t kLdsBudget = 60 * 1024;  // two workgroups per CU, under the 64 KB per-group limit
+// LDS of one workgroup: the per-wave stages plus the label tile, under the
+// 64 KB per-group limit (static LDS takes a few more bytes)
+constexpr size_t kLdsBudget = 60 * 1024;
+constexpr size_t kLdsBudget4 = 40 * 1024;  // four workgroups per CU (160 KB LDS)
 
 tnni_status compute_frame(const tnni_tile_desc &d, tnni_frame_info &f, std::string &err);
 

@@ -44,10 +44,19 @@ __device__ __forceinline__ int32_t site_value(const TileDev &t, int32_t Z)
     return (int32_t)(uint32_t)(int64_t)v;
 }
 
+// v / d for 0 <= v < 2^30: FP64 reciprocal, then at most one correction.
+__device__ __forceinline__ int div_pos(int v, int d, double inv_d)
+{
+    int q = (int)((double)v * inv_d);
+    if (q * d > v) --q;
+    else if ((q + 1) * d <= v) ++q;
+    return q;
+}
+
 __device__ __forceinline__ uint32_t bucket_of(const TileDev &t, int xu, int yu)
 {
-    const int bx = (xu - t.gx0) / t.bs;
-    const int by = (yu - t.gy0) / t.bs;
+    const int bx = div_pos(xu - t.gx0, t.bs, t.inv_bs);
+    const int by = div_pos(yu - t.gy0, t.bs, t.inv_bs);
     return (uint32_t)by * (uint32_t)t.nbx + (uint32_t)bx;
 }
 
@@ -59,6 +68,8 @@ __global__ void __launch_bounds__(kBlock) k_zero(BatchDev *b, uint32_t *counts)
     if (blockIdx.x == 0 && threadIdx.x == 0) b->work_counter = 0u;
 }
 
+constexpr int kPtsPerStep = 4;  // points per thread with their memory operations in flight
+
 __global__ void __launch_bounds__(kBlock)
 k_count(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
         uint32_t *counts, uint32_t *rank)
@@ -67,12 +78,24 @@ k_count(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
     for (int c = blockIdx.x; c < nch; c += gridDim.x) {
         const Chunk ch = chunks[c];
         const TileDev &t = tiles[ch.tile];
-        for (int64_t i = ch.begin + threadIdx.x; i < ch.end; i += kBlock) {
-            int xu, yu;
-            uint32_t r = kNoD2;
-            if (point_units(t, t.X[i], t.Y[i], xu, yu))
-                r = atomicAdd(&counts[t.bk_base + bucket_of(t, xu, yu)], 1u);
-            rank[t.pt_base + i] = r;
+        for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kBlock * kPtsPerStep) {
+            uint32_t bk[kPtsPerStep], r[kPtsPerStep];
+#pragma unroll
+            for (int q = 0; q < kPtsPerStep; ++q) {
+                const int64_t i = i0 + (int64_t)q * kBlock;
+                int xu, yu;
+                bk[q] = kNoD2;
+                if (i < ch.end && point_units(t, t.X[i], t.Y[i], xu, yu))
+                    bk[q] = (uint32_t)t.bk_base + bucket_of(t, xu, yu);
+            }
+#pragma unroll
+            for (int q = 0; q < kPtsPerStep; ++q)
+                r[q] = bk[q] != kNoD2 ? atomicAdd(&counts[bk[q]], 1u) : kNoD2;
+#pragma unroll
+            for (int q = 0; q < kPtsPerStep; ++q) {
+                const int64_t i = i0 + (int64_t)q * kBlock;
+                if (i < ch.end) rank[t.pt_base + i] = r[q];
+            }
         }
     }
 }
@@ -85,13 +108,26 @@ k_scatter(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
     for (int c = blockIdx.x; c < nch; c += gridDim.x) {
         const Chunk ch = chunks[c];
         const TileDev &t = tiles[ch.tile];
-        for (int64_t i = ch.begin + threadIdx.x; i < ch.end; i += kBlock) {
-            const uint32_t r = rank[t.pt_base + i];
-            if (r == kNoD2) continue;
-            int xu, yu;
-            point_units(t, t.X[i], t.Y[i], xu, yu);
-            const uint32_t pos = offsets[t.bk_base + bucket_of(t, xu, yu)] + r;
-            sorted[pos] = make_int4(xu, yu, (int)i, site_value(t, t.Z[i]));
+        for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kBlock * kPtsPerStep) {
+            int4 rec[kPtsPerStep];
+            uint32_t bk[kPtsPerStep], r[kPtsPerStep], pos[kPtsPerStep];
+#pragma unroll
+            for (int q = 0; q < kPtsPerStep; ++q) {
+                const int64_t i = i0 + (int64_t)q * kBlock;
+                r[q] = i < ch.end ? rank[t.pt_base + i] : kNoD2;
+                bk[q] = 0;
+                if (r[q] != kNoD2) {
+                    int xu, yu;
+                    point_units(t, t.X[i], t.Y[i], xu, yu);
+                    bk[q] = (uint32_t)t.bk_base + bucket_of(t, xu, yu);
+                    rec[q] = make_int4(xu, yu, (int)i, site_value(t, t.Z[i]));
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kPtsPerStep; ++q) pos[q] = r[q] != kNoD2 ? offsets[bk[q]] + r[q] : 0u;
+#pragma unroll
+            for (int q = 0; q < kPtsPerStep; ++q)
+                if (r[q] != kNoD2) sorted[pos[q]] = rec[q];
         }
     }
 }
@@ -203,14 +239,25 @@ k_edist_rows(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict
         const uint32_t *o = offs + rb;
         const int nbx = t.nbx, cap = t.ecap;
         for (int x = threadIdx.x; x < nbx; x += kBlock) {
+            // bucket b holds sites iff o[b + 1] > o[b]; four distances per
+            // step, their offsets loaded up front
             int d = cap;
-            for (int q = 0; q < cap; ++q) {
-                const bool l = x - q >= 0 && o[x - q + 1] > o[x - q];
-                const bool r = x + q < nbx && o[x + q + 1] > o[x + q];
-                if (l || r) {
-                    d = q;
-                    break;
+            for (int q0 = 0; q0 < cap; q0 += 4) {
+                uint32_t lo[5], hi[5];
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    const int a = x - q0 - q + 1, c = x + q0 + q;
+                    lo[q] = (a >= 0) ? o[a] : 0u;   // o[x - dq + 1] for dq = q0 + q
+                    hi[q] = (c <= nbx) ? o[c] : 0u; // o[x + dq]
                 }
+#pragma unroll
+                for (int q = 3; q >= 0; --q) {
+                    const int dq = q0 + q;
+                    const bool l = x - dq >= 0 && lo[q] > lo[q + 1];   // bucket x-dq
+                    const bool r = x + dq < nbx && hi[q + 1] > hi[q];  // bucket x+dq
+                    if ((l || r) && dq < cap) d = min(d, dq);
+                }
+                if (d < cap) break;
             }
             hdist[rb + x] = (uint8_t)d;
         }
@@ -230,9 +277,17 @@ k_edist_cols(const BatchDev *b, const TileDev *tiles, const uint8_t *__restrict_
         const uint8_t *h = hdist + t.bk_base;
         for (int x = threadIdx.x; x < nbx; x += kBlock) {
             int e = t.ecap;
-            for (int q = 0; q < e; ++q) {
-                if (row - q >= 0) e = min(e, max(q, (int)h[(int64_t)(row - q) * nbx + x]));
-                if (row + q < nby) e = min(e, max(q, (int)h[(int64_t)(row + q) * nbx + x]));
+            for (int q0 = 0; q0 < e; q0 += 4) {
+                int up[4], dn[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int a = row - q0 - q, c = row + q0 + q;
+                    up[q] = a >= 0 ? (int)h[(int64_t)a * nbx + x] : 255;
+                    dn[q] = c < nby ? (int)h[(int64_t)c * nbx + x] : 255;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    e = min(e, max(q0 + q, min(up[q], dn[q])));
             }
             edist[t.bk_base + (int64_t)row * nbx + x] = (uint8_t)e;
         }
@@ -241,71 +296,243 @@ k_edist_cols(const BatchDev *b, const TileDev *tiles, const uint8_t *__restrict_
 
 // ---- the tile kernel -------------------------------------------------------
 
-// Visit the sites stored at [s, e) for the pixel centre (px, py).
+// One site against the pixel centre (px, py): keep the smallest
+// (d2, index) among the sites whose region covers the pixel.  Branch-free,
+// so the whole record is one 16-byte load.  |dx|, |dy| <= R < 2^23 for a
+// covering site, so 24-bit multiplies are exact there.
+// ALG == kAllCover: the caller has shown that every visited site covers the
+// pixel, so the region test is skipped.
+constexpr int kAllCover = 2;
+
 template <int ALG>
-__device__ __forceinline__ void scan_range(uint32_t s, uint32_t e, const int4 *__restrict__ sorted,
+__device__ __forceinline__ void visit_site(const int4 st, int px, int py, int R, uint32_t R2,
+                                           uint64_t &best, int &bval)
+{
+    const int dx = px - st.x, dy = py - st.y;
+    const uint32_t dd = (uint32_t)__mul24(dx, dx) + (uint32_t)__mul24(dy, dy);
+    if (ALG == kAllCover) {
+        const uint64_t key = ((uint64_t)dd << 32) | (uint32_t)st.z;
+        const bool better = key < best;
+        best = better ? key : best;
+        bval = better ? st.w : bval;
+        return;
+    }
+    bool cov = max((uint32_t)(dx + R), (uint32_t)(dy + R)) <= 2u * (uint32_t)R;
+    if (ALG == TNNI_ALG_CONES) cov = cov && dd <= R2;
+    // a site that does not cover the pixel gets the key of "no site"
+    const uint32_t hi = cov ? dd : 0xFFFFFFFFu;
+    const uint32_t lo = cov ? (uint32_t)st.z : 0xFFFFFFFFu;
+    const uint64_t key = ((uint64_t)hi << 32) | lo;
+    const bool better = key < best;
+    best = better ? key : best;
+    bval = better ? st.w : bval;
+}
+
+// Visit the sites stored at [s, e) of a site array (global or LDS).
+template <int ALG>
+__device__ __forceinline__ void scan_range(uint32_t s, uint32_t e, const int4 *__restrict__ sites,
                                            int px, int py, int R, uint32_t R2,
                                            uint64_t &best, int &bval)
 {
-    const uint32_t twoR = 2u * (uint32_t)R;
-    for (uint32_t p = s; p < e; ++p) {
-        const int4 st = sorted[p];
-        const int dx = px - st.x, dy = py - st.y;
-        const uint32_t ux = (uint32_t)dx, uy = (uint32_t)dy;
-        const uint32_t dd = ux * ux + uy * uy;
-        // inside the L-inf extent of the region (planes: the region itself)
-        bool cov = max((uint32_t)(dx + R), (uint32_t)(dy + R)) <= twoR;
-        if (ALG == TNNI_ALG_CONES) cov = cov && dd <= R2;
-        const uint64_t key = cov ? (((uint64_t)dd << 32) | (uint32_t)st.z) : ~0ull;
-        if (key < best) {
-            best = key;
-            bval = st.w;
+    uint32_t p = s;
+    // two records in flight per step
+    for (; p + 1 < e; p += 2) {
+        const int4 a = sites[p];
+        const int4 b = sites[p + 1];
+        visit_site<ALG>(a, px, py, R, R2, best, bval);
+        visit_site<ALG>(b, px, py, R, R2, best, bval);
+    }
+    if (p < e) visit_site<ALG>(sites[p], px, py, R, R2, best, bval);
+}
+
+// Ring r of the bucket square around (bx, by), from the global arrays.
+template <int ALG>
+__device__ __forceinline__ void ring_global(const TileDev &t, const uint32_t *__restrict__ offs,
+                                            const int4 *__restrict__ sorted, int bx, int by, int r,
+                                            int px, int py, int R, uint64_t &best, int &bval)
+{
+    const uint32_t *base = offs + t.bk_base;
+    for (int dy = -r; dy <= r; ++dy) {
+        const int row = by + dy;
+        if (row < 0 || row >= t.nby) continue;
+        const uint32_t *rs = base + (int64_t)row * t.nbx;
+        uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+        if (dy == -r || dy == r) {
+            const int c0 = max(bx - r, 0), c1 = min(bx + r, t.nbx - 1);
+            if (c0 <= c1) {
+                a0 = rs[c0];
+                a1 = rs[c1 + 1];
+            }
+        } else {
+            if (bx - r >= 0) {
+                a0 = rs[bx - r];
+                a1 = rs[bx - r + 1];
+            }
+            if (bx + r < t.nbx) {
+                b0 = rs[bx + r];
+                b1 = rs[bx + r + 1];
+            }
         }
+        scan_range<ALG>(a0, a1, sorted, px, py, R, t.Rs2, best, bval);
+        scan_range<ALG>(b0, b1, sorted, px, py, R, t.Rs2, best, bval);
     }
 }
 
-// Nearest covering site of pixel (k, l): ring search over the bucket grid.
-// Rings closer than the bucket's empty distance hold no site and are
-// skipped; the search stops when every unvisited site is provably farther
-// than the best so far (strictly, so an equal distance still reaches the
-// lower index) or lies outside the site region.
+// k / bm for a pixel index 0 <= k < 2^20: multiply by ceil(2^32 / bm)
+// (exact since k * (ceil(2^32/bm) * bm - 2^32) < 2^32); bm == 1 has no
+// 32-bit multiplier and is taken as is.
+__device__ __forceinline__ int div_bm(const TileDev &t, int k)
+{
+    return t.bm_magic ? (int)__umulhi((uint32_t)k, t.bm_magic) : k;
+}
+
+// Per-wave LDS staging of the sites around one 8x8 block of pixels: the
+// block's buckets plus one bucket on every side (rings 0 and 1 of every
+// pixel of the block).
+constexpr int kWinMax = 10;      // window side in buckets (8 pixels + 2)
+constexpr int kStageCap = 384;   // staged sites per wave
+struct WaveStage {
+    uint32_t woff[kWinMax * (kWinMax + 1)];  // bucket offsets of the window rows
+    uint32_t rowpre[kWinMax];                // first staged slot of each row
+    uint32_t pad[2];
+    int4 sites[kStageCap];
+};
+static_assert(sizeof(WaveStage) % 16 == 0, "WaveStage must keep int4 alignment");
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Ring r (0 or 1) of the bucket square around (bx, by), from the stage.
 template <int ALG>
-__device__ __forceinline__ void label_pixel(const TileDev &t, const uint32_t *__restrict__ offs,
-                                            const uint8_t *__restrict__ edist,
-                                            const int4 *__restrict__ sorted, int k, int l,
+__device__ __forceinline__ void ring_staged(const TileDev &t, const WaveStage &ws, int cx0, int cy0,
+                                            int wp, int bx, int by, int r, int px, int py, int R,
                                             uint64_t &best, int &bval)
 {
+    for (int dy = -r; dy <= r; ++dy) {
+        const int row = by + dy;
+        if (row < 0 || row >= t.nby) continue;
+        const int j = row - cy0;
+        const uint32_t *wr = ws.woff + j * wp;
+        const uint32_t rel = ws.rowpre[j] - wr[0];  // staged slot = offset + rel
+        uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+        if (dy == -r || dy == r) {
+            const int c0 = max(bx - r, 0), c1 = min(bx + r, t.nbx - 1);
+            if (c0 <= c1) {
+                a0 = wr[c0 - cx0] + rel;
+                a1 = wr[c1 + 1 - cx0] + rel;
+            }
+        } else {
+            if (bx - r >= 0) {
+                a0 = wr[bx - r - cx0] + rel;
+                a1 = wr[bx - r + 1 - cx0] + rel;
+            }
+            if (bx + r < t.nbx) {
+                b0 = wr[bx + r - cx0] + rel;
+                b1 = wr[bx + r + 1 - cx0] + rel;
+            }
+        }
+        scan_range<ALG>(a0, a1, ws.sites, px, py, R, t.Rs2, best, bval);
+        scan_range<ALG>(b0, b1, ws.sites, px, py, R, t.Rs2, best, bval);
+    }
+}
+
+// Nearest covering site of every pixel of one 8x8 block (one wave, one
+// pixel per lane; inactive lanes still take part in the staging).  Each
+// lane runs a ring search over the bucket grid: rings closer than its
+// bucket's empty distance hold no site and are skipped; rings 0 and 1 come
+// from the wave's LDS stage when the window fits it; the search stops when
+// every unvisited site is provably farther than the best so far (strictly,
+// so an equal distance still reaches the lower index) or outside the site
+// region.
+template <int ALG>
+__device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__restrict__ offs,
+                                            const uint8_t *__restrict__ edist,
+                                            const int4 *__restrict__ sorted, WaveStage &ws,
+                                            int kA, int lA, int kB, int lB, int k, int l,
+                                            bool active, uint64_t &best, int &bval)
+{
+    const int lane = threadIdx.x & 63;
     best = ~0ull;
     bval = 0;
+    const int bs = t.bs;
+    const int bx = t.bK + div_bm(t, k), by = t.bK + div_bm(t, l);
+    const int r0 = active ? (int)edist[t.bk_base + (int64_t)by * t.nbx + bx] : 1 << 20;
+
+    // the window: the block's buckets and one more on each side
+    const int cx0 = max(t.bK + div_bm(t, kA) - 1, 0), cx1 = min(t.bK + div_bm(t, kB) + 1, t.nbx - 1);
+    const int cy0 = max(t.bK + div_bm(t, lA) - 1, 0), cy1 = min(t.bK + div_bm(t, lB) + 1, t.nby - 1);
+    const int ww = cx1 - cx0 + 1, wh = cy1 - cy0 + 1, wp = ww + 1;
+    bool staged = false;
+    if (__any(r0 <= 1)) {
+        wave_sync();  // the previous block's reads of the stage are done
+        for (int i = lane; i < wh * wp; i += 64) {
+            const int j = i / wp, c = i - j * wp;
+            ws.woff[i] = offs[t.bk_base + (int64_t)(cy0 + j) * t.nbx + cx0 + c];
+        }
+        wave_sync();
+        uint32_t cnt = 0;
+        if (lane < wh) cnt = ws.woff[lane * wp + ww] - ws.woff[lane * wp];
+        uint32_t inc = cnt;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const uint32_t o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        const uint32_t pre = inc - cnt;
+        const uint32_t total = __shfl(inc, wh - 1, 64);
+        if (lane < wh) ws.rowpre[lane] = pre;
+        staged = total <= (uint32_t)kStageCap;
+        if (staged) {
+            wave_sync();
+            // every load in flight before the first store
+            int4 v[kStageCap / 64];
+#pragma unroll
+            for (int q = 0; q < kStageCap / 64; ++q) {
+                const uint32_t i = lane + 64u * q;
+                if (i < total) {
+                    int j = 0;
+#pragma unroll
+                    for (int jj = 1; jj < kWinMax; ++jj)
+                        j += (jj < wh && ws.rowpre[jj] <= i) ? 1 : 0;
+                    v[q] = sorted[ws.woff[j * wp] + (i - ws.rowpre[j])];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kStageCap / 64; ++q) {
+                const uint32_t i = lane + 64u * q;
+                if (i < total) ws.sites[i] = v[q];
+            }
+        }
+        wave_sync();
+    }
+    if (!active) return;
+
     const int px = t.x0 + k * t.ps_u, py = t.y0 + l * t.ps_u;
-    const int bs = t.bs, nbx = t.nbx, nby = t.nby;
-    const int bx = t.bK + k / t.bm;
-    const int by = t.bK + l / t.bm;
-    const uint32_t *base = offs + t.bk_base;
     const int R = (ALG == TNNI_ALG_PLANES) ? t.Rs : t.Rsb;
-    // distances from the pixel centre to the edges of its own bucket
+    // distance from the pixel centre to the outside of its own bucket
     const int el = px - (t.gx0 + bx * bs) + 1, er = t.gx0 + (bx + 1) * bs - px;
     const int eb = py - (t.gy0 + by * bs) + 1, et = t.gy0 + (by + 1) * bs - py;
     const int e0 = min(min(el, er), min(eb, et));
-    for (int r = edist[t.bk_base + (int64_t)by * nbx + bx];; ++r) {
+    for (int r = r0;; ++r) {
         if (r > 0) {
             // every site outside the visited (2r-1)^2 square is at least gap away
             const int gap = e0 + (r - 1) * bs;
             if (gap > t.Rsb) break;
             if ((uint64_t)gap * (uint64_t)gap > (best >> 32)) break;
         }
-        for (int dy = -r; dy <= r; ++dy) {
-            const int row = by + dy;
-            if (row < 0 || row >= nby) continue;
-            const uint32_t *rs = base + (int64_t)row * nbx;
-            if (dy == -r || dy == r) {
-                const int c0 = max(bx - r, 0), c1 = min(bx + r, nbx - 1);
-                if (c0 <= c1) scan_range<ALG>(rs[c0], rs[c1 + 1], sorted, px, py, R, t.Rs2, best, bval);
-            } else {
-                if (bx - r >= 0) scan_range<ALG>(rs[bx - r], rs[bx - r + 1], sorted, px, py, R, t.Rs2, best, bval);
-                if (bx + r < nbx) scan_range<ALG>(rs[bx + r], rs[bx + r + 1], sorted, px, py, R, t.Rs2, best, bval);
-            }
+        if (staged && r <= 1) {
+            // the sites of rings 0 and 1 lie within 2*bs (L-inf) of the pixel
+            if (t.ring1_covered)
+                ring_staged<kAllCover>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
+            else
+                ring_staged<ALG>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
         }
+        else
+            ring_global<ALG>(t, offs, sorted, bx, by, r, px, py, R, best, bval);
     }
 }
 
@@ -369,19 +596,22 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
 {
     const Region g = region_of(t, ti, tj);
     const int RW = g.rk1 - g.rk0 + 1, RH = g.rl1 - g.rl0 + 1;
-    uint32_t *s_d2 = smem;
-    int32_t *s_val = reinterpret_cast<int32_t *>(smem + RW * RH);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveStage &ws = reinterpret_cast<WaveStage *>(smem)[wave];
+    uint32_t *s_d2 = smem + kWaves * (sizeof(WaveStage) / 4);
+    int32_t *s_val = reinterpret_cast<int32_t *>(s_d2 + RW * RH);
 
     // 1. discrete Voronoi of the block's pixels, 8x8 pixels per wave
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nbx8 = (RW + 7) >> 3, nby8 = (RH + 7) >> 3;
     for (int blk = wave; blk < nbx8 * nby8; blk += kWaves) {
-        const int k = g.rk0 + (blk % nbx8) * 8 + (lane & 7);
-        const int l = g.rl0 + (blk / nbx8) * 8 + (lane >> 3);
-        if (k > g.rk1 || l > g.rl1) continue;
+        const int kA = g.rk0 + (blk % nbx8) * 8, lA = g.rl0 + (blk / nbx8) * 8;
+        const int kB = min(kA + 7, g.rk1), lB = min(lA + 7, g.rl1);
+        const int k = kA + (lane & 7), l = lA + (lane >> 3);
+        const bool active = k <= kB && l <= lB;
         uint64_t best;
         int bval;
-        label_pixel<ALG>(t, offs, edist, sorted, k, l, best, bval);
+        label_block<ALG>(t, offs, edist, sorted, ws, kA, lA, kB, lB, k, l, active, best, bval);
+        if (!active) continue;
         const uint32_t d2 = (uint32_t)(best >> 32);
         s_d2[(l - g.rl0) * RW + (k - g.rk0)] = d2;
         s_val[(l - g.rl0) * RW + (k - g.rk0)] = bval;
@@ -423,7 +653,7 @@ __global__ void __launch_bounds__(kBlock)
 k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
         const uint8_t *__restrict__ edist, const int4 *__restrict__ sorted)
 {
-    extern __shared__ uint32_t smem[];
+    extern __shared__ __align__(16) uint32_t smem[];
     __shared__ uint32_t s_item;
     const int64_t n_items = b->n_items;
     const int n_tiles = b->n_tiles;
@@ -439,7 +669,9 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
             if (tiles[mid].item_base <= item) lo = mid;
             else hi = mid - 1;
         }
-        const TileDev &t = tiles[lo];
+        // a private copy: loaded once per item, never re-read after the
+        // kernel's own stores
+        const TileDev t = tiles[lo];
         const int64_t local = item - t.item_base;
         const int tj = (int)(local / t.ntx), ti = (int)(local % t.ntx);
         process_block<ALG, MODE>(t, ti, tj, smem, offs, edist, sorted);
@@ -491,6 +723,8 @@ hipError_t enqueue_pipeline(const DevBuffers &d, const LaunchCfg &cfg, hipStream
     if (ev1 && (e = hipEventRecord(ev1, s)) != hipSuccess) return e;
     return hipSuccess;
 }
+
+size_t tile_kernel_stage_bytes() { return (size_t)kWaves * sizeof(WaveStage); }
 
 int tile_kernel_occupancy(size_t lds_bytes)
 {
