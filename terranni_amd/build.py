@@ -45,12 +45,40 @@ def build_lib(verbose: bool = False) -> str:
     return out
 
 
+CXX = os.environ.get("CXX", "g++")
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra"]
+
+
+def build_nni_api(verbose: bool = False) -> str:
+    """libterranni_nni.so: the nni.h API over the engine's C ABI (host C++),
+    and the small driver program the API test runs."""
+    out = os.path.join(LIB_DIR, "libterranni_nni.so")
+    tmp = out + ".tmp"
+    inc = ["-I", os.path.join(ROOT, "include")]
+    _run([CXX, *CXX_FLAGS, "-shared", *inc, os.path.join(CSRC, "nni_shim.cpp"), "-o", tmp,
+          "-L", LIB_DIR, "-lterranni_hip", "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(tmp, out)
+    tdir = os.path.join(ROOT, "tests", "cpp", "_build")
+    os.makedirs(tdir, exist_ok=True)
+    exe = os.path.join(tdir, "nni_api_test")
+    _run([CXX, *CXX_FLAGS, *inc, os.path.join(ROOT, "tests", "cpp", "nni_api_test.cpp"), "-o",
+          exe + ".tmp", "-L", LIB_DIR, "-lterranni_nni", "-lterranni_hip",
+          "-Wl,-rpath," + LIB_DIR, "-Wl,-rpath,$ORIGIN/../../../terranni_amd/_lib"], verbose)
+    os.replace(exe + ".tmp", exe)
+    tp = os.path.join(tdir, "tpie_test")
+    _run([CXX, *CXX_FLAGS, *inc, os.path.join(ROOT, "tests", "cpp", "tpie_test.cpp"), "-o",
+          tp + ".tmp"], verbose)
+    os.replace(tp + ".tmp", tp)
+    return out
+
+
 def build_oracle(verbose: bool = False):
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], verbose)
 
 
 def build_all(verbose: bool = False):
     build_lib(verbose)
+    build_nni_api(verbose)
     build_oracle(verbose)
 
 

@@ -1,0 +1,185 @@
+// nni.h (TerraNNI's per-tile API) implemented on the C ABI of terranni.h.
+//
+// performNNI reads the current time slice's points from the caller's TPIE
+// stream exactly as renderInputPoints does (nni.cpp:741-788: time filter,
+// stop at the first later point, seek back for the next slice), puts them
+// on the engine's fixed-point grid, runs the tile on the GPU and appends the
+// cells to the output stream in the reference's order and filter
+// (nni.cpp:836-860).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nni.h"
+#include "terranni.h"
+
+namespace {
+
+nni::Settings g_settings;
+tnni_ctx *g_ctx = nullptr;
+int g_mode = TNNI_ACC_FIXED64;
+int g_status = TNNI_OK;
+std::string g_err;
+tpie::cpu_timer g_t_read, g_t_tile, g_t_write;
+
+void set_error(int st, const std::string &msg)
+{
+    g_status = st;
+    g_err = msg;
+    std::fprintf(stderr, "nni: %s\n", msg.c_str());
+}
+
+int32_t clamp_i32(long long v)
+{
+    return (int32_t)std::max<long long>(INT32_MIN, std::min<long long>(INT32_MAX, v));
+}
+
+}  // namespace
+
+namespace nni {
+
+int init(const Settings &s)
+{
+    g_settings = s;
+    g_status = TNNI_OK;
+    g_err.clear();
+    if (const char *m = std::getenv("TERRANNI_ACCUM")) {
+        if (!std::strcmp(m, "fp32")) g_mode = TNNI_ACC_FP32_2SUM;
+        else if (!std::strcmp(m, "s0") || !std::strcmp(m, "reference")) g_mode = TNNI_ACC_S0;
+        else g_mode = TNNI_ACC_FIXED64;
+    }
+    if (!g_ctx) {
+        tnni_ctx_opts o;
+        std::memset(&o, 0, sizeof o);
+        if (const char *dev = std::getenv("TERRANNI_DEVICE")) o.device = std::atoi(dev);
+        const tnni_status st = tnni_ctx_create(&o, &g_ctx);
+        if (st != TNNI_OK) {
+            set_error(st, std::string("engine initialisation failed: ") + tnni_last_error(nullptr));
+            g_ctx = nullptr;
+            return 0;
+        }
+    }
+    // largest raster side a tile may use (the GL texture limit of the
+    // reference, nni.cpp:591-597); the driver caps work packages itself
+    return 1 << 16;
+}
+
+void cleanup()
+{
+    if (g_ctx) tnni_ctx_destroy(g_ctx);
+    g_ctx = nullptr;
+}
+
+void setRenderbuffers(int, int, int) {}
+void setupRun(const WorkPackage &, int, int) {}
+void clearRenderbuffers() {}
+
+void printTimes()
+{
+    std::fprintf(stdout, "Time Results:\n");
+    std::fprintf(stdout, "\tRead site points  : \t\t%f\n", g_t_read.wall_time());
+    std::fprintf(stdout, "\tTile on the GPU   : \t\t%f\n", g_t_tile.wall_time());
+    std::fprintf(stdout, "\tWrite query points: \t\t%f\n", g_t_write.wall_time());
+}
+
+void setAccumulationMode(int mode) { g_mode = mode; }
+int lastStatus() { return g_status; }
+const char *lastError() { return g_err.c_str(); }
+
+void performNNI(const WorkPackage &wp, tpie::ami::stream<OutPoint> &out,
+                tpie::ami::stream<nni::Point4D> &vec, float timeRadius, float curTime,
+                bool useDelta, bool debug)
+{
+    (void)debug;
+    g_status = TNNI_OK;
+    if (!g_ctx) {
+        set_error(TNNI_E_NODEVICE, "performNNI before a successful nni::init");
+        return;
+    }
+    if (timeRadius > 0) {
+        set_error(TNNI_E_INVALID, "space-time interpolation (timeRadius > 0) is not provided by this engine");
+        return;
+    }
+    // 1. the slice's points (renderInputPoints, nni.cpp:741-788; one slice)
+    g_t_read.start();
+    const float t = curTime;
+    TPIE_OS_OFFSET seekto = vec.tell(), finalSeekTo = vec.tell();
+    bool found = false;
+    std::vector<Point4D> pts;
+    Point4D *p = nullptr;
+    while (vec.read_item(&p) == tpie::ami::NO_ERROR) {
+        if (!found && p->t >= t + 1 - timeRadius) {
+            seekto = vec.tell() - 1;
+            finalSeekTo = seekto;
+            found = true;
+        }
+        if (std::fabs(t - p->t) <= timeRadius + 0.000001f) pts.push_back(*p);
+        else if (p->t > t + timeRadius) break;
+    }
+    vec.seek(useDelta ? seekto : finalSeekTo);
+
+    // 2. the tile on the engine's grid: coordinates in its own unit
+    //    relative to the tile origin, heights in millimetres
+    tnni_tile_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.origin_x = (double)wp.origin_x;
+    d.origin_y = (double)wp.origin_y;
+    d.cell_size = g_settings.cell_size;
+    d.scale = g_settings.scale;
+    d.nrows = wp.nrows;
+    d.ncols = wp.ncols;
+    d.site_radius = g_settings.site_radius;
+    d.query_radius = (float)g_settings.query_radius;
+    d.algorithm = g_settings.algorithm;
+    d.accum_mode = g_mode;
+    d.pixel_offset = -1;
+    d.point_min = g_settings.pointMin;
+    d.multiplier = g_settings.multiplier;
+    tnni_frame_info f;
+    tnni_status st = tnni_frame(&d, &f);
+    if (st != TNNI_OK) {
+        g_t_read.stop();
+        set_error(st, tnni_last_error(nullptr));
+        return;
+    }
+    const double upm = (double)f.upm;
+    d.las_scale[0] = d.las_scale[1] = 1.0 / upm;
+    d.las_scale[2] = 1e-3;
+    d.las_offset[0] = d.origin_x;
+    d.las_offset[1] = d.origin_y;
+    d.las_offset[2] = 0.0;
+    const size_t n = pts.size();
+    std::vector<int32_t> X(n), Y(n), Z(n);
+    for (size_t i = 0; i < n; ++i) {
+        X[i] = clamp_i32(std::llround((pts[i].p[0] - d.origin_x) * upm));
+        Y[i] = clamp_i32(std::llround((pts[i].p[1] - d.origin_y) * upm));
+        Z[i] = clamp_i32(std::llround(pts[i].p[2] * 1e3));
+    }
+    g_t_read.stop();
+
+    g_t_tile.start();
+    std::vector<float> h((size_t)wp.nrows * wp.ncols);
+    st = tnni_tile_run(g_ctx, &d, X.data(), Y.data(), Z.data(), n, h.data(), nullptr, nullptr);
+    g_t_tile.stop();
+    if (st != TNNI_OK) {
+        set_error(st, tnni_last_error(g_ctx));
+        return;
+    }
+
+    // 3. the cells, column by column, inside the total grid (nni.cpp:836-860)
+    g_t_write.start();
+    for (int i = 0; i < wp.ncols; ++i) {
+        for (int j = 0; j < wp.nrows; ++j) {
+            const OutPoint o((int)((float)i + wp.grid_origin_x), (int)((float)j + wp.grid_origin_y),
+                             h[(size_t)j * wp.ncols + i]);
+            if (o.i < g_settings.ncols && o.j < g_settings.nrows) out.write_item(o);
+        }
+    }
+    g_t_write.stop();
+}
+
+}  // namespace nni

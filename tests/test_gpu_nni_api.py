@@ -1,0 +1,71 @@
+"""The reference's C++ API (nni.h) end to end on the GPU: a C++ program
+feeds a time-sorted TPIE stream of Point4D to nni::performNNI slice by slice,
+as terranni.cpp's binRun does, and the OutPoints it appends are compared
+with the oracle on the same (quantised) points."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import terranni_amd as T
+import terranni_amd.synth as S
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "_build", "nni_api_test")
+P4 = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("t", "<i4"), ("pad", "<i4")])
+OUT = np.dtype([("i", "<i4"), ("j", "<i4"), ("h", "<f4")])
+
+
+def _run_api(tmp_path, pts, args):
+    pin, pout = tmp_path / "pts.bin", tmp_path / "out.bin"
+    pts.tofile(pin)
+    cmd = [EXE, str(pin), str(pout)] + [f"{k}={v}" for k, v in args.items()]
+    print("run", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return np.fromfile(pout, dtype=OUT)
+
+
+@pytest.mark.parametrize("grid,n,cs,scale,slices,gox", [
+    (64, 3000, 0.5, 1, 1, 0), (40, 2000, 2.0, 5, 3, 0), (50, 4000, 1.0, 1, 2, 7)])
+def test_perform_nni_matches_oracle(tmp_path, grid, n, cs, scale, slices, gox):
+    tile = S.make_tile(40 + grid, n, grid, cell_size=cs, scale=scale, kind="clustered",
+                       origin=(1000.0, 2000.0))
+    p = tile.params
+    x = tile.X * 0.01 + p.las_offset[0]
+    y = tile.Y * 0.01 + p.las_offset[1]
+    z = tile.Z * 0.01
+    rng = np.random.default_rng(grid)
+    t = np.sort(rng.integers(0, slices, tile.n)).astype(np.int32)  # time-sorted stream
+    pts = np.zeros(tile.n, P4)
+    pts["x"], pts["y"], pts["z"], pts["t"] = x, y, z, t
+    total = grid + gox - 3  # the total grid clips the tile's last columns/rows
+    args = dict(scale=scale, cell_size=cs, nrows=total, ncols=total, origin_x=p.origin_x,
+                origin_y=p.origin_y, grid_origin_x=gox, grid_origin_y=gox, wp_nrows=grid,
+                wp_ncols=grid, slices=slices)
+    out = _run_api(tmp_path, pts, args)
+    f = T.frame(p)
+    upm = f["upm"]
+    exp = []
+    for s in range(slices):
+        sel = t == s
+        X = np.rint((x[sel] - p.origin_x) * upm).astype(np.int32)
+        Y = np.rint((y[sel] - p.origin_y) * upm).astype(np.int32)
+        Z = np.rint(z[sel] * 1e3).astype(np.int32)
+        q = T.TileParams(nrows=grid, ncols=grid, cell_size=cs, scale=scale,
+                         origin_x=p.origin_x, origin_y=p.origin_y,
+                         las_scale=(1.0 / upm, 1.0 / upm, 1e-3),
+                         las_offset=(p.origin_x, p.origin_y, 0.0))
+        ref = O.run(q, X, Y, Z)["h_fixed"]
+        for i in range(grid):          # column-major, as nni.cpp:836-860
+            for j in range(grid):
+                if i + gox < total and j + gox < total:
+                    exp.append((i + gox, j + gox, ref[j, i]))
+    exp = np.array(exp, dtype=OUT)
+    assert out.shape == exp.shape
+    assert np.array_equal(out["i"], exp["i"]) and np.array_equal(out["j"], exp["j"])
+    assert np.array_equal(out["h"].view(np.uint32), exp["h"].view(np.uint32))
