@@ -488,24 +488,23 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
         staged = total <= (uint32_t)kStageCap;
         if (staged) {
             wave_sync();
-            // every load in flight before the first store
-            int4 v[kStageCap / 64];
-#pragma unroll
+            // direct global -> LDS loads (no VGPRs): lane L of step q writes
+            // slot 64q + L; all steps in flight before the one wait
             for (int q = 0; q < kStageCap / 64; ++q) {
+                if (64u * q >= total) break;
                 const uint32_t i = lane + 64u * q;
                 if (i < total) {
                     int j = 0;
 #pragma unroll
                     for (int jj = 1; jj < kWinMax; ++jj)
                         j += (jj < wh && ws.rowpre[jj] <= i) ? 1 : 0;
-                    v[q] = sorted[ws.woff[j * wp] + (i - ws.rowpre[j])];
+                    const int4 *src = sorted + (ws.woff[j * wp] + (i - ws.rowpre[j]));
+                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                                     (__attribute__((address_space(3))) void *)(ws.sites + 64 * q),
+                                                     16, 0, 0);
                 }
             }
-#pragma unroll
-            for (int q = 0; q < kStageCap / 64; ++q) {
-                const uint32_t i = lane + 64u * q;
-                if (i < total) ws.sites[i] = v[q];
-            }
+            __builtin_amdgcn_s_waitcnt(0);
         }
         wave_sync();
     }
