@@ -43,7 +43,7 @@ struct HostBuf {
 
 // Everything a captured pipeline graph depends on.
 struct GraphKey {
-    const void *ptrs[9];
+    const void *ptrs[10];
     size_t lds;
     int alg, mode, main_grid;
     bool operator==(const GraphKey &o) const
@@ -56,7 +56,7 @@ struct GraphKey {
 struct Arena {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    DevBuf tiles, chunks, batch, counts, rank, partials, sorted, hdist, edist;
+    DevBuf tiles, chunks, batch, hist, partials, banded, sorted, offs, hdist, edist;
     DevBuf sx, sy, sz, sh, sl, sd;  // device staging of host-pointer calls
     HostBuf h_desc;                 // pinned descriptors
     HostBuf h_in, h_out;            // pinned staging of the served form
@@ -67,8 +67,8 @@ struct Arena {
 
 // Fixed launch grids of graph-captured pipelines: the kernels read every
 // size from device memory and stride over their work.
-constexpr int kGraphFillGrid = 1024;
 constexpr int kGraphSortGrid = 2048;
+constexpr int kGraphBandGrid = 2048;
 constexpr int kGraphRowGrid = 2048;
 
 struct Request {
@@ -210,8 +210,8 @@ tnni_status arena_init(tnni_ctx *ctx, Arena &a)
 void arena_free(tnni_ctx *ctx, Arena &a)
 {
     if (a.stream) (void)hipStreamSynchronize(a.stream);
-    DevBuf *bufs[] = {&a.tiles, &a.chunks, &a.batch, &a.counts, &a.rank, &a.partials, &a.sorted,
-                      &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd};
+    DevBuf *bufs[] = {&a.tiles, &a.chunks, &a.batch, &a.hist, &a.partials, &a.banded, &a.sorted,
+                      &a.offs, &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd};
     for (DevBuf *b : bufs) release(ctx, *b);
     HostBuf *hb[] = {&a.h_desc, &a.h_in, &a.h_out};
     for (HostBuf *b : hb) {
@@ -243,7 +243,7 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
 {
     if (tiles.empty()) return TNNI_OK;
     const int alg = tiles[0].t.algorithm, mode = tiles[0].t.mode;
-    int64_t pts = 0, bks = 0, items = 0, rows = 0;
+    int64_t pts = 0, bks = 0, items = 0, rows = 0, bands = 0, hents = 0;
     size_t lds = 0;
     std::vector<Chunk> chunks;
     for (size_t k = 0; k < tiles.size(); ++k) {
@@ -254,7 +254,11 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
         t.bk_base = bks;
         t.item_base = items;
         t.row_base = rows;
-        for (int64_t b = 0; b < t.n; b += kChunkPts) {
+        t.band_base = bands;
+        t.hist_base = hents;
+        t.chunk_base = (int)chunks.size();
+        // every tile has at least one (possibly empty) chunk
+        for (int64_t b = 0; b < std::max<int64_t>(t.n, 1); b += kChunkPts) {
             Chunk c;
             c.tile = (int)k;
             c.pad = 0;
@@ -262,23 +266,26 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
             c.end = std::min<int64_t>(t.n, b + kChunkPts);
             chunks.push_back(c);
         }
+        t.nchunks = (int)chunks.size() - t.chunk_base;
         pts += t.n;
         bks += (int64_t)t.nbx * t.nby;
         rows += t.nby;
+        bands += t.nbands;
+        hents += (int64_t)t.nbands * t.nchunks;
         items += (int64_t)t.ntx * t.nty;
         lds = std::max(lds, tiles[k].lds);
     }
-    if (bks + 1 > 0xFFFFFFFFll || pts > 0xFFFFFFFFll)
+    if (bks + 1 > 0xFFFFFFFFll || pts > 0xFFFFFFFFll || hents + 1 > 0xFFFFFFFFll)
         return fail(ctx, TNNI_E_INVALID, "batch exceeds 2^32 buckets or points");
-    if (chunks.empty()) chunks.push_back(Chunk{0, 0, 0, 0});
     tnni_status st;
     if ((st = ensure(ctx, a.tiles, tiles.size() * sizeof(TileDev))) != TNNI_OK) return st;
     if ((st = ensure(ctx, a.chunks, chunks.size() * sizeof(Chunk))) != TNNI_OK) return st;
     if ((st = ensure(ctx, a.batch, sizeof(BatchDev))) != TNNI_OK) return st;
-    if ((st = ensure(ctx, a.counts, (size_t)(bks + 1) * 4)) != TNNI_OK) return st;
-    if ((st = ensure(ctx, a.rank, (size_t)pts * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.hist, (size_t)(hents + 1) * 4)) != TNNI_OK) return st;
     if ((st = ensure(ctx, a.partials, kScanBlocks * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.banded, (size_t)pts * 16)) != TNNI_OK) return st;
     if ((st = ensure(ctx, a.sorted, (size_t)pts * 16)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.offs, (size_t)(bks + 1) * 4)) != TNNI_OK) return st;
     if ((st = ensure(ctx, a.hdist, (size_t)bks)) != TNNI_OK) return st;
     if ((st = ensure(ctx, a.edist, (size_t)bks)) != TNNI_OK) return st;
 
@@ -286,9 +293,10 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     std::memset(&bd, 0, sizeof bd);
     bd.n_tiles = (int)tiles.size();
     bd.n_chunks = (int)chunks.size();
-    bd.n_scan = (uint32_t)(bks + 1);
+    bd.n_scan = (uint32_t)(hents + 1);
     bd.n_items = items;
     bd.n_rows = rows;
+    bd.n_bands = bands;
     bd.scan_chunk = (uint32_t)((bd.n_scan + kScanBlocks - 1) / kScanBlocks);
 
     // descriptors through one pinned staging block; it may still feed the
@@ -314,28 +322,29 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     db.batch = (BatchDev *)a.batch.p;
     db.tiles = (const TileDev *)a.tiles.p;
     db.chunks = (const Chunk *)a.chunks.p;
-    db.counts = (uint32_t *)a.counts.p;
-    db.rank = (uint32_t *)a.rank.p;
+    db.hist = (uint32_t *)a.hist.p;
     db.partials = (uint32_t *)a.partials.p;
+    db.banded = (int4 *)a.banded.p;
     db.sorted = (int4 *)a.sorted.p;
+    db.offs = (uint32_t *)a.offs.p;
     db.hdist = (uint8_t *)a.hdist.p;
     db.edist = (uint8_t *)a.edist.p;
     const bool prof = ctx->prof && how == Launch::Direct;
     if (how == Launch::Direct) {
-        cfg.fill_grid = (int)std::min<int64_t>(1024, (bks + 1 + kBlock - 1) / kBlock);
         cfg.sort_grid = (int)std::min<size_t>(4096, chunks.size());
+        cfg.band_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, bands));
         cfg.row_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, rows));
         cfg.main_grid = (int)std::max<int64_t>(1, std::min<int64_t>(items, (int64_t)ctx->n_cu * occ));
         HIP_TRY(ctx, enqueue_pipeline(db, cfg, a.stream, prof ? ctx->ev0 : nullptr,
                                       prof ? ctx->ev1 : nullptr));
     } else {
-        cfg.fill_grid = kGraphFillGrid;
         cfg.sort_grid = kGraphSortGrid;
+        cfg.band_grid = kGraphBandGrid;
         cfg.row_grid = kGraphRowGrid;
         cfg.main_grid = ctx->n_cu * occ;
         GraphKey key;
-        const void *p[9] = {a.batch.p, a.tiles.p, a.chunks.p, a.counts.p, a.rank.p,
-                            a.partials.p, a.sorted.p, a.hdist.p, a.edist.p};
+        const void *p[10] = {a.batch.p, a.tiles.p, a.chunks.p, a.hist.p, a.partials.p,
+                             a.banded.p, a.sorted.p, a.offs.p, a.hdist.p, a.edist.p};
         std::memcpy(key.ptrs, p, sizeof p);
         key.lds = lds;
         key.alg = alg;
@@ -383,7 +392,7 @@ double tile_cost(const TileDev &t) { return (double)t.W * t.H + 8.0 * (double)t.
 uint64_t request_bytes(const TileDev &t, bool labels, bool d2)
 {
     const uint64_t cells = (uint64_t)t.nrows * t.ncols, pix = (uint64_t)t.W * t.H;
-    uint64_t b = (uint64_t)t.n * (12 + 4 + 16) + ((uint64_t)t.nbx * t.nby + 1) * 6 + cells * 4;
+    uint64_t b = (uint64_t)t.n * (12 + 16 + 16) + ((uint64_t)t.nbx * t.nby + 1) * 6 + cells * 4;
     if (labels) b += pix * 4;
     if (d2) b += pix * 4;
     return b + sizeof(TileDev) + ((uint64_t)t.n / kChunkPts + 1) * sizeof(Chunk);
@@ -399,8 +408,8 @@ void finish_request(tnni_ctx *ctx, const std::shared_ptr<Request> &r, tnni_statu
 // Drop the cached device buffers of an idle arena (its graph goes with them).
 void arena_trim(tnni_ctx *ctx, Arena &a)
 {
-    DevBuf *bufs[] = {&a.tiles, &a.chunks, &a.batch, &a.counts, &a.rank, &a.partials, &a.sorted,
-                      &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd};
+    DevBuf *bufs[] = {&a.tiles, &a.chunks, &a.batch, &a.hist, &a.partials, &a.banded, &a.sorted,
+                      &a.offs, &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd};
     for (DevBuf *b : bufs) release(ctx, *b);
     if (a.gexec) (void)hipGraphExecDestroy(a.gexec);
     a.gexec = nullptr;

@@ -159,6 +159,12 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
         bm = per_px > 0 ? (int)std::lround(std::sqrt(1.0 / per_px)) : 64;
     }
     bm = std::max(1, std::min(bm, f.Rsb / f.ps_u + 1));
+    // the band sort keeps one bucket row (and the bands of a tile) in LDS
+    {
+        const int64_t span = (int64_t)std::max(f.W, f.H) * f.ps_u + 2 * (int64_t)f.Rsb + f.ps_u;
+        const int64_t need = (span + (int64_t)kBandBuckets * f.ps_u - 1) / ((int64_t)kBandBuckets * f.ps_u);
+        bm = std::max<int>(bm, (int)need);
+    }
     const int64_t bs = (int64_t)bm * f.ps_u;
     const int64_t half = f.ps_u / 2;
     const int64_t K = f.Rsb > half ? (f.Rsb - half + bs - 1) / bs : 0;
@@ -178,6 +184,13 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
     t.bm = bm;
     t.bK = (int)K;
     t.ecap = (int)std::min<int64_t>(250, f.Rsb / bs + 2);
+    t.br = (int)std::max<int64_t>(1, std::min<int64_t>(nby, kBandBuckets / nbx));
+    t.nbands = (int)((nby + t.br - 1) / t.br);
+    t.inv_br = 1.0 / (double)t.br;
+    if (nbx > kBandBuckets || t.nbands > kMaxBands) {
+        err = "bucket grid too large for the band sort: raise bucket_px";
+        return TNNI_E_GEOMETRY;
+    }
     t.bm_magic = bm == 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)bm - 1) / (uint64_t)bm);
     t.inv_bs = 1.0 / (double)bs;
     // A site of ring 0 or 1 around a pixel's bucket is within 2*bs of the

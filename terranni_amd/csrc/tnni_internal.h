@@ -16,6 +16,8 @@ constexpr int kChunkPts = 4096;        // points per chunk of the sort passes
 constexpr uint32_t kNoD2 = 0xFFFFFFFFu;
 constexpr int kScanBlocks = 512;       // fixed grid of the device-wide scan
 constexpr int kScanItems = 4;          // items per thread per scan step
+constexpr int kMaxBands = 8192;        // bucket-row bands per tile (LDS histogram)
+constexpr int kBandBuckets = 8192;     // buckets per band (LDS counters)
 
 // Everything a kernel needs to know about one tile of a batch.  Lengths are
 // in the tile-local fixed-point unit (1/upm metre).
@@ -42,6 +44,11 @@ struct TileDev {
     double inv_bs;              // 1 / bs
     int64_t bk_base;            // first bucket of this tile in the batch
     int64_t row_base;           // first bucket row of this tile in the batch
+    int br, nbands;             // bucket rows per band, bands
+    double inv_br;              // 1 / br
+    int64_t band_base;          // first band of this tile in the batch
+    int64_t hist_base;          // first band-histogram entry of this tile
+    int chunk_base, nchunks;    // this tile's chunks of points
     int tc, ntx, nty;           // cell-tile side and tiles along x / y
     int pad0;
     int64_t item_base;          // first work item of this tile
@@ -60,10 +67,11 @@ struct Chunk {
 struct BatchDev {
     int n_tiles;
     int n_chunks;
-    uint32_t n_scan;       // buckets of the whole batch + 1
+    uint32_t n_scan;       // band-histogram entries of the whole batch + 1
     uint32_t work_counter; // next work item of the persistent kernel
     int64_t n_items;
     int64_t n_rows;        // bucket rows of the whole batch
+    int64_t n_bands;       // bands of the whole batch
     uint32_t scan_chunk;   // elements per scan block
     uint32_t pad;
 };
@@ -116,8 +124,8 @@ __host__ __device__ inline Region region_of(const TileDev &t, int ti, int tj)
 
 // Host-side launch helpers (tnni_kernels.hip).
 struct LaunchCfg {
-    int fill_grid;      // blocks of the zero pass
-    int sort_grid;      // blocks of the count/scatter passes
+    int sort_grid;      // blocks of the band count/scatter passes
+    int band_grid;      // blocks of the band sort
     int main_grid;      // blocks of the persistent tile kernel
     int row_grid;       // blocks of the empty-distance passes
     size_t main_lds;    // dynamic LDS bytes of the tile kernel
@@ -129,10 +137,11 @@ struct DevBuffers {
     BatchDev *batch;
     const TileDev *tiles;
     const Chunk *chunks;
-    uint32_t *counts;   // n_scan entries, scanned in place into offsets
-    uint32_t *rank;     // one per point
+    uint32_t *hist;     // n_scan band-histogram entries, scanned in place
     uint32_t *partials; // kScanBlocks
-    int4 *sorted;       // one per point
+    int4 *banded;       // one per point: sites in band order
+    int4 *sorted;       // one per point: sites in bucket order
+    uint32_t *offs;     // first site of every bucket, + 1 sentinel
     uint8_t *hdist;     // per bucket: distance to a non-empty bucket in its row
     uint8_t *edist;     // per bucket: Chebyshev distance to a non-empty bucket
 };

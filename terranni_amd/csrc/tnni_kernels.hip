@@ -1,12 +1,15 @@
 // gfx950 kernels of the discrete-Sibson tile engine.
 //
 // Pipeline for a batch of tiles (a single tile is a batch of one):
-//   k_zero      clear the bucket counters and the work counter
-//   k_count     FP64 de-scale of every LAS point into tile-local units,
-//               bucket histogram (one atomic per point, its return value is
-//               the point's rank inside the bucket)
-//   k_scan_*    device-wide exclusive scan of the counters -> bucket offsets
-//   k_scatter   place every point's site record at offset + rank
+//   k_band_count    FP64 de-scale of every LAS point into tile-local units;
+//                   per chunk of points an LDS histogram over bands of
+//                   bucket rows, written band-major
+//   k_scan_*        device-wide exclusive scan of those histograms
+//   k_band_scatter  every chunk places its sites in band order
+//   k_band_sort     per band: LDS counting sort into buckets, the bucket
+//                   offsets, and each bucket's distance to a non-empty one
+//                   along its row
+//   k_edist_cols    the same distance across rows (Chebyshev, capped)
 //   k_tiles     persistent kernel over (tile, cell-block) work items: the
 //               discrete Voronoi of the block's pixels (plus the query halo)
 //               into LDS, then the query gather out of LDS, then the divide.
@@ -60,75 +63,91 @@ __device__ __forceinline__ uint32_t bucket_of(const TileDev &t, int xu, int yu)
     return (uint32_t)by * (uint32_t)t.nbx + (uint32_t)bx;
 }
 
-__global__ void __launch_bounds__(kBlock) k_zero(BatchDev *b, uint32_t *counts)
+// Bucket row band of a kept site: bands of br bucket rows.
+__device__ __forceinline__ int point_band(const TileDev &t, int yu)
 {
-    const uint32_t n = b->n_scan;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-        counts[i] = 0u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) b->work_counter = 0u;
+    return div_pos(div_pos(yu - t.gy0, t.bs, t.inv_bs), t.br, t.inv_br);
 }
 
-constexpr int kPtsPerStep = 4;  // points per thread with their memory operations in flight
+constexpr int kPtsPerStep = 4;  // points per thread with their loads in flight
 
+// Pass 1 of the site sort: per chunk of one tile, the number of its sites in
+// each band of the tile, written band-major (band, chunk) so that one scan
+// gives every chunk its first slot in every band.
 __global__ void __launch_bounds__(kBlock)
-k_count(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
-        uint32_t *counts, uint32_t *rank)
+k_band_count(BatchDev *b, const TileDev *tiles, const Chunk *chunks, uint32_t *hist)
 {
+    __shared__ uint32_t h[kMaxBands];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        b->work_counter = 0u;
+        hist[b->n_scan - 1] = 0u;  // the scan's total lands here
+    }
     const int nch = b->n_chunks;
     for (int c = blockIdx.x; c < nch; c += gridDim.x) {
         const Chunk ch = chunks[c];
         const TileDev &t = tiles[ch.tile];
+        const int nb = t.nbands;
+        for (int i = threadIdx.x; i < nb; i += kBlock) h[i] = 0u;
+        __syncthreads();
         for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kBlock * kPtsPerStep) {
-            uint32_t bk[kPtsPerStep], r[kPtsPerStep];
+            int32_t X[kPtsPerStep], Y[kPtsPerStep];
+#pragma unroll
+            for (int q = 0; q < kPtsPerStep; ++q) {
+                const int64_t i = i0 + (int64_t)q * kBlock;
+                X[q] = i < ch.end ? t.X[i] : 0;
+                Y[q] = i < ch.end ? t.Y[i] : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < kPtsPerStep; ++q) {
+                int xu, yu;
+                if (i0 + (int64_t)q * kBlock < ch.end && point_units(t, X[q], Y[q], xu, yu))
+                    atomicAdd(&h[point_band(t, yu)], 1u);
+            }
+        }
+        __syncthreads();
+        const int64_t col = c - t.chunk_base;
+        for (int i = threadIdx.x; i < nb; i += kBlock)
+            hist[t.hist_base + (int64_t)i * t.nchunks + col] = h[i];
+        __syncthreads();
+    }
+}
+
+// Pass 2: every chunk places its sites in band order (LDS cursors seeded
+// from the scanned band-major histogram).
+__global__ void __launch_bounds__(kBlock)
+k_band_scatter(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
+               const uint32_t *__restrict__ hist, int4 *__restrict__ banded)
+{
+    __shared__ uint32_t cur[kMaxBands];
+    const int nch = b->n_chunks;
+    for (int c = blockIdx.x; c < nch; c += gridDim.x) {
+        const Chunk ch = chunks[c];
+        const TileDev &t = tiles[ch.tile];
+        const int nb = t.nbands;
+        const int64_t col = c - t.chunk_base;
+        for (int i = threadIdx.x; i < nb; i += kBlock)
+            cur[i] = hist[t.hist_base + (int64_t)i * t.nchunks + col];
+        __syncthreads();
+        for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kBlock * kPtsPerStep) {
+            int32_t X[kPtsPerStep], Y[kPtsPerStep], Z[kPtsPerStep];
+#pragma unroll
+            for (int q = 0; q < kPtsPerStep; ++q) {
+                const int64_t i = i0 + (int64_t)q * kBlock;
+                X[q] = i < ch.end ? t.X[i] : 0;
+                Y[q] = i < ch.end ? t.Y[i] : 0;
+                Z[q] = i < ch.end ? t.Z[i] : 0;
+            }
 #pragma unroll
             for (int q = 0; q < kPtsPerStep; ++q) {
                 const int64_t i = i0 + (int64_t)q * kBlock;
                 int xu, yu;
-                bk[q] = kNoD2;
-                if (i < ch.end && point_units(t, t.X[i], t.Y[i], xu, yu))
-                    bk[q] = (uint32_t)t.bk_base + bucket_of(t, xu, yu);
-            }
-#pragma unroll
-            for (int q = 0; q < kPtsPerStep; ++q)
-                r[q] = bk[q] != kNoD2 ? atomicAdd(&counts[bk[q]], 1u) : kNoD2;
-#pragma unroll
-            for (int q = 0; q < kPtsPerStep; ++q) {
-                const int64_t i = i0 + (int64_t)q * kBlock;
-                if (i < ch.end) rank[t.pt_base + i] = r[q];
-            }
-        }
-    }
-}
-
-__global__ void __launch_bounds__(kBlock)
-k_scatter(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
-          const uint32_t *offsets, const uint32_t *rank, int4 *sorted)
-{
-    const int nch = b->n_chunks;
-    for (int c = blockIdx.x; c < nch; c += gridDim.x) {
-        const Chunk ch = chunks[c];
-        const TileDev &t = tiles[ch.tile];
-        for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kBlock * kPtsPerStep) {
-            int4 rec[kPtsPerStep];
-            uint32_t bk[kPtsPerStep], r[kPtsPerStep], pos[kPtsPerStep];
-#pragma unroll
-            for (int q = 0; q < kPtsPerStep; ++q) {
-                const int64_t i = i0 + (int64_t)q * kBlock;
-                r[q] = i < ch.end ? rank[t.pt_base + i] : kNoD2;
-                bk[q] = 0;
-                if (r[q] != kNoD2) {
-                    int xu, yu;
-                    point_units(t, t.X[i], t.Y[i], xu, yu);
-                    bk[q] = (uint32_t)t.bk_base + bucket_of(t, xu, yu);
-                    rec[q] = make_int4(xu, yu, (int)i, site_value(t, t.Z[i]));
+                if (i < ch.end && point_units(t, X[q], Y[q], xu, yu)) {
+                    const uint32_t pos = atomicAdd(&cur[point_band(t, yu)], 1u);
+                    banded[pos] = make_int4(xu, yu, (int)i, site_value(t, Z[q]));
                 }
             }
-#pragma unroll
-            for (int q = 0; q < kPtsPerStep; ++q) pos[q] = r[q] != kNoD2 ? offsets[bk[q]] + r[q] : 0u;
-#pragma unroll
-            for (int q = 0; q < kPtsPerStep; ++q)
-                if (r[q] != kNoD2) sorted[pos[q]] = rec[q];
         }
+        __syncthreads();
     }
 }
 
@@ -227,40 +246,83 @@ __device__ __forceinline__ int tile_of_row(const TileDev *tiles, int n_tiles, in
     return lo;
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_edist_rows(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
-             uint8_t *__restrict__ hdist)
+__device__ __forceinline__ int tile_of_band(const TileDev *tiles, int n_tiles, int64_t band)
 {
-    const int64_t nr = b->n_rows;
+    int lo = 0, hi = n_tiles - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tiles[mid].band_base <= band) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Pass 3: one workgroup per band sorts the band's sites into its buckets
+// (LDS counters), writes the bucket offsets, and the distance of every
+// bucket to the nearest bucket holding a site in its own row (hdist).
+__global__ void __launch_bounds__(kBlock)
+k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ hist,
+            const int4 *__restrict__ banded, int4 *__restrict__ sorted, uint32_t *__restrict__ offs,
+            uint8_t *__restrict__ hdist)
+{
+    __shared__ uint32_t cnt[kBandBuckets];
+    __shared__ uint32_t s_w[kWaves];
+    const int64_t n_bands = b->n_bands;
     const int n_tiles = b->n_tiles;
-    for (int64_t item = blockIdx.x; item < nr; item += gridDim.x) {
-        const TileDev &t = tiles[tile_of_row(tiles, n_tiles, item)];
-        const int64_t rb = t.bk_base + (item - t.row_base) * t.nbx;
-        const uint32_t *o = offs + rb;
-        const int nbx = t.nbx, cap = t.ecap;
-        for (int x = threadIdx.x; x < nbx; x += kBlock) {
-            // bucket b holds sites iff o[b + 1] > o[b]; four distances per
-            // step, their offsets loaded up front
-            int d = cap;
-            for (int q0 = 0; q0 < cap; q0 += 4) {
-                uint32_t lo[5], hi[5];
-#pragma unroll
-                for (int q = 0; q < 5; ++q) {
-                    const int a = x - q0 - q + 1, c = x + q0 + q;
-                    lo[q] = (a >= 0) ? o[a] : 0u;   // o[x - dq + 1] for dq = q0 + q
-                    hi[q] = (c <= nbx) ? o[c] : 0u; // o[x + dq]
-                }
-#pragma unroll
-                for (int q = 3; q >= 0; --q) {
-                    const int dq = q0 + q;
-                    const bool l = x - dq >= 0 && lo[q] > lo[q + 1];   // bucket x-dq
-                    const bool r = x + dq < nbx && hi[q + 1] > hi[q];  // bucket x+dq
-                    if ((l || r) && dq < cap) d = min(d, dq);
-                }
-                if (d < cap) break;
-            }
-            hdist[rb + x] = (uint8_t)d;
+    for (int64_t g = blockIdx.x; g < n_bands; g += gridDim.x) {
+        const TileDev &t = tiles[tile_of_band(tiles, n_tiles, g)];
+        const int band = (int)(g - t.band_base);
+        const uint32_t s0 = hist[t.hist_base + (int64_t)band * t.nchunks];
+        const uint32_t s1 = hist[t.hist_base + (int64_t)(band + 1) * t.nchunks];
+        const int nbx = t.nbx, r0 = band * t.br, r1 = min(r0 + t.br, t.nby);
+        const int nloc = (r1 - r0) * nbx;
+        const int64_t obase = t.bk_base + (int64_t)r0 * nbx;
+        for (int i = threadIdx.x; i < nloc; i += kBlock) cnt[i] = 0u;
+        __syncthreads();
+        for (uint32_t p = s0 + threadIdx.x; p < s1; p += kBlock) {
+            const int4 rec = banded[p];
+            const int bx = div_pos(rec.x - t.gx0, t.bs, t.inv_bs);
+            const int by = div_pos(rec.y - t.gy0, t.bs, t.inv_bs);
+            atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
         }
+        __syncthreads();
+        // row distances to a bucket holding a site, capped at ecap
+        const int cap = t.ecap;
+        for (int i = threadIdx.x; i < nloc; i += kBlock) {
+            const int x = i % nbx;
+            int d = cap;
+            for (int q = 0; q < cap; ++q)
+                if ((x - q >= 0 && cnt[i - q]) || (x + q < nbx && cnt[i + q])) {
+                    d = q;
+                    break;
+                }
+            hdist[obase + i] = (uint8_t)d;
+        }
+        __syncthreads();
+        // exclusive scan of the counters: each thread a contiguous segment
+        const int seg = (nloc + kBlock - 1) / kBlock;
+        const int sb = min(nloc, (int)threadIdx.x * seg), se = min(nloc, sb + seg);
+        uint32_t sum = 0;
+        for (int i = sb; i < se; ++i) sum += cnt[i];
+        uint32_t total;
+        uint32_t run = block_excl_scan<kBlock>(sum, s_w, total);
+        for (int i = sb; i < se; ++i) {
+            const uint32_t v = cnt[i];
+            cnt[i] = run;
+            run += v;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nloc; i += kBlock) offs[obase + i] = s0 + cnt[i];
+        if (r1 == t.nby && threadIdx.x == 0) offs[t.bk_base + (int64_t)t.nby * nbx] = s1;
+        __syncthreads();
+        for (uint32_t p = s0 + threadIdx.x; p < s1; p += kBlock) {
+            const int4 rec = banded[p];
+            const int bx = div_pos(rec.x - t.gx0, t.bs, t.inv_bs);
+            const int by = div_pos(rec.y - t.gy0, t.bs, t.inv_bs);
+            const uint32_t pos = atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
+            sorted[s0 + pos] = rec;
+        }
+        __syncthreads();
     }
 }
 
@@ -682,7 +744,7 @@ template <int ALG, int MODE>
 hipError_t launch_tiles(const DevBuffers &d, const LaunchCfg &cfg, hipStream_t s)
 {
     hipLaunchKernelGGL((k_tiles<ALG, MODE>), dim3(cfg.main_grid), dim3(kBlock), cfg.main_lds, s,
-                       d.batch, d.tiles, d.counts, d.edist, d.sorted);
+                       d.batch, d.tiles, d.offs, d.edist, d.sorted);
     return hipGetLastError();
 }
 
@@ -692,18 +754,17 @@ hipError_t enqueue_pipeline(const DevBuffers &d, const LaunchCfg &cfg, hipStream
                             hipEvent_t ev0, hipEvent_t ev1)
 {
     hipError_t e;
-    hipLaunchKernelGGL(k_zero, dim3(cfg.fill_grid), dim3(kBlock), 0, s, d.batch, d.counts);
-    hipLaunchKernelGGL(k_count, dim3(cfg.sort_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
-                       d.chunks, d.counts, d.rank);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(kScanBlocks), dim3(kBlock), 0, s, d.batch, d.counts,
+    hipLaunchKernelGGL(k_band_count, dim3(cfg.sort_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
+                       d.chunks, d.hist);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(kScanBlocks), dim3(kBlock), 0, s, d.batch, d.hist,
                        d.partials);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanBlocks), 0, s, d.partials);
-    hipLaunchKernelGGL(k_scan_down, dim3(kScanBlocks), dim3(kBlock), 0, s, d.batch, d.counts,
+    hipLaunchKernelGGL(k_scan_down, dim3(kScanBlocks), dim3(kBlock), 0, s, d.batch, d.hist,
                        d.partials);
-    hipLaunchKernelGGL(k_scatter, dim3(cfg.sort_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
-                       d.chunks, d.counts, d.rank, d.sorted);
-    hipLaunchKernelGGL(k_edist_rows, dim3(cfg.row_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
-                       d.counts, d.hdist);
+    hipLaunchKernelGGL(k_band_scatter, dim3(cfg.sort_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
+                       d.chunks, d.hist, d.banded);
+    hipLaunchKernelGGL(k_band_sort, dim3(cfg.band_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
+                       d.hist, d.banded, d.sorted, d.offs, d.hdist);
     hipLaunchKernelGGL(k_edist_cols, dim3(cfg.row_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
                        d.hdist, d.edist);
     if ((e = hipGetLastError()) != hipSuccess) return e;
