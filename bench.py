@@ -97,7 +97,102 @@ def bench_batch(T, S, ctx, reps: int = 3):
         "batched_ms": batch_s * 1e3,
         "speedup": plain_s / batch_s,
         "batched_equals_plain": bool(same),
-    }
+    }, tiles
+
+
+def bench_served(T, tiles, rates, per_rate, resident: bool, seed=4):
+    """Served form: Poisson arrivals at each rate of tiles drawn from the
+    uneven-tile pool; one thread submits, another waits in order.  resident:
+    the tiles' points and the output rasters live in HBM (tnni_submit_dev);
+    otherwise host buffers in and out (tnni_submit).  Latency is submit ->
+    results in device memory (resident) or in the caller's host buffers,
+    timed by the engine."""
+    import threading
+    out = []
+    rng = np.random.default_rng(seed)
+    ring = 192  # output rasters in flight (resident)
+    with T.Context(0, n_slots=3, max_batch_tiles=64) as c:
+        if resident:
+            dev = [(c.upload(t.X), c.upload(t.Y), c.upload(t.Z)) for t in tiles]
+            big = max(t.cells for t in tiles)
+            outs = [c.alloc((big,), np.float32) for _ in range(ring)]
+
+        def submit(k, i):
+            t = tiles[i]
+            if resident:
+                d = dev[i]
+                return c.submit_dev(t.params, d[0], d[1], d[2], t.n, outs[k % ring])
+            return c.submit(t.params, t.X, t.Y, t.Z)
+
+        for p in [submit(k, k) for k in range(16)]:  # warm the slots and graphs
+            p.wait()
+        for rate in rates:
+            picks = rng.integers(0, len(tiles), per_rate)
+            gaps = rng.exponential(1.0 / rate, per_rate)
+            pend, lat = [], []
+            cv = threading.Condition()
+            state = {"waited": 0}
+
+            def waiter():
+                for k in range(per_rate):
+                    with cv:
+                        while len(pend) <= k:
+                            cv.wait()
+                        p = pend[k]
+                    p.wait()
+                    lat.append(p.latency_s)
+                    with cv:
+                        state["waited"] = k + 1
+                        cv.notify_all()
+                state["t_end"] = time.perf_counter()
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            t0 = time.perf_counter()
+            t_next = t0
+            for k in range(per_rate):
+                t_next += gaps[k]
+                d = t_next - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                if resident:
+                    with cv:  # the output raster k % ring is free again
+                        while state["waited"] < k - ring + 1:
+                            cv.wait()
+                p = submit(k, int(picks[k]))
+                with cv:
+                    pend.append(p)
+                    cv.notify_all()
+            th.join()
+            span = state["t_end"] - t0
+            cells = float(sum(tiles[i].cells for i in picks))
+            st = c.serve_stats()
+            out.append({"rate_tiles_per_s": rate, "requests": per_rate,
+                        "achieved_tiles_per_s": per_rate / span, "cells_per_s": cells / span,
+                        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                                       "p99": float(np.percentile(lat, 99) * 1e3)},
+                        "max_inflight_tiles": st["max_inflight_tiles"],
+                        "max_inflight_batches": st["max_inflight_batches"]})
+            log(f"served ({'HBM-resident' if resident else 'host buffers'}) rate {rate}/s: {out[-1]}")
+        if resident:
+            for d in dev:
+                for a in d:
+                    a.free()
+            for o in outs:
+                o.free()
+    return out
+
+
+def load_traffic(kernel="k_tiles<0, 0>"):
+    """HBM bytes per k_tiles launch on this workload, from the committed
+    FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_single_4m.json)."""
+    path = os.path.join(ROOT, "profiles", "traffic_single_4m.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel)
+    return None if k is None else k["bytes"]
 
 
 def main():
@@ -111,6 +206,10 @@ def main():
     ap.add_argument("--tile-cells", type=int, default=0, help="cell-tile side (0 = auto)")
     ap.add_argument("--bucket-px", type=int, default=0, help="bucket side in pixels (0 = auto)")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-rates", default="50,200,500,1000,2000")
+    ap.add_argument("--served-requests", type=int, default=300,
+                    help="requests per arrival rate")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters against known traffic)")
@@ -147,6 +246,11 @@ def main():
     launches, kern_ms = ctx.profile_read(reset=True)
     ctx.profile(False)
     h = dH.to_host()
+    # PCIe-inclusive: host buffers in, heights back to host memory
+    t_h = time.perf_counter()
+    for _ in range(3):
+        ctx.tile_run(tile.params, tile.X, tile.Y, tile.Z)
+    pcie_cells_s = 3 * tile.cells / (time.perf_counter() - t_h)
     if args.copy_probe_mb:
         nb = args.copy_probe_mb << 20
         a, c = ctx.alloc((nb,), np.uint8), ctx.alloc((nb,), np.uint8)
@@ -177,16 +281,27 @@ def main():
                    "points": tile.n, "cells": tile.cells, "pixels": fr["W"] * fr["H"],
                    "form": "plain"},
         "points_per_s": pts_s,
+        "pcie_inclusive_cells_per_s": pcie_cells_s,
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
                      "kernel": "k_tiles", "kernel_ms": k_ms, "alg_bytes": alg_bytes},
     }
+    for d in (dX, dY, dZ, dH):
+        d.free()
+    pool = None
     if not args.no_batch:
-        for d in (dX, dY, dZ, dH):
-            d.free()
-        out["batched_vs_plain"] = bench_batch(T, S, ctx)
+        out["batched_vs_plain"], pool = bench_batch(T, S, ctx)
+    if not args.no_served:
+        if pool is None:
+            pool = S.workload_batch()
+        rates = [float(r) for r in args.served_rates.split(",") if r]
+        out["served"] = {
+            "workload": "Poisson arrivals of tiles drawn from the 256 uneven tiles",
+            "hbm_resident": bench_served(T, pool, rates, args.served_requests, True),
+            "host_buffers": bench_served(T, pool, rates[:2], args.served_requests, False),
+        }
     if not args.no_cpu:
         c_cells, threads, per, reps, ref = cpu_baseline(tile)
         out["cpu_baseline"] = {"value": c_cells, "unit": "cells/s", "cores": threads,

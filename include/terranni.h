@@ -178,11 +178,21 @@ tnni_status tnni_submit(tnni_ctx *ctx, const tnni_tile_desc *desc,
                         const int32_t *X, const int32_t *Y, const int32_t *Z,
                         uint64_t n, float *out_h, int32_t *out_labels,
                         uint32_t *out_d2, uint64_t *ticket);
+/* The same with device-resident inputs and outputs (device pointers that
+ * stay valid until tnni_wait returns): no host copies; latency is submit ->
+ * results in device memory. */
+tnni_status tnni_submit_dev(tnni_ctx *ctx, const tnni_tile_desc *desc,
+                            const int32_t *dX, const int32_t *dY,
+                            const int32_t *dZ, uint64_t n, float *d_out_h,
+                            int32_t *d_out_labels, uint32_t *d_out_d2,
+                            uint64_t *ticket);
 /* TNNI_OK when the result is in host memory, TNNI_PENDING while queued or
  * running, or the request's error. */
 tnni_status tnni_poll(tnni_ctx *ctx, uint64_t ticket);
-/* Block until done; returns the request's status and releases the ticket.
- * latency_s (nullable): submit -> results in host memory. */
+/* Block until done, copy a host request's results into its buffers; returns
+ * the request's status and releases the ticket.  latency_s (nullable):
+ * submit -> results on the host side of the engine (device memory for
+ * tnni_submit_dev). */
 tnni_status tnni_wait(tnni_ctx *ctx, uint64_t ticket, double *latency_s);
 
 typedef struct tnni_serve_stats {

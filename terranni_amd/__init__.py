@@ -171,13 +171,16 @@ def lib():
                                     ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     L.tnni_submit.argtypes = [P, ctypes.POINTER(TileDesc), P, P, P, ctypes.c_uint64, P, P, P,
                               ctypes.POINTER(ctypes.c_uint64)]
+    L.tnni_submit_dev.argtypes = [P, ctypes.POINTER(TileDesc), P, P, P, ctypes.c_uint64, P, P, P,
+                                  ctypes.POINTER(ctypes.c_uint64)]
     L.tnni_poll.argtypes = [P, ctypes.c_uint64]
     L.tnni_wait.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
     L.tnni_serve_stats_read.argtypes = [P, ctypes.POINTER(ServeStats)]
     for name in ("tnni_frame", "tnni_footprint", "tnni_ctx_create", "tnni_tile_run",
                  "tnni_tile_run_dev", "tnni_batch_run_dev", "tnni_dev_alloc", "tnni_dev_free",
                  "tnni_h2d", "tnni_d2h", "tnni_d2d", "tnni_profile_enable", "tnni_profile_read",
-                 "tnni_submit", "tnni_poll", "tnni_wait", "tnni_serve_stats_read"):
+                 "tnni_submit", "tnni_submit_dev", "tnni_poll", "tnni_wait",
+                 "tnni_serve_stats_read"):
         getattr(L, name).restype = ctypes.c_int
     del i32p
     _lib = L
@@ -335,6 +338,17 @@ class Context:
         self._chk(lib().tnni_submit(self.h, ctypes.byref(d), _ptr(X), _ptr(Y), _ptr(Z), X.shape[0],
                                     _ptr(h), _ptr(lab), _ptr(d2), ctypes.byref(t)))
         return Pending(self, t.value, (X, Y, Z), h, lab, d2)
+
+    def submit_dev(self, p: TileParams, dX: "DevArray", dY: "DevArray", dZ: "DevArray", n: int,
+                   d_h: "DevArray", d_lab: "DevArray | None" = None,
+                   d_d2: "DevArray | None" = None):
+        """Queue a tile whose inputs and outputs live in device memory."""
+        d = p.to_desc()
+        t = ctypes.c_uint64()
+        vp = lambda a: ctypes.c_void_p(a.ptr) if a is not None else None  # noqa: E731
+        self._chk(lib().tnni_submit_dev(self.h, ctypes.byref(d), vp(dX), vp(dY), vp(dZ), n,
+                                        vp(d_h), vp(d_lab), vp(d_d2), ctypes.byref(t)))
+        return Pending(self, t.value, (dX, dY, dZ), None, None, None)
 
     def serve_stats(self) -> dict:
         s = ServeStats()

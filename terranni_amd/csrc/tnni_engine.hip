@@ -10,6 +10,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -73,6 +74,7 @@ constexpr int kGraphRowGrid = 2048;
 
 struct Request {
     uint64_t id = 0;
+    bool dev = false;  // device-resident inputs and outputs (tnni_submit_dev)
     const int32_t *X = nullptr, *Y = nullptr, *Z = nullptr;
     uint64_t n = 0;
     float *out_h = nullptr;
@@ -83,8 +85,61 @@ struct Request {
     uint64_t footprint = 0;
     double t_submit = 0, t_done = 0;
     tnni_status status = TNNI_PENDING;
-    // placement inside its batch's staging
+    // host requests: pinned copies of the inputs (X|Y|Z) and of the outputs
+    // (h|labels|d2), filled by tnni_submit and drained by tnni_wait
+    void *pin_in = nullptr, *pin_out = nullptr;
+    size_t pin_in_cap = 0, pin_out_cap = 0;
+    // placement inside its batch's device staging
     size_t in_off = 0, out_h_off = 0, out_l_off = 0, out_d_off = 0;
+};
+
+// Pinned host blocks reused across requests (pinning is slow).
+struct PinnedPool {
+    std::mutex mu;
+    std::multimap<size_t, void *> free;
+    size_t held = 0;
+    static constexpr size_t kMaxHeld = (size_t)8 << 30;
+
+    void *acquire(size_t bytes, size_t &cap)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto it = free.lower_bound(bytes);
+            if (it != free.end() && it->first <= 2 * bytes + (1 << 20)) {
+                void *p = it->second;
+                cap = it->first;
+                held -= cap;
+                free.erase(it);
+                return p;
+            }
+        }
+        size_t want = 1 << 16;
+        while (want < bytes) want <<= 1;
+        void *p = nullptr;
+        if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+        cap = want;
+        return p;
+    }
+    void release(void *p, size_t cap)
+    {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(mu);
+        free.emplace(cap, p);
+        held += cap;
+        while (held > kMaxHeld && !free.empty()) {
+            auto it = std::prev(free.end());
+            (void)hipHostFree(it->second);
+            held -= it->first;
+            free.erase(it);
+        }
+    }
+    void clear()
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto &kv : free) (void)hipHostFree(kv.second);
+        free.clear();
+        held = 0;
+    }
 };
 
 struct Slot {
@@ -130,6 +185,7 @@ struct tnni_ctx {
     uint64_t serve_budget = 0;
     uint64_t inflight_bytes = 0;
     tnni_serve_stats stats{};
+    PinnedPool pinned;
 };
 
 namespace {
@@ -430,6 +486,7 @@ tnni_status launch_batch(tnni_ctx *ctx, Slot &s)
             if (q.get() != &s && !q->busy) arena_trim(ctx, q->a);
     size_t n_in = 0, n_h = 0, n_l = 0, n_d = 0;
     for (auto &r : s.reqs) {
+        if (r->dev) continue;
         r->in_off = n_in;
         n_in += r->n;
         r->out_h_off = n_h;
@@ -446,60 +503,50 @@ tnni_status launch_batch(tnni_ctx *ctx, Slot &s)
     if ((st = ensure(ctx, a.sh, n_h * 4)) != TNNI_OK) return st;
     if ((st = ensure(ctx, a.sl, n_l * 4)) != TNNI_OK) return st;
     if ((st = ensure(ctx, a.sd, n_d * 4)) != TNNI_OK) return st;
-    HIP_TRY(ctx, hipStreamSynchronize(a.stream));
-    if ((st = ensure_host(ctx, a.h_in, n_in * 12)) != TNNI_OK) return st;
-    if ((st = ensure_host(ctx, a.h_out, (n_h + n_l + n_d) * 4)) != TNNI_OK) return st;
-    int32_t *hx = (int32_t *)a.h_in.p, *hy = hx + n_in, *hz = hy + n_in;
     std::vector<Prepared> prep(s.reqs.size());
     for (size_t k = 0; k < s.reqs.size(); ++k) {
         Request &r = *s.reqs[k];
-        if (r.n) {
-            std::memcpy(hx + r.in_off, r.X, r.n * 4);
-            std::memcpy(hy + r.in_off, r.Y, r.n * 4);
-            std::memcpy(hz + r.in_off, r.Z, r.n * 4);
-        }
         prep[k].t = r.t;
         prep[k].lds = r.lds;
+        if (r.dev) {
+            prep[k].t.X = r.X;
+            prep[k].t.Y = r.Y;
+            prep[k].t.Z = r.Z;
+            prep[k].t.out_h = r.out_h;
+            prep[k].t.out_labels = r.out_labels;
+            prep[k].t.out_d2 = r.out_d2;
+            continue;
+        }
         prep[k].t.X = (const int32_t *)a.sx.p + r.in_off;
         prep[k].t.Y = (const int32_t *)a.sy.p + r.in_off;
         prep[k].t.Z = (const int32_t *)a.sz.p + r.in_off;
         prep[k].t.out_h = (float *)a.sh.p + r.out_h_off;
         prep[k].t.out_labels = r.out_labels ? (int32_t *)a.sl.p + r.out_l_off : nullptr;
         prep[k].t.out_d2 = r.out_d2 ? (uint32_t *)a.sd.p + r.out_d_off : nullptr;
-    }
-    if (n_in) {
-        HIP_TRY(ctx, hipMemcpyAsync(a.sx.p, hx, n_in * 4, hipMemcpyHostToDevice, a.stream));
-        HIP_TRY(ctx, hipMemcpyAsync(a.sy.p, hy, n_in * 4, hipMemcpyHostToDevice, a.stream));
-        HIP_TRY(ctx, hipMemcpyAsync(a.sz.p, hz, n_in * 4, hipMemcpyHostToDevice, a.stream));
+        if (r.n) {
+            const int32_t *pin = (const int32_t *)r.pin_in;
+            HIP_TRY(ctx, hipMemcpyAsync((int32_t *)a.sx.p + r.in_off, pin, r.n * 4, hipMemcpyHostToDevice, a.stream));
+            HIP_TRY(ctx, hipMemcpyAsync((int32_t *)a.sy.p + r.in_off, pin + r.n, r.n * 4, hipMemcpyHostToDevice, a.stream));
+            HIP_TRY(ctx, hipMemcpyAsync((int32_t *)a.sz.p + r.in_off, pin + 2 * r.n, r.n * 4, hipMemcpyHostToDevice, a.stream));
+        }
     }
     if ((st = run_prepared(ctx, a, prep, false, Launch::Graph)) != TNNI_OK) return st;
-    float *oh = (float *)a.h_out.p;
-    int32_t *ol = (int32_t *)(oh + n_h);
-    uint32_t *od = (uint32_t *)(ol + n_l);
-    HIP_TRY(ctx, hipMemcpyAsync(oh, a.sh.p, n_h * 4, hipMemcpyDeviceToHost, a.stream));
-    if (n_l) HIP_TRY(ctx, hipMemcpyAsync(ol, a.sl.p, n_l * 4, hipMemcpyDeviceToHost, a.stream));
-    if (n_d) HIP_TRY(ctx, hipMemcpyAsync(od, a.sd.p, n_d * 4, hipMemcpyDeviceToHost, a.stream));
+    for (auto &rp : s.reqs) {
+        Request &r = *rp;
+        if (r.dev) continue;
+        const size_t cells = (size_t)r.t.nrows * r.t.ncols, pix = (size_t)r.t.W * r.t.H;
+        char *po = (char *)r.pin_out;
+        HIP_TRY(ctx, hipMemcpyAsync(po, (float *)a.sh.p + r.out_h_off, cells * 4, hipMemcpyDeviceToHost, a.stream));
+        po += cells * 4;
+        if (r.out_labels) {
+            HIP_TRY(ctx, hipMemcpyAsync(po, (int32_t *)a.sl.p + r.out_l_off, pix * 4, hipMemcpyDeviceToHost, a.stream));
+            po += pix * 4;
+        }
+        if (r.out_d2)
+            HIP_TRY(ctx, hipMemcpyAsync(po, (uint32_t *)a.sd.p + r.out_d_off, pix * 4, hipMemcpyDeviceToHost, a.stream));
+    }
     HIP_TRY(ctx, hipEventRecord(a.done, a.stream));
     return TNNI_OK;
-}
-
-// Copy a finished batch's results out of the slot's pinned staging.
-void complete_batch(Slot &s)
-{
-    size_t n_h = 0, n_l = 0;
-    for (auto &r : s.reqs) {
-        n_h += (size_t)r->t.nrows * r->t.ncols;
-        if (r->out_labels) n_l += (size_t)r->t.W * r->t.H;
-    }
-    const float *oh = (const float *)s.a.h_out.p;
-    const int32_t *ol = (const int32_t *)(oh + n_h);
-    const uint32_t *od = (const uint32_t *)(ol + n_l);
-    for (auto &r : s.reqs) {
-        const size_t cells = (size_t)r->t.nrows * r->t.ncols, pix = (size_t)r->t.W * r->t.H;
-        std::memcpy(r->out_h, oh + r->out_h_off, cells * 4);
-        if (r->out_labels) std::memcpy(r->out_labels, ol + r->out_l_off, pix * 4);
-        if (r->out_d2) std::memcpy(r->out_d2, od + r->out_d_off, pix * 4);
-    }
 }
 
 void scheduler(tnni_ctx *ctx)
@@ -517,9 +564,6 @@ void scheduler(tnni_ctx *ctx)
                 any_busy = true;
                 continue;
             }
-            lk.unlock();
-            if (q == hipSuccess) complete_batch(s);
-            lk.lock();
             for (auto &r : s.reqs) finish_request(ctx, r, q == hipSuccess ? TNNI_OK : TNNI_E_HIP);
             ctx->inflight_bytes -= s.bytes;
             s.reqs.clear();
@@ -708,6 +752,11 @@ void tnni_ctx_destroy(tnni_ctx *ctx)
     }
     for (auto &s : ctx->slots) arena_free(ctx, s->a);
     arena_free(ctx, ctx->main);
+    for (auto &kv : ctx->reqs) {
+        ctx->pinned.release(kv.second->pin_in, kv.second->pin_in_cap);
+        ctx->pinned.release(kv.second->pin_out, kv.second->pin_out_cap);
+    }
+    ctx->pinned.clear();
     for (void *p : ctx->user_allocs) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -859,9 +908,13 @@ tnni_status tnni_batch_run_dev(tnni_ctx *ctx, int n_tiles, const tnni_tile_desc 
     return run_prepared(ctx, ctx->main, v, true, ctx->prof ? Launch::Direct : Launch::Graph);
 }
 
-tnni_status tnni_submit(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *X,
-                        const int32_t *Y, const int32_t *Z, uint64_t n, float *out_h,
-                        int32_t *out_labels, uint32_t *out_d2, uint64_t *ticket)
+}  // extern "C"
+
+namespace {
+
+tnni_status submit_common(tnni_ctx *ctx, const tnni_tile_desc *desc, bool dev, const int32_t *X,
+                          const int32_t *Y, const int32_t *Z, uint64_t n, float *out_h,
+                          int32_t *out_labels, uint32_t *out_d2, uint64_t *ticket)
 {
     if (!ctx || !desc || !out_h || !ticket || (n && (!X || !Y || !Z)))
         return fail(ctx, TNNI_E_INVALID, "null argument");
@@ -869,6 +922,7 @@ tnni_status tnni_submit(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t
     std::string err;
     tnni_status st = prepare_tile(*desc, n, ctx->n_cu, true, r->t, r->lds, err);
     if (st != TNNI_OK) return fail(ctx, st, err);
+    r->dev = dev;
     r->X = X;
     r->Y = Y;
     r->Z = Z;
@@ -877,11 +931,42 @@ tnni_status tnni_submit(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t
     r->out_labels = out_labels;
     r->out_d2 = out_d2;
     r->footprint = request_bytes(r->t, out_labels != nullptr, out_d2 != nullptr);
+    if (!dev) {
+        // the inputs are copied now, in the caller's thread, so that the
+        // scheduler only issues asynchronous copies
+        const size_t pix = (size_t)r->t.W * r->t.H;
+        const size_t out_b = ((size_t)r->t.nrows * r->t.ncols + (out_labels ? pix : 0) + (out_d2 ? pix : 0)) * 4;
+        r->pin_in = ctx->pinned.acquire(std::max<size_t>(n * 12, 4), r->pin_in_cap);
+        r->pin_out = ctx->pinned.acquire(out_b, r->pin_out_cap);
+        if (!r->pin_in || !r->pin_out) {
+            ctx->pinned.release(r->pin_in, r->pin_in_cap);
+            ctx->pinned.release(r->pin_out, r->pin_out_cap);
+            return fail(ctx, TNNI_E_NOMEM, "pinned host staging");
+        }
+        int32_t *p = (int32_t *)r->pin_in;
+        if (n) {
+            std::memcpy(p, X, n * 4);
+            std::memcpy(p + n, Y, n * 4);
+            std::memcpy(p + 2 * n, Z, n * 4);
+        }
+    }
     std::lock_guard<std::mutex> lk(ctx->smu);
-    if (ctx->shutdown) return fail(ctx, TNNI_E_SHUTDOWN, "context is shutting down");
-    if ((st = start_scheduler(ctx)) != TNNI_OK) return st;
-    if (r->footprint > ctx->serve_budget)
+    auto release = [&] {
+        ctx->pinned.release(r->pin_in, r->pin_in_cap);
+        ctx->pinned.release(r->pin_out, r->pin_out_cap);
+    };
+    if (ctx->shutdown) {
+        release();
+        return fail(ctx, TNNI_E_SHUTDOWN, "context is shutting down");
+    }
+    if ((st = start_scheduler(ctx)) != TNNI_OK) {
+        release();
+        return st;
+    }
+    if (r->footprint > ctx->serve_budget) {
+        release();
         return fail(ctx, TNNI_E_BUDGET, "tile footprint exceeds the context's HBM budget");
+    }
     r->id = ctx->next_id++;
     r->t_submit = now_s();
     ctx->reqs[r->id] = r;
@@ -890,6 +975,24 @@ tnni_status tnni_submit(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t
     ctx->scv.notify_all();
     *ticket = r->id;
     return TNNI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tnni_status tnni_submit(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *X,
+                        const int32_t *Y, const int32_t *Z, uint64_t n, float *out_h,
+                        int32_t *out_labels, uint32_t *out_d2, uint64_t *ticket)
+{
+    return submit_common(ctx, desc, false, X, Y, Z, n, out_h, out_labels, out_d2, ticket);
+}
+
+tnni_status tnni_submit_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *dX,
+                            const int32_t *dY, const int32_t *dZ, uint64_t n, float *d_out_h,
+                            int32_t *d_out_labels, uint32_t *d_out_d2, uint64_t *ticket)
+{
+    return submit_common(ctx, desc, true, dX, dY, dZ, n, d_out_h, d_out_labels, d_out_d2, ticket);
 }
 
 tnni_status tnni_poll(tnni_ctx *ctx, uint64_t ticket)
@@ -910,6 +1013,25 @@ tnni_status tnni_wait(tnni_ctx *ctx, uint64_t ticket, double *latency_s)
     std::shared_ptr<Request> r = it->second;
     ctx->dcv.wait(lk, [&r] { return r->status != TNNI_PENDING; });
     ctx->reqs.erase(ticket);
+    lk.unlock();
+    if (!r->dev) {
+        // results out of the pinned staging, in the caller's thread
+        if (r->status == TNNI_OK) {
+            const size_t cells = (size_t)r->t.nrows * r->t.ncols, pix = (size_t)r->t.W * r->t.H;
+            const char *po = (const char *)r->pin_out;
+            std::memcpy(r->out_h, po, cells * 4);
+            po += cells * 4;
+            if (r->out_labels) {
+                std::memcpy(r->out_labels, po, pix * 4);
+                po += pix * 4;
+            }
+            if (r->out_d2) std::memcpy(r->out_d2, po, pix * 4);
+        }
+        ctx->pinned.release(r->pin_in, r->pin_in_cap);
+        ctx->pinned.release(r->pin_out, r->pin_out_cap);
+        r->pin_in = r->pin_out = nullptr;
+        r->t_done = std::max(r->t_done, now_s());  // results in the caller's buffers
+    }
     if (latency_s) *latency_s = r->t_done - r->t_submit;
     if (r->status != TNNI_OK) ctx->err = "request failed";
     return r->status;

@@ -138,3 +138,17 @@ def test_served_admission_waits_for_memory(ctx):
     for t, h in zip(tiles, outs):
         h0, _, _ = ctx.tile_run(t.params, t.X, t.Y, t.Z)
         assert np.array_equal(h0.view(np.uint32), h.view(np.uint32))
+
+
+def test_served_device_resident_equals_plain(ctx):
+    _say("served device-resident")
+    tiles = _mixed_tiles(10, seed=600)
+    with T.Context(0, n_slots=2, max_batch_tiles=4) as c:
+        dev = [(c.upload(t.X), c.upload(t.Y), c.upload(t.Z),
+                c.alloc((t.params.nrows, t.params.ncols), np.float32)) for t in tiles]
+        pends = [c.submit_dev(t.params, d[0], d[1], d[2], t.n, d[3]) for t, d in zip(tiles, dev)]
+        for p in pends:
+            p.wait()
+        for t, d in zip(tiles, dev):
+            h0, _, _ = ctx.tile_run(t.params, t.X, t.Y, t.Z)
+            assert np.array_equal(d[3].to_host().view(np.uint32), h0.view(np.uint32))
