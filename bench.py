@@ -49,11 +49,13 @@ def cpu_baseline(tile, budget_s: float = 12.0):
     return tile.cells / per, threads, per, reps, ref
 
 
-def bench_batch(T, S, ctx, reps: int = 3):
+def bench_batch(T, S, ctx, reps: int = 3, tune: bool = True):
     """The 256 uneven tiles (log-normal point counts, grids 256^2..2048^2,
     half uniform, half clustered): the plain form (one tile per launch
     sequence, synchronised) against the batched form (one bucket sort + one
-    persistent launch over all tiles, replayed from a captured graph)."""
+    persistent launch over all tiles, replayed from a captured graph), each
+    with the built-in sizes and with sizes the autotuner measured for its
+    form on this workload."""
     t0 = time.perf_counter()
     tiles = S.workload_batch()
     log(f"batched workload: {len(tiles)} tiles, {sum(t.n for t in tiles)} points, "
@@ -62,45 +64,74 @@ def bench_batch(T, S, ctx, reps: int = 3):
             ctx.alloc((t.params.nrows, t.params.ncols), np.float32)) for t in tiles]
     cells = sum(t.cells for t in tiles)
     pts = sum(t.n for t in tiles)
-    # plain: one tile per launch sequence, in manifest order
-    for t, d in zip(tiles[:8], dev[:8]):
-        ctx.tile_run_dev(t.params, d[0], d[1], d[2], t.n, d[3])
-    lat = []
-    s0 = time.perf_counter()
-    for t, d in zip(tiles, dev):
-        a = time.perf_counter()
-        ctx.tile_run_dev(t.params, d[0], d[1], d[2], t.n, d[3])
-        lat.append(time.perf_counter() - a)
-    plain_s = time.perf_counter() - s0
-    plain_out = [d[3].to_host() for d in dev]
-    # batched: all tiles in one graph-replayed pipeline
-    args = ([t.params for t in tiles], [d[0] for d in dev], [d[1] for d in dev],
-            [d[2] for d in dev], [t.n for t in tiles], [d[3] for d in dev])
-    ctx.batch_run_dev(*args)
-    s0 = time.perf_counter()
-    for _ in range(reps):
-        ctx.batch_run_dev(*args)
-    batch_s = (time.perf_counter() - s0) / reps
-    same = all(np.array_equal(po.view(np.uint32), d[3].to_host().view(np.uint32))
-               for po, d in zip(plain_out, dev))
+    P = [t.params for t in tiles]
+    cols = ([d[0] for d in dev], [d[1] for d in dev], [d[2] for d in dev])
+    ns = [t.n for t in tiles]
+    hs = [d[3] for d in dev]
+
+    def plain():
+        for t, d in zip(tiles[:8], dev[:8]):
+            ctx.tile_run_dev(t.params, d[0], d[1], d[2], t.n, d[3])
+        lat = []
+        s0 = time.perf_counter()
+        for t, d in zip(tiles, dev):
+            a = time.perf_counter()
+            ctx.tile_run_dev(t.params, d[0], d[1], d[2], t.n, d[3])
+            lat.append(time.perf_counter() - a)
+        return time.perf_counter() - s0, lat, [h.to_host() for h in hs]
+
+    def batched():
+        ctx.batch_run_dev(P, *cols, ns, hs)
+        s0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.batch_run_dev(P, *cols, ns, hs)
+        return (time.perf_counter() - s0) / reps, [h.to_host() for h in hs]
+
+    def same(a, b):
+        return all(np.array_equal(x.view(np.uint32), y.view(np.uint32)) for x, y in zip(a, b))
+
+    plain_s, lat, plain_out = plain()
+    batch_s, batch_out = batched()
+    tuned_b = tuned_p = []
+    tune_s = 0.0
+    if tune:
+        t_tune = time.perf_counter()
+        tuned_b = ctx.autotune(P, *cols, ns, batched=True, reps=2)
+        tuned_p = ctx.autotune(P, *cols, ns, batched=False, reps=1)
+        tune_s = time.perf_counter() - t_tune
+        log(f"autotuned {len(tuned_b)} batched + {len(tuned_p)} plain shape classes in {tune_s:.1f}s")
+    plain_t_s, lat_t, plain_t_out = plain()
+    batch_t_s, batch_t_out = batched()
     for d in dev:
         for a in d:
             a.free()
+
+    def pct(v):
+        return {"p50": float(np.percentile(v, 50) * 1e3), "p99": float(np.percentile(v, 99) * 1e3)}
+
     return {
         "workload": "256 uneven tiles, log-normal points (median 200k, 10k-2M), grids "
                     "256^2-2048^2 at 0.5 m, half uniform / half clustered",
         "tiles": len(tiles), "points": pts, "cells": cells,
         "plain_cells_per_s": cells / plain_s, "plain_points_per_s": pts / plain_s,
-        "plain_tile_latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
-                                  "p99": float(np.percentile(lat, 99) * 1e3)},
+        "plain_tile_latency_ms": pct(lat),
         "batched_cells_per_s": cells / batch_s, "batched_points_per_s": pts / batch_s,
         "batched_ms": batch_s * 1e3,
         "speedup": plain_s / batch_s,
-        "batched_equals_plain": bool(same),
-    }, tiles
+        "tuned": {"classes_batched": len(tuned_b), "classes_plain": len(tuned_p),
+                  "tuning_s": tune_s,
+                  "plain_cells_per_s": cells / plain_t_s, "plain_tile_latency_ms": pct(lat_t),
+                  "batched_cells_per_s": cells / batch_t_s, "batched_ms": batch_t_s * 1e3,
+                  "speedup": plain_t_s / batch_t_s,
+                  "batched_choices": [{k: e[k] for k in ("bucket_px", "tile_cells", "wg_per_cu",
+                                                          "ms_default", "ms_tuned")}
+                                      for e in tuned_b]} if tune else None,
+        "batched_equals_plain": bool(same(plain_out, batch_out) and same(plain_out, plain_t_out)
+                                     and same(plain_out, batch_t_out)),
+    }, tiles, ctx.tune_table()
 
 
-def bench_served(T, tiles, rates, per_rate, resident: bool, seed=4):
+def bench_served(T, tiles, rates, per_rate, resident: bool, seed=4, tune_table=None):
     """Served form: Poisson arrivals at each rate of tiles drawn from the
     uneven-tile pool; one thread submits, another waits in order.  resident:
     the tiles' points and the output rasters live in HBM (tnni_submit_dev);
@@ -112,6 +143,8 @@ def bench_served(T, tiles, rates, per_rate, resident: bool, seed=4):
     rng = np.random.default_rng(seed)
     ring = 192  # output rasters in flight (resident)
     with T.Context(0, n_slots=3, max_batch_tiles=64) as c:
+        if tune_table:
+            c.tune_load(tune_table)
         if resident:
             dev = [(c.upload(t.X), c.upload(t.Y), c.upload(t.Z)) for t in tiles]
             big = max(t.cells for t in tiles)
@@ -183,6 +216,45 @@ def bench_served(T, tiles, rates, per_rate, resident: bool, seed=4):
     return out
 
 
+def bench_batched_only(T, S, args):
+    """--workload batch: only the batched form over the 256 uneven tiles,
+    K graph replays timed (for profiling that form alone)."""
+    tiles = S.workload_batch()
+    ctx = T.Context(0)
+    dev = [(ctx.upload(t.X), ctx.upload(t.Y), ctx.upload(t.Z),
+            ctx.alloc((t.params.nrows, t.params.ncols), np.float32)) for t in tiles]
+    P = [t.params for t in tiles]
+    cols = ([d[0] for d in dev], [d[1] for d in dev], [d[2] for d in dev])
+    ns = [t.n for t in tiles]
+    hs = [d[3] for d in dev]
+    tuned = None
+    if not args.no_tune:
+        tuned = ctx.autotune(P, *cols, ns, batched=True, reps=2)
+    for _ in range(max(args.warmup, 1)):
+        ctx.batch_run_dev(P, *cols, ns, hs)
+    s0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.batch_run_dev(P, *cols, ns, hs)
+    el = time.perf_counter() - s0
+    cells = sum(t.cells for t in tiles)
+    pts = sum(ns)
+    out = {"metric": "DEM cells/s (256 uneven tiles, batched form)", "value": cells * args.steps / el,
+           "unit": "cells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "int32",
+           "data": "synthetic (seeded uniform / Matern-cluster LiDAR, LAS int32 at 0.01 m)",
+           "config": {"workload": "256 uneven tiles, log-normal points (median 200k, 10k-2M), "
+                                  "grids 256^2-2048^2 at 0.5 m, one graph-replayed launch",
+                      "points": pts, "cells": cells, "form": "batched",
+                      "tuned_classes": None if tuned is None else len(tuned)},
+           "points_per_s": pts * args.steps / el}
+    for d in dev:
+        for a in d:
+            a.free()
+    ctx.close()
+    print(json.dumps(out))
+
+
 def load_traffic(kernel="k_tiles<0, 0>"):
     """HBM bytes per k_tiles launch on this workload, from the committed
     FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_single_4m.json)."""
@@ -205,7 +277,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-cells", type=int, default=0, help="cell-tile side (0 = auto)")
     ap.add_argument("--bucket-px", type=int, default=0, help="bucket side in pixels (0 = auto)")
+    ap.add_argument("--workload", choices=("single", "batch"), default="single",
+                    help="single: the 4096^2 tile (default line, plus the batched and served "
+                         "comparisons); batch: only the batched 256-tile form")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (built-in bucket and cell-tile sizes)")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--served-rates", default="50,200,500,1000,2000")
     ap.add_argument("--served-requests", type=int, default=300,
@@ -222,6 +299,9 @@ def main():
     import terranni_amd as T
     import terranni_amd.synth as S
 
+    if args.workload == "batch":
+        bench_batched_only(T, S, args)
+        return
     t0 = time.perf_counter()
     tile = S.workload_single_4m(seed=1, n=args.points, grid=args.grid,
                                 tile_cells=args.tile_cells, bucket_px=args.bucket_px)
@@ -232,6 +312,10 @@ def main():
     dX, dY, dZ = ctx.upload(tile.X), ctx.upload(tile.Y), ctx.upload(tile.Z)
     dH = ctx.alloc((tile.params.nrows, tile.params.ncols), np.float32)
 
+    tuning = None
+    if not args.no_tune:
+        tuning = ctx.autotune([tile.params], [dX], [dY], [dZ], [tile.n], batched=False, reps=3)
+        log(f"autotuned single tile: {tuning}")
     for _ in range(args.warmup):
         ctx.tile_run_dev(tile.params, dX, dY, dZ, tile.n, dH)
     ctx.profile(True)
@@ -279,7 +363,10 @@ def main():
         "data": "synthetic (seeded Matern-cluster LiDAR, LAS int32 at 0.01 m)",
         "config": {"workload": "single tile 4096x4096 cells at 0.5 m, scale 1, 4M clustered points",
                    "points": tile.n, "cells": tile.cells, "pixels": fr["W"] * fr["H"],
-                   "form": "plain"},
+                   "form": "plain",
+                   "tuning": (None if tuning is None else
+                              {k: tuning[0][k] for k in ("bucket_px", "tile_cells", "wg_per_cu",
+                                                         "ms_default", "ms_tuned")})},
         "points_per_s": pts_s,
         "pcie_inclusive_cells_per_s": pcie_cells_s,
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
@@ -290,17 +377,19 @@ def main():
     }
     for d in (dX, dY, dZ, dH):
         d.free()
-    pool = None
+    pool, table = None, None
     if not args.no_batch:
-        out["batched_vs_plain"], pool = bench_batch(T, S, ctx)
+        out["batched_vs_plain"], pool, table = bench_batch(T, S, ctx, tune=not args.no_tune)
     if not args.no_served:
         if pool is None:
             pool = S.workload_batch()
         rates = [float(r) for r in args.served_rates.split(",") if r]
         out["served"] = {
             "workload": "Poisson arrivals of tiles drawn from the 256 uneven tiles",
-            "hbm_resident": bench_served(T, pool, rates, args.served_requests, True),
-            "host_buffers": bench_served(T, pool, rates[:2], args.served_requests, False),
+            "hbm_resident": bench_served(T, pool, rates, args.served_requests, True,
+                                         tune_table=table),
+            "host_buffers": bench_served(T, pool, rates[:2], args.served_requests, False,
+                                         tune_table=table),
         }
     if not args.no_cpu:
         c_cells, threads, per, reps, ref = cpu_baseline(tile)

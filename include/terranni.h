@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define TNNI_ABI_VERSION 1
+#define TNNI_ABI_VERSION 2
 
 typedef enum tnni_status {
     TNNI_OK = 0,
@@ -202,6 +202,48 @@ typedef struct tnni_serve_stats {
     uint64_t inflight_bytes;       /* footprint admitted right now       */
 } tnni_serve_stats;
 tnni_status tnni_serve_stats_read(tnni_ctx *ctx, tnni_serve_stats *out);
+
+/* ---- autotuning --------------------------------------------------------
+ * New (the reference has fixed GL raster sizes, nni.cpp:601-668): the bucket
+ * side, the cell-tile side and the workgroups per CU of the tile kernel are
+ * measured per shape class on sample tiles and kept in the context.  A shape
+ * class groups tiles by algorithm, accumulation mode, scale, query span and
+ * site radius in pixels, grid side (to a power of two), point density (to
+ * half a power of two) and form (plain or batched/served).  Descriptors
+ * whose bucket_px and tile_cells are 0 then use the tuned sizes of their
+ * class; classes never tuned use the built-in choice. */
+typedef struct tnni_tune_entry {
+    uint64_t shape_class;
+    int32_t bucket_px;   /* bucket side in pixels                           */
+    int32_t tile_cells;  /* cell-tile side                                  */
+    int32_t wg_per_cu;   /* workgroups per CU of the tile kernel (0 = all)  */
+    int32_t candidates;  /* settings timed (tnni_autotune only)             */
+    float ms_default;    /* sample time, built-in choice (tnni_autotune)    */
+    float ms_tuned;      /* sample time, chosen setting (tnni_autotune)     */
+} tnni_tune_entry;
+
+/* Shape class of a tile (batched != 0: the batched and served forms). */
+tnni_status tnni_shape_class(const tnni_tile_desc *desc, uint64_t n_points, int batched,
+                             uint64_t *shape_class);
+
+/* Time candidate settings on device-resident sample tiles, grouped by shape
+ * class, and keep the fastest per class.  batched != 0 tunes the batched and
+ * served forms (each class's samples are replicated to about 2^25 pixels
+ * and timed as one launch between HIP events); 0 tunes the plain form (one
+ * synchronous launch per tile, host clock).  Each setting runs once to warm
+ * up and then reps times (best kept).  Outputs go to scratch rasters the
+ * context owns.  out (nullable) receives one entry per class, at most
+ * n_tiles; *n_classes their number. */
+tnni_status tnni_autotune(tnni_ctx *ctx, int n_tiles, const tnni_tile_desc *descs,
+                          const int32_t *const *dX, const int32_t *const *dY,
+                          const int32_t *const *dZ, const uint64_t *n, int batched, int reps,
+                          tnni_tune_entry *out, int *n_classes);
+
+/* The tuned table: read (up to cap entries; *n = entries held), load (for
+ * example a table saved by an earlier run) and clear. */
+tnni_status tnni_tune_read(tnni_ctx *ctx, tnni_tune_entry *out, int cap, int *n);
+tnni_status tnni_tune_load(tnni_ctx *ctx, const tnni_tune_entry *in, int n);
+tnni_status tnni_tune_clear(tnni_ctx *ctx);
 
 /* ---- profiling -------------------------------------------------------- */
 /* When enabled, HIP events bracket every launch of the main tile kernel on

@@ -85,6 +85,18 @@ class ServeStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [
+        ("shape_class", ctypes.c_uint64), ("bucket_px", ctypes.c_int32),
+        ("tile_cells", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
+        ("candidates", ctypes.c_int32), ("ms_default", ctypes.c_float),
+        ("ms_tuned", ctypes.c_float),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 @dataclass
 class TileParams:
     """One work package: nni::Settings + nni::WorkPackage + LAS scale/offset."""
@@ -176,11 +188,21 @@ def lib():
     L.tnni_poll.argtypes = [P, ctypes.c_uint64]
     L.tnni_wait.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
     L.tnni_serve_stats_read.argtypes = [P, ctypes.POINTER(ServeStats)]
+    L.tnni_shape_class.argtypes = [ctypes.POINTER(TileDesc), ctypes.c_uint64, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_uint64)]
+    L.tnni_autotune.argtypes = [P, ctypes.c_int, ctypes.POINTER(TileDesc), P, P, P,
+                                ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(TuneEntry), ctypes.POINTER(ctypes.c_int)]
+    L.tnni_tune_read.argtypes = [P, ctypes.POINTER(TuneEntry), ctypes.c_int,
+                                 ctypes.POINTER(ctypes.c_int)]
+    L.tnni_tune_load.argtypes = [P, ctypes.POINTER(TuneEntry), ctypes.c_int]
+    L.tnni_tune_clear.argtypes = [P]
     for name in ("tnni_frame", "tnni_footprint", "tnni_ctx_create", "tnni_tile_run",
                  "tnni_tile_run_dev", "tnni_batch_run_dev", "tnni_dev_alloc", "tnni_dev_free",
                  "tnni_h2d", "tnni_d2h", "tnni_d2d", "tnni_profile_enable", "tnni_profile_read",
                  "tnni_submit", "tnni_submit_dev", "tnni_poll", "tnni_wait",
-                 "tnni_serve_stats_read"):
+                 "tnni_serve_stats_read", "tnni_shape_class", "tnni_autotune",
+                 "tnni_tune_read", "tnni_tune_load", "tnni_tune_clear"):
         getattr(L, name).restype = ctypes.c_int
     del i32p
     _lib = L
@@ -191,6 +213,14 @@ def _check(st: int, ctx=None):
     if st != 0:
         msg = lib().tnni_last_error(ctx)
         raise TnniError(st, msg.decode() if msg else "")
+
+
+def shape_class(p: TileParams, n_points: int, batched: bool = True) -> int:
+    """Shape class under which tuned sizes are kept (host only)."""
+    c = ctypes.c_uint64()
+    d = p.to_desc()
+    _check(lib().tnni_shape_class(ctypes.byref(d), n_points, int(batched), ctypes.byref(c)))
+    return c.value
 
 
 def frame(p: TileParams) -> dict:
@@ -321,6 +351,38 @@ class Context:
             self.h, k, descs, arr(dXs), arr(dYs), arr(dZs), nn, arr(d_hs),
             arr(d_labs) if d_labs is not None else None,
             arr(d_d2s) if d_d2s is not None else None))
+
+    # ---- autotuning ----
+    def autotune(self, params, dXs, dYs, dZs, ns, batched: bool = True, reps: int = 3):
+        """Tune bucket / cell-tile sizes and workgroups per CU per shape
+        class on device-resident sample tiles; returns one dict per class."""
+        k = len(params)
+        descs = (TileDesc * k)(*[p.to_desc() for p in params])
+        arr = lambda xs: (ctypes.c_void_p * k)(*[x.ptr for x in xs])  # noqa
+        nn = (ctypes.c_uint64 * k)(*ns)
+        out = (TuneEntry * max(k, 1))()
+        nc = ctypes.c_int()
+        self._chk(lib().tnni_autotune(self.h, k, descs, arr(dXs), arr(dYs), arr(dZs), nn,
+                                      int(batched), reps, out, ctypes.byref(nc)))
+        return [out[i].as_dict() for i in range(nc.value)]
+
+    def tune_table(self):
+        n = ctypes.c_int()
+        self._chk(lib().tnni_tune_read(self.h, None, 0, ctypes.byref(n)))
+        out = (TuneEntry * max(n.value, 1))()
+        self._chk(lib().tnni_tune_read(self.h, out, n.value, ctypes.byref(n)))
+        return [out[i].as_dict() for i in range(n.value)]
+
+    def tune_load(self, entries):
+        arr = (TuneEntry * max(len(entries), 1))()
+        for i, e in enumerate(entries):
+            for f, _ in TuneEntry._fields_:
+                if f in e:
+                    setattr(arr[i], f, e[f])
+        self._chk(lib().tnni_tune_load(self.h, arr, len(entries)))
+
+    def tune_clear(self):
+        self._chk(lib().tnni_tune_clear(self.h))
 
     # ---- served form ----
     def submit(self, p: TileParams, X, Y, Z, labels: bool = False):

@@ -81,7 +81,7 @@ struct Request {
     int32_t *out_labels = nullptr;
     uint32_t *out_d2 = nullptr;
     TileDev t{};
-    size_t lds = 0;
+    PrepInfo info;
     uint64_t footprint = 0;
     double t_submit = 0, t_done = 0;
     tnni_status status = TNNI_PENDING;
@@ -167,6 +167,10 @@ struct tnni_ctx {
     double prof_ms = 0.0;
 
     std::vector<void *> user_allocs;
+
+    // measured bucket / cell-tile sizes per shape class (tnni_autotune)
+    TuneTable tuned;
+    hipEvent_t tev0 = nullptr, tev1 = nullptr;
 
     // served form
     int n_slots = 2;
@@ -286,7 +290,7 @@ void arena_free(tnni_ctx *ctx, Arena &a)
 // One prepared tile of a batch, before the batch layout is assigned.
 struct Prepared {
     TileDev t;
-    size_t lds;
+    PrepInfo info;
 };
 
 enum class Launch { Direct, Graph };
@@ -295,12 +299,14 @@ enum class Launch { Direct, Graph };
 // enqueue the pipeline on the arena's stream.  Tiles keep their order; the
 // caller orders them.  sync: wait for the stream before returning.
 tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, bool sync,
-                         Launch how)
+                         Launch how, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr)
 {
     if (tiles.empty()) return TNNI_OK;
     const int alg = tiles[0].t.algorithm, mode = tiles[0].t.mode;
     int64_t pts = 0, bks = 0, items = 0, rows = 0, bands = 0, hents = 0;
     size_t lds = 0;
+    int wg_cap = 0;       // tuned workgroups per CU of the tile kernel
+    bool wg_any = false;  // some tile has no tuned cap
     std::vector<Chunk> chunks;
     for (size_t k = 0; k < tiles.size(); ++k) {
         TileDev &t = tiles[k].t;
@@ -329,7 +335,9 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
         bands += t.nbands;
         hents += (int64_t)t.nbands * t.nchunks;
         items += (int64_t)t.ntx * t.nty;
-        lds = std::max(lds, tiles[k].lds);
+        lds = std::max(lds, tiles[k].info.lds);
+        wg_any = wg_any || tiles[k].info.wg_per_cu == 0;
+        wg_cap = std::max(wg_cap, tiles[k].info.wg_per_cu);
     }
     if (bks + 1 > 0xFFFFFFFFll || pts > 0xFFFFFFFFll || hents + 1 > 0xFFFFFFFFll)
         return fail(ctx, TNNI_E_INVALID, "batch exceeds 2^32 buckets or points");
@@ -365,12 +373,14 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     for (size_t k = 0; k < tiles.size(); ++k) std::memcpy(h + k * sizeof(TileDev), &tiles[k].t, sizeof(TileDev));
     std::memcpy(h + sz_t, chunks.data(), sz_c);
     std::memcpy(h + sz_t + sz_c, &bd, sizeof bd);
+    if (ev_begin) HIP_TRY(ctx, hipEventRecord(ev_begin, a.stream));
     HIP_TRY(ctx, hipMemcpyAsync(a.tiles.p, h, sz_t, hipMemcpyHostToDevice, a.stream));
     HIP_TRY(ctx, hipMemcpyAsync(a.chunks.p, h + sz_t, sz_c, hipMemcpyHostToDevice, a.stream));
     HIP_TRY(ctx, hipMemcpyAsync(a.batch.p, h + sz_t + sz_c, sizeof bd, hipMemcpyHostToDevice, a.stream));
 
     LaunchCfg cfg;
-    const int occ = tile_kernel_occupancy(lds);
+    int occ = tile_kernel_occupancy(lds);
+    if (!wg_any && wg_cap > 0) occ = std::min(occ, wg_cap);
     cfg.main_lds = lds;
     cfg.algorithm = alg;
     cfg.mode = mode;
@@ -429,6 +439,7 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
         HIP_TRY(ctx, hipGraphLaunch(a.gexec, a.stream));
         a.graph_launches++;
     }
+    if (ev_end) HIP_TRY(ctx, hipEventRecord(ev_end, a.stream));
     if (sync || prof) HIP_TRY(ctx, hipStreamSynchronize(a.stream));
     if (prof) {
         float ms = 0.f;
@@ -507,7 +518,7 @@ tnni_status launch_batch(tnni_ctx *ctx, Slot &s)
     for (size_t k = 0; k < s.reqs.size(); ++k) {
         Request &r = *s.reqs[k];
         prep[k].t = r.t;
-        prep[k].lds = r.lds;
+        prep[k].info = r.info;
         if (r.dev) {
             prep[k].t.X = r.X;
             prep[k].t.Y = r.Y;
@@ -685,9 +696,9 @@ tnni_status tnni_footprint(const tnni_tile_desc *desc, uint64_t n_points, int wi
 {
     if (!desc || !bytes) return fail(nullptr, TNNI_E_INVALID, "null argument");
     TileDev t;
-    size_t lds;
     std::string err;
-    tnni_status st = prepare_tile(*desc, n_points, 256, true, t, lds, err);
+    PrepInfo info;
+    tnni_status st = prepare_tile(*desc, n_points, 256, true, nullptr, t, info, err);
     if (st != TNNI_OK) return fail(nullptr, st, err);
     *bytes = request_bytes(t, with_labels != 0, with_labels != 0);
     return TNNI_OK;
@@ -728,7 +739,9 @@ tnni_status tnni_ctx_create(const tnni_ctx_opts *opts, tnni_ctx **out)
     ctx->budget = o.hbm_budget ? o.hbm_budget : (uint64_t)(0.8 * (double)fr);
     tnni_status st = arena_init(ctx, ctx->main);
     if (st == TNNI_OK && ((e = hipEventCreate(&ctx->ev0)) != hipSuccess ||
-                          (e = hipEventCreate(&ctx->ev1)) != hipSuccess))
+                          (e = hipEventCreate(&ctx->ev1)) != hipSuccess ||
+                          (e = hipEventCreate(&ctx->tev0)) != hipSuccess ||
+                          (e = hipEventCreate(&ctx->tev1)) != hipSuccess))
         st = hip_fail(nullptr, e, "hipEventCreate");
     if (st != TNNI_OK) {
         tnni_ctx_destroy(ctx);
@@ -760,6 +773,8 @@ void tnni_ctx_destroy(tnni_ctx *ctx)
     for (void *p : ctx->user_allocs) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->tev0) (void)hipEventDestroy(ctx->tev0);
+    if (ctx->tev1) (void)hipEventDestroy(ctx->tev1);
     delete ctx;
 }
 
@@ -827,7 +842,7 @@ tnni_status tnni_tile_run_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, const i
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     std::vector<Prepared> v(1);
     std::string err;
-    tnni_status st = prepare_tile(*desc, n, ctx->n_cu, false, v[0].t, v[0].lds, err);
+    tnni_status st = prepare_tile(*desc, n, ctx->n_cu, false, &ctx->tuned, v[0].t, v[0].info, err);
     if (st != TNNI_OK) return fail(ctx, st, err);
     v[0].t.X = dX;
     v[0].t.Y = dY;
@@ -848,7 +863,7 @@ tnni_status tnni_tile_run(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     std::vector<Prepared> v(1);
     std::string err;
-    tnni_status st = prepare_tile(*desc, n, ctx->n_cu, false, v[0].t, v[0].lds, err);
+    tnni_status st = prepare_tile(*desc, n, ctx->n_cu, false, &ctx->tuned, v[0].t, v[0].info, err);
     if (st != TNNI_OK) return fail(ctx, st, err);
     Arena &a = ctx->main;
     TileDev &t = v[0].t;
@@ -892,7 +907,7 @@ tnni_status tnni_batch_run_dev(tnni_ctx *ctx, int n_tiles, const tnni_tile_desc 
     std::vector<Prepared> v((size_t)n_tiles);
     std::string err;
     for (int k = 0; k < n_tiles; ++k) {
-        tnni_status st = prepare_tile(descs[k], n[k], ctx->n_cu, true, v[k].t, v[k].lds, err);
+        tnni_status st = prepare_tile(descs[k], n[k], ctx->n_cu, true, &ctx->tuned, v[k].t, v[k].info, err);
         if (st != TNNI_OK) return fail(ctx, st, "tile " + std::to_string(k) + ": " + err);
         if (!d_out_h[k] || (n[k] && (!dX[k] || !dY[k] || !dZ[k])))
             return fail(ctx, TNNI_E_INVALID, "tile " + std::to_string(k) + ": null pointer");
@@ -920,7 +935,7 @@ tnni_status submit_common(tnni_ctx *ctx, const tnni_tile_desc *desc, bool dev, c
         return fail(ctx, TNNI_E_INVALID, "null argument");
     auto r = std::make_shared<Request>();
     std::string err;
-    tnni_status st = prepare_tile(*desc, n, ctx->n_cu, true, r->t, r->lds, err);
+    tnni_status st = prepare_tile(*desc, n, ctx->n_cu, true, &ctx->tuned, r->t, r->info, err);
     if (st != TNNI_OK) return fail(ctx, st, err);
     r->dev = dev;
     r->X = X;
@@ -1064,6 +1079,240 @@ tnni_status tnni_profile_read(tnni_ctx *ctx, uint64_t *launches, double *total_m
         ctx->prof_launches = 0;
         ctx->prof_ms = 0.0;
     }
+    return TNNI_OK;
+}
+
+}  // extern "C"
+
+// ---- autotuning ----------------------------------------------------------
+
+namespace {
+
+// Time one setting (bucket side bm, cell-tile side tc, at most wg
+// workgroups per CU; 0 = all) on `replicas` copies of the sample tiles idx.
+// Batched: one launch over all of them, GPU time between events around the
+// pipeline; plain: one synchronous launch per tile, host clock.  One
+// warm-up run, then the best of reps.
+tnni_status time_setting(tnni_ctx *ctx, const std::vector<int> &idx, const tnni_tile_desc *descs,
+                         const int32_t *const *dX, const int32_t *const *dY,
+                         const int32_t *const *dZ, const uint64_t *n, bool batched, int replicas,
+                         int bm, int tc, int wg, int reps, float &best_ms)
+{
+    Arena &a = ctx->main;
+    std::vector<Prepared> v;
+    std::vector<size_t> out_off;
+    std::string err;
+    size_t cells = 0;
+    for (int r = 0; r < replicas; ++r) {
+        for (int k : idx) {
+            tnni_tile_desc d = descs[k];
+            d.bucket_px = bm;
+            d.tile_cells = tc;
+            Prepared p;
+            tnni_status st = prepare_tile(d, n[k], ctx->n_cu, batched, nullptr, p.t, p.info, err);
+            if (st != TNNI_OK) return fail(ctx, st, err);
+            p.info.wg_per_cu = wg;
+            p.t.X = dX[k];
+            p.t.Y = dY[k];
+            p.t.Z = dZ[k];
+            out_off.push_back(cells);
+            cells += (size_t)p.t.nrows * p.t.ncols;
+            v.push_back(p);
+        }
+    }
+    tnni_status st = ensure(ctx, a.sh, cells * 4);
+    if (st != TNNI_OK) return st;
+    for (size_t k = 0; k < v.size(); ++k) v[k].t.out_h = (float *)a.sh.p + out_off[k];
+    if (batched)
+        std::stable_sort(v.begin(), v.end(), [](const Prepared &x, const Prepared &y) {
+            return tile_cost(x.t) > tile_cost(y.t);
+        });
+    best_ms = 3.0e38f;
+    for (int rep = 0; rep <= reps; ++rep) {
+        float ms = 0.f;
+        if (batched) {
+            std::vector<Prepared> w = v;
+            if ((st = run_prepared(ctx, a, w, true, Launch::Direct, ctx->tev0, ctx->tev1)) != TNNI_OK)
+                return st;
+            HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->tev0, ctx->tev1));
+        } else {
+            const double t0 = now_s();
+            for (const Prepared &p : v) {
+                std::vector<Prepared> w(1, p);
+                if ((st = run_prepared(ctx, a, w, true, Launch::Direct)) != TNNI_OK) return st;
+            }
+            ms = (float)((now_s() - t0) * 1e3);
+        }
+        if (rep > 0) best_ms = std::min(best_ms, ms);
+    }
+    return TNNI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tnni_status tnni_shape_class(const tnni_tile_desc *desc, uint64_t n_points, int batched,
+                             uint64_t *cls)
+{
+    if (!desc || !cls) return fail(nullptr, TNNI_E_INVALID, "null argument");
+    tnni_frame_info f;
+    std::string err;
+    tnni_status st = compute_frame(*desc, f, err);
+    if (st != TNNI_OK) return fail(nullptr, st, err);
+    *cls = shape_class(*desc, n_points, f, batched != 0);
+    return TNNI_OK;
+}
+
+tnni_status tnni_autotune(tnni_ctx *ctx, int n_tiles, const tnni_tile_desc *descs,
+                          const int32_t *const *dX, const int32_t *const *dY,
+                          const int32_t *const *dZ, const uint64_t *n, int batched, int reps,
+                          tnni_tune_entry *out, int *n_classes)
+{
+    if (!ctx || n_tiles < 0 || (n_tiles && (!descs || !dX || !dY || !dZ || !n)))
+        return fail(ctx, TNNI_E_INVALID, "null argument");
+    if (n_classes) *n_classes = 0;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    reps = std::max(1, std::min(reps, 20));
+    const bool bat = batched != 0;
+    std::map<uint64_t, std::vector<int>> groups;
+    std::vector<uint64_t> pix((size_t)n_tiles);
+    for (int k = 0; k < n_tiles; ++k) {
+        if (n[k] && (!dX[k] || !dY[k] || !dZ[k]))
+            return fail(ctx, TNNI_E_INVALID, "tile " + std::to_string(k) + ": null pointer");
+        tnni_frame_info f;
+        std::string err;
+        tnni_status st = compute_frame(descs[k], f, err);
+        if (st != TNNI_OK) return fail(ctx, st, "tile " + std::to_string(k) + ": " + err);
+        groups[shape_class(descs[k], n[k], f, bat)].push_back(k);
+        pix[(size_t)k] = (uint64_t)f.W * (uint64_t)f.H;
+    }
+    int written = 0;
+    for (const auto &g : groups) {
+        const std::vector<int> &idx = g.second;
+        tnni_tile_desc d0 = descs[idx[0]];
+        d0.bucket_px = 0;
+        d0.tile_cells = 0;
+        TileDev t0;
+        PrepInfo i0;
+        std::string err;
+        tnni_status st = prepare_tile(d0, n[idx[0]], ctx->n_cu, bat, nullptr, t0, i0, err);
+        if (st != TNNI_OK) return fail(ctx, st, err);
+        const int bm0 = t0.bm, tc0 = t0.tc;
+        uint64_t p = 0;
+        for (int k : idx) p += pix[(size_t)k];
+        const int replicas =
+            bat ? (int)std::max<uint64_t>(1, std::min<uint64_t>(64, ((uint64_t)1 << 25) / std::max<uint64_t>(p, 1)))
+                : 1;
+        // candidates: buckets around the built-in side, the most lane-efficient
+        // cell tiles under the LDS limit (for a batch, the four-workgroup
+        // limit: one launch runs every tile at the largest LDS among them)
+        std::vector<int> bms;
+        for (int b : {bm0, bm0 - 1, bm0 + 1, bm0 + 2})
+            if (b >= 1 && std::find(bms.begin(), bms.end(), b) == bms.end()) bms.push_back(b);
+        std::vector<int> tcs{tc0};
+        for (int c : rank_tile_cells(t0, bat ? kLdsBudget4 : kLdsBudget, bat ? 0 : ctx->n_cu)) {
+            if (tcs.size() >= 5) break;
+            if (std::find(tcs.begin(), tcs.end(), c) == tcs.end()) tcs.push_back(c);
+        }
+        float ms_default = 0.f;
+        st = time_setting(ctx, idx, descs, dX, dY, dZ, n, bat, replicas, bm0, tc0, 0, reps, ms_default);
+        if (st != TNNI_OK) return st;
+        TuneChoice best;
+        best.bucket_px = bm0;
+        best.tile_cells = tc0;
+        float best_ms = ms_default;
+        int tried = 1;
+        for (int b : bms) {
+            for (int c : tcs) {
+                if (b == bm0 && c == tc0) continue;
+                float ms = 0.f;
+                st = time_setting(ctx, idx, descs, dX, dY, dZ, n, bat, replicas, b, c, 0, reps, ms);
+                if (st == TNNI_E_GEOMETRY) continue;  // e.g. a bucket grid too large
+                if (st != TNNI_OK) return st;
+                ++tried;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best.bucket_px = b;
+                    best.tile_cells = c;
+                }
+            }
+        }
+        // fewer workgroups per CU at the best sizes (kept only if clearly faster)
+        {
+            tnni_tile_desc db = d0;
+            db.bucket_px = best.bucket_px;
+            db.tile_cells = best.tile_cells;
+            TileDev tb;
+            PrepInfo ib;
+            if ((st = prepare_tile(db, n[idx[0]], ctx->n_cu, bat, nullptr, tb, ib, err)) != TNNI_OK)
+                return fail(ctx, st, err);
+            const int occ = tile_kernel_occupancy(ib.lds);
+            for (int wg = occ - 1; wg >= std::max(1, occ - 2); --wg) {
+                float ms = 0.f;
+                st = time_setting(ctx, idx, descs, dX, dY, dZ, n, bat, replicas, best.bucket_px,
+                                  best.tile_cells, wg, reps, ms);
+                if (st != TNNI_OK) return st;
+                ++tried;
+                if (ms < 0.98f * best_ms) {
+                    best_ms = ms;
+                    best.wg_per_cu = wg;
+                }
+            }
+        }
+        ctx->tuned.put(g.first, best);
+        if (out) {
+            tnni_tune_entry &e = out[written];
+            e.shape_class = g.first;
+            e.bucket_px = best.bucket_px;
+            e.tile_cells = best.tile_cells;
+            e.wg_per_cu = best.wg_per_cu;
+            e.candidates = tried;
+            e.ms_default = ms_default;
+            e.ms_tuned = best_ms;
+        }
+        ++written;
+    }
+    if (n_classes) *n_classes = written;
+    return TNNI_OK;
+}
+
+tnni_status tnni_tune_read(tnni_ctx *ctx, tnni_tune_entry *out, int cap, int *n)
+{
+    if (!ctx || !n || (cap > 0 && !out)) return fail(ctx, TNNI_E_INVALID, "null argument");
+    const auto e = ctx->tuned.entries();
+    *n = (int)e.size();
+    for (int i = 0; i < std::min(cap, *n); ++i) {
+        std::memset(&out[i], 0, sizeof out[i]);
+        out[i].shape_class = e[(size_t)i].first;
+        out[i].bucket_px = e[(size_t)i].second.bucket_px;
+        out[i].tile_cells = e[(size_t)i].second.tile_cells;
+        out[i].wg_per_cu = e[(size_t)i].second.wg_per_cu;
+    }
+    return TNNI_OK;
+}
+
+tnni_status tnni_tune_load(tnni_ctx *ctx, const tnni_tune_entry *in, int n)
+{
+    if (!ctx || n < 0 || (n && !in)) return fail(ctx, TNNI_E_INVALID, "null argument");
+    for (int i = 0; i < n; ++i)
+        if (in[i].bucket_px < 0 || in[i].tile_cells < 0 || in[i].tile_cells > 64 || in[i].wg_per_cu < 0)
+            return fail(ctx, TNNI_E_INVALID, "tune entry " + std::to_string(i) + " out of range");
+    for (int i = 0; i < n; ++i) {
+        TuneChoice c;
+        c.bucket_px = in[i].bucket_px;
+        c.tile_cells = in[i].tile_cells;
+        c.wg_per_cu = in[i].wg_per_cu;
+        ctx->tuned.put(in[i].shape_class, c);
+    }
+    return TNNI_OK;
+}
+
+tnni_status tnni_tune_clear(tnni_ctx *ctx)
+{
+    if (!ctx) return fail(ctx, TNNI_E_INVALID, "null argument");
+    ctx->tuned.clear();
     return TNNI_OK;
 }
 

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "tnni_internal.h"
 
@@ -94,10 +95,93 @@ tnni_status compute_frame(const tnni_tile_desc &d, tnni_frame_info &f, std::stri
     return TNNI_E_GEOMETRY;
 }
 
+// Shape class of a tile: what decides its best bucket and cell-tile sizes
+// (algorithm, accumulation, scale, query span and site radius in pixels,
+// grid side to a power of two, point density to half a power of two, and
+// whether it runs alone or in a batch).
+uint64_t shape_class(const tnni_tile_desc &d, uint64_t n, const tnni_frame_info &f, bool batched)
+{
+    const int qa = -floordiv_i(f.Rqb + f.x0, f.ps_u), qb = floordiv_i(f.Rqb - f.x0, f.ps_u);
+    const int span = std::min(255, std::max(0, qb - qa + 1));
+    const int rs_px = std::min(255, f.Rsb / std::max(1, f.ps_u));
+    const int side = std::max(d.nrows, d.ncols);
+    int gc = 0;
+    while ((1 << gc) < side && gc < 31) ++gc;
+    const double per_px = (double)n / ((double)f.W * (double)f.H);
+    const int dc = n == 0 ? 0 : std::max(1, std::min(127, (int)std::lround(2.0 * std::log2(per_px)) + 64));
+    return (uint64_t)(d.algorithm & 1) | ((uint64_t)(d.accum_mode & 3) << 1) |
+           ((uint64_t)std::min(d.scale, 255) << 3) | ((uint64_t)span << 11) | ((uint64_t)rs_px << 19) |
+           ((uint64_t)gc << 27) | ((uint64_t)dc << 32) | ((uint64_t)(batched ? 1 : 0) << 39);
+}
+
+// Set the cell-tile size of t to tc and return the LDS its items need: the
+// per-wave stages plus the label tile of the widest item (the first and
+// last items of a row also own the raster border, interior ones are alike).
+size_t tile_lds_bytes(TileDev &t, int tc)
+{
+    t.tc = tc;
+    t.ntx = (t.ncols + tc - 1) / tc;
+    t.nty = (t.nrows + tc - 1) / tc;
+    int rw = 0, rh = 0;
+    const int xs[3] = {0, std::min(1, t.ntx - 1), t.ntx - 1};
+    const int ys[3] = {0, std::min(1, t.nty - 1), t.nty - 1};
+    for (int ti : xs) {
+        const Region g = region_of(t, ti, 0);
+        rw = std::max(rw, g.rk1 - g.rk0 + 1);
+    }
+    for (int tj : ys) {
+        const Region g = region_of(t, 0, tj);
+        rh = std::max(rh, g.rl1 - g.rl0 + 1);
+    }
+    return (size_t)rw * rh * 8 + tile_kernel_stage_bytes();
+}
+
+// Useful share of the label lanes of an interior item of tc cells: it labels
+// (tc-1)*scale + span pixels a side in 8x8 wave blocks, four waves a
+// workgroup, for the tc*scale pixels a side its cells own.
+double tile_efficiency(int tc, int scale, int span)
+{
+    const int P = (tc - 1) * scale + span;
+    const int nb = (P + 7) / 8, blocks = nb * nb;
+    const double own = (double)tc * scale;
+    return own * own / (64.0 * blocks) * (double)blocks / (4.0 * ((blocks + 3) / 4));
+}
+
+// Cell-tile sizes 1..64 whose items fit `budget` bytes of LDS, best lane
+// efficiency first (ties: the larger tile).  min_items_cu > 0 (plain form):
+// sizes above 16 that leave fewer than 2 * min_items_cu items go last.
+std::vector<int> rank_tile_cells(TileDev &t, size_t budget, int min_items_cu)
+{
+    const int qa = query_lo(t), qb = query_hi(t);
+    const int span = std::max(1, qb - qa + 1);
+    struct C {
+        int tc;
+        double e;
+        bool few;
+    };
+    std::vector<C> c;
+    for (int tc = 1; tc <= 64; ++tc) {
+        if (tile_lds_bytes(t, tc) > budget) continue;
+        const bool few = min_items_cu > 0 && tc > 16 && (int64_t)t.ntx * t.nty < 2ll * min_items_cu;
+        c.push_back({tc, tile_efficiency(tc, t.scale, span), few});
+    }
+    std::stable_sort(c.begin(), c.end(), [](const C &a, const C &b) {
+        if (a.few != b.few) return !a.few;
+        if (std::fabs(a.e - b.e) > 1e-9) return a.e > b.e;
+        return a.tc > b.tc;
+    });
+    std::vector<int> out;
+    for (const C &x : c) out.push_back(x.tc);
+    if (!out.empty()) tile_lds_bytes(t, out[0]);
+    return out;
+}
+
 // Fill everything of a TileDev that depends on the descriptor and the
-// point count (not on the batch layout).  lds_out: bytes of LDS it needs.
-tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool batched, TileDev &t,
-                         size_t &lds_out, std::string &err)
+// point count (not on the batch layout).  tuned (nullable): bucket and
+// cell-tile sizes measured for the tile's shape class, used where the
+// descriptor leaves them 0.
+tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool batched,
+                         const TuneTable *tuned, TileDev &t, PrepInfo &info, std::string &err)
 {
     tnni_frame_info f;
     tnni_status st = compute_frame(d, f, err);
@@ -153,7 +237,12 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
     // Bucket grid over that window: square buckets of bm x bm pixels, edges
     // on pixel edges, so pixel k sits in bucket column bK + k / bm.  bm is
     // set for about one site per bucket at the tile's mean density.
-    int bm = d.bucket_px;
+    info = PrepInfo();
+    info.shape_class = shape_class(d, n, f, batched);
+    TuneChoice tc_tuned;
+    if (tuned && (d.bucket_px <= 0 || d.tile_cells <= 0) && tuned->lookup(info.shape_class, tc_tuned))
+        info.wg_per_cu = tc_tuned.wg_per_cu;
+    int bm = d.bucket_px > 0 ? d.bucket_px : tc_tuned.bucket_px;
     if (bm <= 0) {
         const double per_px = (double)n / ((double)f.W * (double)f.H);
         bm = per_px > 0 ? (int)std::lround(std::sqrt(1.0 / per_px)) : 64;
@@ -184,7 +273,12 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
     t.bm = bm;
     t.bK = (int)K;
     t.ecap = (int)std::min<int64_t>(250, f.Rsb / bs + 2);
-    t.br = (int)std::max<int64_t>(1, std::min<int64_t>(nby, kBandBuckets / nbx));
+    // bands of bucket rows: at most kBandBuckets buckets (the LDS counters of
+    // one band-sort workgroup) and, at the tile's mean density, about
+    // kBandPts points, so that dense tiles spread over many workgroups
+    const int64_t per_row = std::max<int64_t>(1, (int64_t)n / nby);
+    const int64_t br_pts = std::max<int64_t>(1, kBandPts / per_row);
+    t.br = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(nby, kBandBuckets / nbx), br_pts));
     t.nbands = (int)((nby + t.br - 1) / t.br);
     t.inv_br = 1.0 / (double)t.br;
     if (nbx > kBandBuckets || t.nbands > kMaxBands) {
@@ -196,50 +290,27 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
     // A site of ring 0 or 1 around a pixel's bucket is within 2*bs of the
     // pixel on both axes (strictly less than 2*bs).
     t.ring1_covered = d.algorithm == TNNI_ALG_PLANES ? (2 * bs <= f.Rs) : (8 * bs * bs <= f.Rs2);
-    // cell tiles: the largest that fits the LDS budget; the plain form
-    // shrinks them so that a lone small tile still spreads over the CUs
-    auto lds_for = [&t, &d](int tc) {
-        t.tc = tc;
-        t.ntx = (d.ncols + tc - 1) / tc;
-        t.nty = (d.nrows + tc - 1) / tc;
-        int rw = 0, rh = 0;
-        for (int ti = 0; ti < t.ntx; ++ti) {
-            const Region g = region_of(t, ti, 0);
-            rw = std::max(rw, g.rk1 - g.rk0 + 1);
-        }
-        for (int tj = 0; tj < t.nty; ++tj) {
-            const Region g = region_of(t, 0, tj);
-            rh = std::max(rh, g.rl1 - g.rl0 + 1);
-        }
-        return (size_t)rw * rh * 8 + tile_kernel_stage_bytes();
-    };
+    // cell tiles: the most lane-efficient size that fits the LDS budget
+    // of four workgroups per CU; the plain form avoids sizes that leave a
+    // lone small tile with fewer items than twice the CUs
+    const int tc_fixed = d.tile_cells > 0 ? d.tile_cells : tc_tuned.tile_cells;
     size_t lds = 0;
-    if (d.tile_cells > 0) {
-        lds = lds_for(d.tile_cells);
+    if (tc_fixed > 0) {
+        lds = tile_lds_bytes(t, tc_fixed);
         if (lds > kLdsBudget) {
             err = "tile_cells too large for the LDS budget";
             return TNNI_E_GEOMETRY;
         }
     } else {
-        // the largest cell tile that leaves room for four workgroups per CU
-        // (16 waves), else the largest that fits at all
-        static const int kTc[] = {64, 48, 40, 32, 24, 16, 12, 8, 6, 4, 3, 2, 1};
-        int chosen = -1;
-        size_t budget = kLdsBudget4;
-        for (int pass = 0; pass < 2 && chosen < 0; ++pass, budget = kLdsBudget)
-        for (int tc : kTc) {
-            if (lds_for(tc) > budget) continue;
-            chosen = tc;
-            if (!batched && tc > 16 && (int64_t)t.ntx * t.nty < 2ll * n_cu) continue;
-            break;
-        }
-        if (chosen < 0) {
+        std::vector<int> order = rank_tile_cells(t, kLdsBudget4, batched ? 0 : n_cu);
+        if (order.empty()) order = rank_tile_cells(t, kLdsBudget, batched ? 0 : n_cu);
+        if (order.empty()) {
             err = "no cell-tile size fits the LDS budget for this geometry";
             return TNNI_E_GEOMETRY;
         }
-        lds = lds_for(chosen);
+        lds = tile_lds_bytes(t, order[0]);
     }
-    lds_out = lds;
+    info.lds = lds;
     return TNNI_OK;
 }
 

@@ -4,7 +4,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "terranni.h"
 
@@ -18,6 +22,7 @@ constexpr int kScanBlocks = 512;       // fixed grid of the device-wide scan
 constexpr int kScanItems = 4;          // items per thread per scan step
 constexpr int kMaxBands = 8192;        // bucket-row bands per tile (LDS histogram)
 constexpr int kBandBuckets = 8192;     // buckets per band (LDS counters)
+constexpr int kBandPts = 16384;        // points per band aimed at (mean density)
 
 // Everything a kernel needs to know about one tile of a batch.  Lengths are
 // in the tile-local fixed-point unit (1/upm metre).
@@ -166,10 +171,61 @@ constexpr size_t kLdsBudget4 = 40 * 1024;  // four workgroups per CU (160 KB LDS
 
 tnni_status compute_frame(const tnni_tile_desc &d, tnni_frame_info &f, std::string &err);
 
+// Bucket side, cell-tile side and workgroups per CU chosen for one shape
+// class (0 = the built-in choice).
+struct TuneChoice {
+    int bucket_px = 0, tile_cells = 0, wg_per_cu = 0;
+};
+
+// Measured choices per shape class; shared by the forms of one context.
+class TuneTable {
+public:
+    bool lookup(uint64_t cls, TuneChoice &c) const
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = map_.find(cls);
+        if (it == map_.end()) return false;
+        c = it->second;
+        return true;
+    }
+    void put(uint64_t cls, const TuneChoice &c)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        map_[cls] = c;
+    }
+    void clear()
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        map_.clear();
+    }
+    std::vector<std::pair<uint64_t, TuneChoice>> entries() const
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        return std::vector<std::pair<uint64_t, TuneChoice>>(map_.begin(), map_.end());
+    }
+
+private:
+    mutable std::mutex mu_;
+    std::map<uint64_t, TuneChoice> map_;
+};
+
+// What prepare_tile derives besides the TileDev.
+struct PrepInfo {
+    size_t lds = 0;            // dynamic LDS of the tile kernel for this tile
+    int wg_per_cu = 0;         // tuned workgroups per CU (0 = all that fit)
+    uint64_t shape_class = 0;  // the tile's shape class
+};
+
+uint64_t shape_class(const tnni_tile_desc &d, uint64_t n, const tnni_frame_info &f, bool batched);
+size_t tile_lds_bytes(TileDev &t, int tc);
+double tile_efficiency(int tc, int scale, int span);
+std::vector<int> rank_tile_cells(TileDev &t, size_t budget, int min_items_cu);
+
 // Fill everything of a TileDev that depends on the descriptor and the point
 // count (not on the batch layout).  batched: the tile shares the GPU with
-// others, so its cell tiles need not be shrunk to fill the CUs.
-tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool batched, TileDev &t,
-                         size_t &lds_out, std::string &err);
+// others, so its cell tiles need not be shrunk to fill the CUs.  tuned
+// (nullable): measured sizes for the tile's shape class.
+tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool batched,
+                         const TuneTable *tuned, TileDev &t, PrepInfo &info, std::string &err);
 
 }  // namespace tnni

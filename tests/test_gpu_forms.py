@@ -152,3 +152,43 @@ def test_served_device_resident_equals_plain(ctx):
         for t, d in zip(tiles, dev):
             h0, _, _ = ctx.tile_run(t.params, t.X, t.Y, t.Z)
             assert np.array_equal(d[3].to_host().view(np.uint32), h0.view(np.uint32))
+
+
+def test_autotune_keeps_results_and_table_round_trips():
+    """Tuned bucket and cell-tile sizes change the work split, never a
+    result: every tile is bit-identical before and after tuning, in the
+    batched and the plain form; the table reads back, clears and reloads."""
+    tiles = _mixed_tiles(8, seed=300)
+    with T.Context(0) as c:
+        dev = _upload(c, tiles)
+        P = [t.params for t in tiles]
+        cols = ([d[0] for d in dev], [d[1] for d in dev], [d[2] for d in dev])
+        ns = [t.n for t in tiles]
+        hs = [d[3] for d in dev]
+        c.batch_run_dev(P, *cols, ns, hs)
+        ref = [h.to_host() for h in hs]
+        n_classes = 0
+        for batched in (True, False):
+            _say(f"autotune batched={batched}")
+            ent = c.autotune(P, *cols, ns, batched=batched, reps=1)
+            want = {T.shape_class(t.params, t.n, batched) for t in tiles}
+            assert {e["shape_class"] for e in ent} == want
+            for e in ent:
+                assert e["candidates"] >= 1
+                assert e["ms_tuned"] <= e["ms_default"]
+                assert e["bucket_px"] >= 1 and 1 <= e["tile_cells"] <= 64 and e["wg_per_cu"] >= 0
+            n_classes += len(want)
+        c.batch_run_dev(P, *cols, ns, hs)
+        for r, h in zip(ref, hs):
+            assert np.array_equal(r.view(np.uint32), h.to_host().view(np.uint32))
+        for t, d, r in zip(tiles, dev, ref):
+            c.tile_run_dev(t.params, d[0], d[1], d[2], t.n, d[3])
+            assert np.array_equal(r.view(np.uint32), d[3].to_host().view(np.uint32))
+        tab = c.tune_table()
+        assert len(tab) == n_classes
+        c.tune_clear()
+        assert c.tune_table() == []
+        c.tune_load(tab)
+        key = lambda e: e["shape_class"]  # noqa: E731
+        assert sorted(c.tune_table(), key=key) == sorted(tab, key=key)
+        _free(dev)

@@ -35,3 +35,19 @@ def test_nni_init_without_gpu_fails_cleanly(tmp_path):
     r = subprocess.run([exe, str(tmp_path / "p.bin"), str(tmp_path / "o.bin")],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 4 and "engine initialisation failed" in r.stderr
+
+
+def test_shape_class_separates_density_grid_and_form():
+    """The tuned table's key (host only): tiles alike in geometry and density
+    share a class; the density class, the grid side class and the form
+    (plain / batched) each split it."""
+    import terranni_amd as T
+    p = T.TileParams(nrows=1000, ncols=1000, cell_size=0.5, scale=1)
+    base = T.shape_class(p, 1_000_000, True)
+    assert T.shape_class(p, 1_050_000, True) == base
+    assert T.shape_class(p, 4_000_000, True) != base
+    assert T.shape_class(p, 1_000_000, False) != base
+    q = T.TileParams(nrows=2000, ncols=2000, cell_size=0.5, scale=1)
+    assert T.shape_class(q, 4_000_000, True) != T.shape_class(p, 1_000_000, True)
+    c = T.TileParams(nrows=1000, ncols=1000, cell_size=0.5, scale=1, algorithm=T.ALG_CONES)
+    assert T.shape_class(c, 1_000_000, True) != base
