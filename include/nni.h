@@ -99,6 +99,12 @@ void clearRenderbuffers();
 // two-sum, 2 the reference's zMultiplier integer arithmetic (needs
 // Settings::pointMin / multiplier).  Takes effect at the next performNNI.
 void setAccumulationMode(int mode);
+// FP64 origin (metres) of the next performNNI's work package.  The
+// WorkPackage's float origin rounds projected coordinates (UTM eastings of
+// 10^5-10^6 m) to centimetres or worse; a driver that calls this first gets
+// the tile placed exactly.  performNNI consumes it (later calls fall back to
+// the float origin until it is set again).
+void setOrigin64(double origin_x, double origin_y);
 // Status of the last call (0 = TNNI_OK) and its message.
 int lastStatus();
 const char *lastError();

@@ -72,6 +72,17 @@ def build_nni_api(verbose: bool = False) -> str:
     return out
 
 
+def build_driver(verbose: bool = False) -> str:
+    """terranni: the streaming DEM driver (LAS in, ESRI grids out) over
+    libterranni_nni.so (plain engine) and libterranni_hip.so (batched)."""
+    out = os.path.join(LIB_DIR, "terranni")
+    _run([CXX, *CXX_FLAGS, "-fopenmp", "-I", os.path.join(ROOT, "include"),
+          os.path.join(CSRC, "driver", "terranni.cpp"), "-o", out + ".tmp", "-L", LIB_DIR,
+          "-lterranni_nni", "-lterranni_hip", "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle(verbose: bool = False):
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], verbose)
 
@@ -79,6 +90,7 @@ def build_oracle(verbose: bool = False):
 def build_all(verbose: bool = False):
     build_lib(verbose)
     build_nni_api(verbose)
+    build_driver(verbose)
     build_oracle(verbose)
 
 

@@ -1,0 +1,826 @@
+// terranni: TerraNNI's streaming DEM driver (terranni.cpp of the reference)
+// over the MI355X tile engine.
+//
+// The steps of the reference's main (terranni.cpp:491-731):
+//   1. find the .las files under --tilepath and their bounding box
+//      (terranni.cpp:86-113, 585-597), here in FP64;
+//   2. derive the settings: border, query radius, zMultiplier, work-package
+//      size (terranni.cpp:656-714);
+//   3. bin the points into one stream per work package, every point within
+//      `border` of a package copied into it, and sort each stream by time
+//      (terranni.cpp:198-368), unless --wp-computed;
+//   4. interpolate every package and time slice (binRun, terranni.cpp:394-426)
+//      and write the grids (outputASC, terranni.cpp:430-488).
+//
+// Two engines run step 4:
+//   --engine plain (default): the reference's loop, nni::performNNI per
+//     package and time slice into per-slice OutPoint streams, sorted and
+//     written as ESRI ASCII afterwards.  Packages are streams of
+//     nni::Point4D, r<row>c<col>.tpie, as in the reference.
+//   --engine batched: packages hold the LAS integers (X, Y, Z, time; 16 bytes
+//     a point) in one common LAS scale/offset; every package of a row of
+//     packages is submitted to the engine's served form at once, the GPU
+//     de-scales in FP64 in each package's frame, and the rasters go straight
+//     to the output rows (no OutPoint streams, no sort).  The next row of
+//     packages is on the GPU while the previous one is written.
+// Outputs: ESRI ASCII grid (<slice>.tpie.asc, the reference's name) and/or
+// an ESRI GridFloat raster (<slice>.flt + .hdr, float32, rows top down).
+#include <algorithm>
+#include <chrono>
+#include <cinttypes>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <limits>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "las_reader.h"
+#include "nni.h"
+#include "terranni.h"
+#include "tpie/sort.h"
+#include "tpie/stream.h"
+#include "tpie/tpie.h"
+
+namespace fs = std::filesystem;
+
+namespace {
+
+[[noreturn]] void die(const std::string &msg)
+{
+    std::fprintf(stderr, "terranni: %s\n", msg.c_str());
+    std::exit(1);
+}
+
+double wall()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ---- options (terranni.cpp:522-553, plus this driver's own) --------------
+
+struct Options {
+    std::string tilepath, output = "./", tmp_path = ".", wp_path = ".";
+    int scale = 5;
+    float cell_size = 2.0f, site_radius = 5.0f;
+    bool has_qr = false;
+    float qr = 0.f;
+    bool has_wpw = false, has_wph = false;
+    int wp_width = 0, wp_height = 0;
+    bool wp_computed = false;
+    bool has_ox = false, has_oy = false;
+    double ox = 0, oy = 0;
+    bool has_gc = false, has_gr = false;
+    int grid_cols = 0, grid_rows = 0;
+    int time_radius = 0;
+    float time_start = 0.f, time_length = 1.f, time_step = 1.f;
+    bool two_d = false, delta_squared = false, draw_cones = false, help = false;
+    std::string engine = "plain", format = "asc", accum = "fixed64";
+    int max_size = 5000;
+    bool bin_only = false;
+};
+
+const char *kUsage =
+    "Allowed options (the reference's, terranni.cpp:522-553):\n"
+    "  --tilepath P        path to the .las tiles (required)\n"
+    "  --output P          output folder (default ./)\n"
+    "  --scaling N         pixels per cell side (default 5)\n"
+    "  --cell-size C       cell size in metres (default 2.0)\n"
+    "  --site-radius R     site region radius in metres (default 5.0)\n"
+    "  --query-radius R    query region radius (default 5*cell/2 - 0.01)\n"
+    "  --wp-width W        work package width in metres\n"
+    "  --wp-height H       work package height in metres\n"
+    "  --tmp-path P        temporary files (default .)\n"
+    "  --wp-path P         work package streams (default .)\n"
+    "  --wp-computed       reuse the work package streams of an earlier run\n"
+    "  --origin-x X        grid origin, x\n"
+    "  --origin-y Y        grid origin, y\n"
+    "  --grid-cols N       grid columns\n"
+    "  --grid-rows N       grid rows\n"
+    "  --time-radius N     time radius (default 0)\n"
+    "  --time-start T      first time slice (default 0)\n"
+    "  --time-length L     time slices to interpolate (default 1)\n"
+    "  --time-step S       time step (default 1)\n"
+    "  --2d                ignore the points' times: one 2-D slice\n"
+    "  --delta-squared     the Delta^2 label cache instead of Delta\n"
+    "  --draw-cones        cones instead of planes\n"
+    "  --help              this message\n"
+    "This driver's own:\n"
+    "  --engine plain|batched   nni.h per package (default) or served batches\n"
+    "  --format asc|flt|both    ESRI ASCII (default), GridFloat, or both\n"
+    "  --accum fixed64|fp32|s0  height arithmetic (default fixed64)\n"
+    "  --max-size N        largest raster side in pixels (default 5000, as the\n"
+    "                      reference forces at terranni.cpp:697-698)\n"
+    "  --bin-only          stop after binning\n";
+
+Options parse(int argc, char **argv)
+{
+    Options o;
+    static const char *flags[] = {"wp-computed", "2d", "delta-squared", "draw-cones", "help", "bin-only"};
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        if (a.rfind("--", 0) != 0) die("unexpected argument " + a + "\n" + kUsage);
+        a = a.substr(2);
+        std::string name = a, val;
+        bool has_val = false;
+        const size_t eq = a.find('=');
+        if (eq != std::string::npos) {
+            name = a.substr(0, eq);
+            val = a.substr(eq + 1);
+            has_val = true;
+        }
+        const bool is_flag = std::find_if(std::begin(flags), std::end(flags),
+                                          [&](const char *f) { return name == f; }) != std::end(flags);
+        if (is_flag) {
+            if (name == "wp-computed") o.wp_computed = true;
+            else if (name == "2d") o.two_d = true;
+            else if (name == "delta-squared") o.delta_squared = true;
+            else if (name == "draw-cones") o.draw_cones = true;
+            else if (name == "help") o.help = true;
+            else if (name == "bin-only") o.bin_only = true;
+            continue;
+        }
+        if (!has_val) {
+            if (i + 1 >= argc) die("--" + name + " needs a value");
+            val = argv[++i];
+        }
+        auto num = [&](double &d) {
+            char *end = nullptr;
+            d = std::strtod(val.c_str(), &end);
+            if (end == val.c_str() || *end) die("--" + name + ": not a number: " + val);
+        };
+        double d = 0;
+        if (name == "tilepath") o.tilepath = val;
+        else if (name == "output") o.output = val;
+        else if (name == "tmp-path") o.tmp_path = val;
+        else if (name == "wp-path") o.wp_path = val;
+        else if (name == "engine") o.engine = val;
+        else if (name == "format") o.format = val;
+        else if (name == "accum") o.accum = val;
+        else {
+            num(d);
+            if (name == "scaling") o.scale = (int)d;
+            else if (name == "cell-size") o.cell_size = (float)d;
+            else if (name == "site-radius") o.site_radius = (float)d;
+            else if (name == "query-radius") { o.has_qr = true; o.qr = (float)d; }
+            else if (name == "wp-width") { o.has_wpw = true; o.wp_width = (int)d; }
+            else if (name == "wp-height") { o.has_wph = true; o.wp_height = (int)d; }
+            else if (name == "origin-x") { o.has_ox = true; o.ox = d; }
+            else if (name == "origin-y") { o.has_oy = true; o.oy = d; }
+            else if (name == "grid-cols") { o.has_gc = true; o.grid_cols = (int)d; }
+            else if (name == "grid-rows") { o.has_gr = true; o.grid_rows = (int)d; }
+            else if (name == "time-radius") o.time_radius = (int)d;
+            else if (name == "time-start") o.time_start = (float)d;
+            else if (name == "time-length") o.time_length = (float)d;
+            else if (name == "time-step") o.time_step = (float)d;
+            else if (name == "max-size") o.max_size = (int)d;
+            else die("unknown option --" + name + "\n" + kUsage);
+        }
+    }
+    if (o.engine != "plain" && o.engine != "batched") die("--engine must be plain or batched");
+    if (o.format != "asc" && o.format != "flt" && o.format != "both") die("--format must be asc, flt or both");
+    if (o.accum != "fixed64" && o.accum != "fp32" && o.accum != "s0") die("--accum must be fixed64, fp32 or s0");
+    return o;
+}
+
+// ---- the grid and its work packages ---------------------------------------
+
+struct GridBox {
+    double minx, miny, maxx, maxy, minz, maxz;
+};
+
+// Work packages: wp_nc x wp_nr cells each, cols x rows of them from the grid
+// origin.  Sizes are whole cells, so package origins sit on cell edges.
+struct Layout {
+    double minx, miny;   // grid origin
+    double cs;           // cell size
+    int nrcols, nrrows;  // cells of the grid
+    int wp_nc, wp_nr;    // cells of one package
+    int cols, rows;      // packages
+    double border;       // metres of neighbouring points each package takes
+    double wpw() const { return wp_nc * cs; }
+    double wph() const { return wp_nr * cs; }
+    double origin_x(int col) const { return minx + col * wpw(); }
+    double origin_y(int row) const { return miny + row * wph(); }
+};
+
+// Packages c on one axis whose extent [c*w - b, (c+1)*w + b] holds v
+// (metres from the grid origin).
+bool wp_span(double v, double w, double b, int n, int &lo, int &hi)
+{
+    lo = std::max(0, (int)std::ceil((v - b) / w - 1.0));
+    hi = std::min(n - 1, (int)std::floor((v + b) / w));
+    return lo <= hi;
+}
+
+std::vector<std::string> find_tiles(const std::string &dir)
+{
+    std::vector<std::string> out;
+    if (!fs::exists(dir)) return out;
+    for (const auto &e : fs::recursive_directory_iterator(dir))
+        if (e.is_regular_file() && e.path().extension() == ".las") out.push_back(e.path().string());
+    std::sort(out.begin(), out.end());  // a fixed order: point indices decide ties
+    return out;
+}
+
+// The file name of a time slice (terranni.cpp:409,443).
+std::string slice_name(const std::string &outdir, float t)
+{
+    std::stringstream ss;
+    ss << outdir << "/" << std::setw(5) << std::left << std::showpoint << t;
+    return ss.str();
+}
+
+std::vector<float> time_slices(const Options &o)
+{
+    std::vector<float> v;
+    for (float t = o.time_start; t <= o.time_start + o.time_length; t += o.time_step) v.push_back(t);
+    return v;
+}
+
+// ---- binning ---------------------------------------------------------------
+
+// A point of the batched engine's packages: LAS integers in the common
+// scale/offset, and the time.
+struct RawPoint {
+    int32_t X, Y, Z, t;
+};
+
+struct Timecmp {  // terranni.cpp:119-126
+    int compare(const nni::Point4D &a, const nni::Point4D &b) { return a.t - b.t; }
+    int compare(const RawPoint &a, const RawPoint &b) { return a.t - b.t; }
+};
+
+std::string wp_file(const std::string &dir, int row, int col, const char *ext)
+{
+    return dir + "/r" + std::to_string(row) + "c" + std::to_string(col) + ext;
+}
+
+// The common LAS frame of the batched engine's packages: the first file's
+// scale and offset; points of files with another frame are re-quantised to
+// it (rounded to its nearest integer).
+struct Frame {
+    double scale[3], offset[3];
+};
+
+// Bin every point of `files` into the streams of the packages whose extent
+// holds it, then sort each stream by time (stable, so equal times keep the
+// file order).  At most kMaxOpen streams are open at once: more packages take
+// more passes over the input.
+template <class Rec, class Make>
+void bin_points(const std::vector<std::string> &files, const Layout &L, const std::string &dir,
+                const char *ext, Make make)
+{
+    constexpr int kMaxOpen = 256;
+    const int n_wp = L.cols * L.rows;
+    std::vector<int32_t> X, Y, Z;
+    std::vector<uint16_t> I;
+    const size_t kBlock = 1 << 16;
+    X.resize(kBlock);
+    Y.resize(kBlock);
+    Z.resize(kBlock);
+    I.resize(kBlock);
+    for (int g0 = 0; g0 < n_wp; g0 += kMaxOpen) {
+        const int g1 = std::min(n_wp, g0 + kMaxOpen);
+        std::vector<std::unique_ptr<tpie::ami::stream<Rec>>> st;
+        for (int k = g0; k < g1; ++k) {
+            st.emplace_back(new tpie::ami::stream<Rec>(wp_file(dir, k / L.cols, k % L.cols, ext)));
+            if (!st.back()->is_valid()) die("cannot create " + wp_file(dir, k / L.cols, k % L.cols, ext));
+            st.back()->truncate(0);
+        }
+        for (const std::string &f : files) {
+            nni_io::las_file lf;
+            lf.open(f);
+            const nni_io::las_header &h = lf.header();
+            size_t got;
+            while ((got = lf.read_raw(kBlock, X.data(), Y.data(), Z.data(), I.data())) > 0) {
+                for (size_t i = 0; i < got; ++i) {
+                    const double x = (double)X[i] * h.scale[0] + h.offset[0];
+                    const double y = (double)Y[i] * h.scale[1] + h.offset[1];
+                    int c0, c1, r0, r1;
+                    if (!wp_span(x - L.minx, L.wpw(), L.border, L.cols, c0, c1)) continue;
+                    if (!wp_span(y - L.miny, L.wph(), L.border, L.rows, r0, r1)) continue;
+                    const Rec rec = make(h, X[i], Y[i], Z[i], I[i], x, y);
+                    for (int r = r0; r <= r1; ++r)
+                        for (int c = c0; c <= c1; ++c) {
+                            const int k = r * L.cols + c;
+                            if (k >= g0 && k < g1) st[(size_t)(k - g0)]->write_item(rec);
+                        }
+                }
+            }
+        }
+        Timecmp cmp;
+        for (auto &s : st) {
+            s->flush();
+            if (tpie::ami::sort(s.get(), &cmp) != tpie::ami::NO_ERROR) die("sorting a work package failed");
+        }
+    }
+}
+
+// ---- output ----------------------------------------------------------------
+
+// ESRI ASCII grid and / or GridFloat raster, rows written top down.
+class RasterOut {
+public:
+    RasterOut(const std::string &base, const std::string &format, int nrows, int ncols, double xll,
+              double yll, double cs)
+        : ncols_(ncols)
+    {
+        if (format == "asc" || format == "both") {
+            asc_ = std::fopen((base + ".tpie.asc").c_str(), "wb");
+            if (!asc_) die("cannot write " + base + ".tpie.asc");
+            // the reference's header (terranni.cpp:458-464), coordinates at
+            // full precision
+            std::fprintf(asc_, "NROWS %d\nNCOLS %d\nXLLCORNER %.15g\nYLLCORNER %.15g\nCELLSIZE %.15g\nNODATA_VALUE -9999\n",
+                         nrows, ncols, xll, yll, cs);
+        }
+        if (format == "flt" || format == "both") {
+            flt_ = std::fopen((base + ".flt").c_str(), "wb");
+            if (!flt_) die("cannot write " + base + ".flt");
+            std::FILE *h = std::fopen((base + ".hdr").c_str(), "wb");
+            if (!h) die("cannot write " + base + ".hdr");
+            std::fprintf(h, "ncols %d\nnrows %d\nxllcorner %.15g\nyllcorner %.15g\ncellsize %.15g\nNODATA_value -9999\nbyteorder LSBFIRST\n",
+                         ncols, nrows, xll, yll, cs);
+            std::fclose(h);
+        }
+    }
+    ~RasterOut()
+    {
+        if (asc_) std::fclose(asc_);
+        if (flt_) std::fclose(flt_);
+    }
+
+    // Rows of ncols heights, the first row the top one of the batch.
+    void write_rows(const float *rows, int n)
+    {
+        if (flt_ && std::fwrite(rows, sizeof(float), (size_t)n * ncols_, flt_) != (size_t)n * ncols_)
+            die("write failed");
+        if (!asc_) return;
+        // each value as the reference's `ostream << float << " "`: %g of the
+        // value, six significant digits (terranni.cpp:471)
+        std::vector<std::string> text((size_t)n);
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int r = 0; r < n; ++r) {
+            std::string &s = text[(size_t)r];
+            s.reserve((size_t)ncols_ * 10 + 1);
+            char b[48];
+            const float *row = rows + (size_t)r * ncols_;
+            for (int i = 0; i < ncols_; ++i) {
+                const int k = std::snprintf(b, sizeof b, "%g ", (double)row[i]);
+                s.append(b, (size_t)k);
+            }
+            s.push_back('\n');
+        }
+        for (const std::string &s : text)
+            if (std::fwrite(s.data(), 1, s.size(), asc_) != s.size()) die("write failed");
+    }
+
+private:
+    int ncols_;
+    std::FILE *asc_ = nullptr, *flt_ = nullptr;
+};
+
+// ---- plain engine: the reference's binRun + outputASC ----------------------
+
+struct XYcmp {  // terranni.cpp:128-138
+    int compare(const nni::OutPoint &a, const nni::OutPoint &b)
+    {
+        if (a.j != b.j) return b.j - a.j;
+        return a.i - b.i;
+    }
+};
+
+void run_plain(const Options &o, const Layout &L, const nni::Settings &s, const std::vector<float> &slices,
+               double &t_nni, double &t_out)
+{
+    const double t0 = wall();
+    bool first = true;
+    for (int row = 0; row < L.rows; ++row) {
+        for (int col = 0; col < L.cols; ++col) {
+            tpie::ami::stream<nni::Point4D> wps(wp_file(o.wp_path, row, col, ".tpie"), tpie::ami::READ_STREAM);
+            if (!wps.is_valid()) die("missing work package " + wp_file(o.wp_path, row, col, ".tpie"));
+            nni::WorkPackage wp;
+            wp.origin_x = (float)L.origin_x(col);
+            wp.origin_y = (float)L.origin_y(row);
+            wp.ncols = L.wp_nc;
+            wp.nrows = L.wp_nr;
+            wp.grid_origin_x = (float)(col * L.wp_nc);
+            wp.grid_origin_y = (float)(row * L.wp_nr);
+            for (float t : slices) {
+                tpie::ami::stream<nni::OutPoint> out(slice_name(o.output, t) + ".tpie", tpie::ami::APPEND_STREAM);
+                out.persist();
+                if (first) out.truncate(0);
+                nni::setOrigin64(L.origin_x(col), L.origin_y(row));
+                nni::performNNI(wp, out, wps, (float)o.time_radius, t, !o.delta_squared);
+                if (nni::lastStatus() != 0) die(std::string("performNNI: ") + nni::lastError());
+            }
+            first = false;
+            nni::clearRenderbuffers();
+        }
+    }
+    (void)s;
+    t_nni = wall() - t0;
+    const double t1 = wall();
+    XYcmp cmp;
+    std::vector<float> rowbuf((size_t)L.nrcols);
+    for (float t : slices) {
+        const std::string base = slice_name(o.output, t);
+        tpie::ami::stream<nni::OutPoint> out(base + ".tpie", tpie::ami::APPEND_STREAM);
+        out.seek(0);
+        if (tpie::ami::sort(&out, &cmp) != tpie::ami::NO_ERROR) die("sorting the output failed");
+        out.seek(0);
+        RasterOut w(base, o.format, L.nrrows, L.nrcols, L.minx, L.miny, L.cs);
+        nni::OutPoint *p;
+        for (int j = L.nrrows - 1; j >= 0; --j) {
+            for (int i = 0; i < L.nrcols; ++i) {
+                if (out.read_item(&p) != tpie::ami::NO_ERROR) die("output stream ends early");
+                if (p->i != i || p->j != j)
+                    std::fprintf(stderr, "Error in output order: %d,%d -- %d,%d\n", i, j, p->i, p->j);
+                rowbuf[(size_t)i] = p->h;
+            }
+            w.write_rows(rowbuf.data(), 1);
+        }
+    }
+    t_out = wall() - t1;
+}
+
+// ---- batched engine ----------------------------------------------------------
+
+struct Meta {
+    Frame frame;
+    double minx, miny, cs, border;
+    int wp_nc, wp_nr, cols, rows, two_d;
+};
+
+void write_meta(const std::string &dir, const Meta &m)
+{
+    std::FILE *f = std::fopen((dir + "/wp.meta").c_str(), "wb");
+    if (!f) die("cannot write " + dir + "/wp.meta");
+    std::fprintf(f, "format=raw1\n");
+    for (int k = 0; k < 3; ++k)
+        std::fprintf(f, "scale%d=%.17g\noffset%d=%.17g\n", k, m.frame.scale[k], k, m.frame.offset[k]);
+    std::fprintf(f, "minx=%.17g\nminy=%.17g\ncell_size=%.17g\nborder=%.17g\nwp_ncols=%d\nwp_nrows=%d\ncols=%d\nrows=%d\ntwo_d=%d\n",
+                 m.minx, m.miny, m.cs, m.border, m.wp_nc, m.wp_nr, m.cols, m.rows, m.two_d);
+    std::fclose(f);
+}
+
+Meta read_meta(const std::string &dir)
+{
+    std::ifstream in(dir + "/wp.meta");
+    if (!in) die("--wp-computed: no " + dir + "/wp.meta (bin with --engine batched first)");
+    std::map<std::string, std::string> kv;
+    std::string line;
+    while (std::getline(in, line)) {
+        const size_t eq = line.find('=');
+        if (eq != std::string::npos) kv[line.substr(0, eq)] = line.substr(eq + 1);
+    }
+    if (kv["format"] != "raw1") die(dir + "/wp.meta: unknown format");
+    Meta m;
+    auto d = [&](const char *k) { return std::strtod(kv[k].c_str(), nullptr); };
+    for (int k = 0; k < 3; ++k) {
+        m.frame.scale[k] = d(("scale" + std::to_string(k)).c_str());
+        m.frame.offset[k] = d(("offset" + std::to_string(k)).c_str());
+    }
+    m.minx = d("minx");
+    m.miny = d("miny");
+    m.cs = d("cell_size");
+    m.border = d("border");
+    m.wp_nc = (int)d("wp_ncols");
+    m.wp_nr = (int)d("wp_nrows");
+    m.cols = (int)d("cols");
+    m.rows = (int)d("rows");
+    m.two_d = (int)d("two_d");
+    return m;
+}
+
+// One package of a row in flight: its points of the slice and its raster.
+struct Flight {
+    std::vector<int32_t> X, Y, Z;
+    std::vector<float> h;
+    uint64_t ticket = 0;
+};
+
+void run_batched(const Options &o, const Layout &L, const nni::Settings &s, const Meta &m,
+                 const std::vector<float> &slices, double &t_nni, double &t_out)
+{
+    tnni_ctx_opts co;
+    std::memset(&co, 0, sizeof co);
+    if (const char *dev = std::getenv("TERRANNI_DEVICE")) co.device = std::atoi(dev);
+    co.n_slots = 3;
+    tnni_ctx *ctx = nullptr;
+    if (tnni_ctx_create(&co, &ctx) != TNNI_OK) die(std::string("engine: ") + tnni_last_error(nullptr));
+    tnni_tile_desc base;
+    std::memset(&base, 0, sizeof base);
+    for (int k = 0; k < 3; ++k) {
+        base.las_scale[k] = m.frame.scale[k];
+        base.las_offset[k] = m.frame.offset[k];
+    }
+    base.cell_size = s.cell_size;
+    base.scale = s.scale;
+    base.nrows = L.wp_nr;
+    base.ncols = L.wp_nc;
+    base.site_radius = s.site_radius;
+    base.query_radius = (float)s.query_radius;
+    base.algorithm = s.algorithm;
+    base.accum_mode = o.accum == "fp32" ? TNNI_ACC_FP32_2SUM : o.accum == "s0" ? TNNI_ACC_S0 : TNNI_ACC_FIXED64;
+    base.pixel_offset = -1;
+    base.point_min = s.pointMin;
+    base.multiplier = s.multiplier;
+
+    double nni_s = 0, out_s = 0;
+    const size_t cells = (size_t)L.wp_nr * L.wp_nc;
+    for (float t : slices) {
+        RasterOut w(slice_name(o.output, t), o.format, L.nrrows, L.nrcols, L.minx, L.miny, L.cs);
+        const float tr = (float)o.time_radius;
+        // submit every package of one row of packages
+        auto submit_row = [&](int row) {
+            std::vector<std::unique_ptr<Flight>> fl;
+            for (int col = 0; col < L.cols; ++col) {
+                std::unique_ptr<Flight> f(new Flight());
+                tpie::ami::stream<RawPoint> st(wp_file(o.wp_path, row, col, ".raw"), tpie::ami::READ_STREAM);
+                if (!st.is_valid()) die("missing work package " + wp_file(o.wp_path, row, col, ".raw"));
+                RawPoint *p;
+                // the slice's points (nni.cpp:770 filter; the stream is sorted
+                // by time, so they are one run of it)
+                while (st.read_item(&p) == tpie::ami::NO_ERROR) {
+                    if (std::fabs(t - (float)p->t) <= tr + 0.000001f) {
+                        f->X.push_back(p->X);
+                        f->Y.push_back(p->Y);
+                        f->Z.push_back(p->Z);
+                    } else if ((float)p->t > t + tr) {
+                        break;
+                    }
+                }
+                f->h.resize(cells);
+                tnni_tile_desc d = base;
+                d.origin_x = L.origin_x(col);
+                d.origin_y = L.origin_y(row);
+                const tnni_status stt = tnni_submit(ctx, &d, f->X.data(), f->Y.data(), f->Z.data(),
+                                                    f->X.size(), f->h.data(), nullptr, nullptr, &f->ticket);
+                if (stt != TNNI_OK) die(std::string("submit: ") + tnni_last_error(ctx));
+                fl.push_back(std::move(f));
+            }
+            return fl;
+        };
+        std::vector<float> band;
+        const double ts = wall();
+        auto pending = submit_row(L.rows - 1);
+        for (int row = L.rows - 1; row >= 0; --row) {
+            std::vector<std::unique_ptr<Flight>> next;
+            if (row > 0) next = submit_row(row - 1);
+            for (auto &f : pending)
+                if (tnni_wait(ctx, f->ticket, nullptr) != TNNI_OK) die(std::string("tile: ") + tnni_last_error(ctx));
+            const double tw = wall();
+            // rows of this band inside the grid, top down
+            const int j_hi = std::min(L.nrrows, (row + 1) * L.wp_nr) - 1, j_lo = row * L.wp_nr;
+            const int nb = j_hi - j_lo + 1;
+            band.assign((size_t)std::max(nb, 0) * L.nrcols, -9999.0f);
+            for (int jj = 0; jj < nb; ++jj) {
+                const int j = j_hi - jj;       // global row
+                const int lj = j - j_lo;       // row inside the packages
+                float *dst = band.data() + (size_t)jj * L.nrcols;
+                for (int col = 0; col < L.cols; ++col) {
+                    const int i0 = col * L.wp_nc, n = std::min(L.wp_nc, L.nrcols - i0);
+                    if (n > 0) std::memcpy(dst + i0, pending[(size_t)col]->h.data() + (size_t)lj * L.wp_nc, (size_t)n * 4);
+                }
+            }
+            if (nb > 0) w.write_rows(band.data(), nb);
+            out_s += wall() - tw;
+            pending = std::move(next);
+        }
+        nni_s += wall() - ts;
+    }
+    tnni_ctx_destroy(ctx);
+    t_nni = nni_s - out_s;
+    t_out = out_s;
+}
+
+}  // namespace
+
+int run(int argc, char **argv)
+{
+    std::cout << "TerraNNI driver on the MI355X tile engine\n";
+    Options o = parse(argc, argv);
+    if (o.help || o.tilepath.empty()) {
+        if (!o.help) std::cout << "No tile path specified.\n";
+        std::cout << kUsage;
+        return EXIT_SUCCESS;
+    }
+    tpie::tpie_init();
+    tpie::MM_manager.set_memory_limit((size_t)2000 * 1024 * 1024);
+
+    o.time_length -= 1;  // one slice means no step (terranni.cpp:560)
+    if (o.two_d) {
+        o.time_radius = 0;
+        o.time_start = 0;
+        o.time_length = 0;
+        o.time_step = 1;
+    }
+    if (o.time_step == 0) {
+        std::cerr << "Cannot set timeStep = 0.  Resetting it to 1.\n";
+        o.time_step = 1;
+    }
+    if (o.time_radius > 0) die("space-time interpolation (--time-radius > 0) is not provided by this engine");
+    tpie::tempname::set_default_path(o.tmp_path);
+
+    // 1. the tiles and their bounding box
+    const double t_find0 = wall();
+    const std::vector<std::string> files = find_tiles(o.tilepath);
+    if (files.empty()) {
+        std::cout << o.tilepath << " contains no .las files, aborting.";
+        return EXIT_SUCCESS;
+    }
+    GridBox g;
+    g.minx = g.miny = g.minz = std::numeric_limits<double>::max();
+    g.maxx = g.maxy = g.maxz = std::numeric_limits<double>::lowest();
+    Frame frame{};
+    for (size_t k = 0; k < files.size(); ++k) {
+        nni_io::las_file lf;
+        lf.open(files[k]);
+        const nni_io::las_header &h = lf.header();
+        g.minx = std::min(g.minx, h.GetMinX());
+        g.miny = std::min(g.miny, h.GetMinY());
+        g.minz = std::min(g.minz, h.GetMinZ());
+        g.maxx = std::max(g.maxx, h.GetMaxX());
+        g.maxy = std::max(g.maxy, h.GetMaxY());
+        g.maxz = std::max(g.maxz, h.GetMaxZ());
+        if (k == 0)
+            for (int a = 0; a < 3; ++a) {
+                frame.scale[a] = h.scale[a];
+                frame.offset[a] = h.offset[a];
+            }
+    }
+    std::cout << std::setprecision(15) << "LAS BBOX: x:[" << g.minx << ", " << g.maxx << "]. y:[" << g.miny
+              << ", " << g.maxy << "]\n";
+    if (o.has_ox) g.minx = o.ox;
+    if (o.has_oy) g.miny = o.oy;
+
+    // 2. settings (terranni.cpp:585-690)
+    nni::Settings s;
+    std::memset(&s, 0, sizeof s);
+    s.scale = o.scale;
+    s.cell_size = o.cell_size;
+    s.site_radius = o.site_radius;
+    if (!(s.cell_size > 0) || s.scale < 1) die("--cell-size and --scaling must be positive");
+    int nrcols = (int)((g.maxx - g.minx) / s.cell_size);
+    int nrrows = (int)((g.maxy - g.miny) / s.cell_size);
+    if (o.has_gc) {
+        g.maxx = o.grid_cols * (double)s.cell_size + g.minx;
+        nrcols = o.grid_cols;
+    }
+    if (o.has_gr) {
+        g.maxy = o.grid_rows * (double)s.cell_size + g.miny;
+        nrrows = o.grid_rows;
+    }
+    if (nrcols < 1 || nrrows < 1) die("the grid has no cells");
+    s.pointMin = (int)std::floor(g.minz);
+    s.ncols = nrcols;
+    s.nrows = nrrows;
+
+    std::error_code ec;
+    if (!fs::is_directory(o.output, ec)) die("Output path is not a folder.");
+    {
+        std::ofstream info(o.output + "/tpie.info");
+        if (!info) die("Output path is not a folder.");
+        info << std::setprecision(15) << "origin-x=" << g.minx << "\n"
+             << "origin-y=" << g.miny + (double)s.cell_size * nrrows << "\n"
+             << "nrrows=" << s.nrows << "\n"
+             << "nrcols=" << s.ncols << "\n"
+             << "grid-length=" << s.cell_size << "\n";
+    }
+    const int blockWidth = 5;
+    const float border = blockWidth * s.cell_size + s.site_radius;
+    double qr = blockWidth * s.cell_size / 2.0 - 0.01;
+    if (o.has_qr) qr = o.qr;
+    s.query_radius = qr;
+    s.algorithm = o.draw_cones ? 1 : 0;
+    const int pw = (int)std::ceil(2.0 * s.site_radius * s.scale / s.cell_size);
+    const double max_sum = (double)pw * pw * (g.maxz - s.pointMin);
+    s.multiplier = 1;
+    if (max_sum < std::numeric_limits<int>::max()) {
+        s.multiplier = (int)std::floor(1.0 * std::numeric_limits<int>::max() / max_sum);
+        s.multiplier = (int)std::pow(10, std::floor(std::log10(s.multiplier)));
+    }
+
+    // work package size (terranni.cpp:697-708), in whole cells
+    const float max_meters =
+        std::floor((o.max_size - (s.scale / s.cell_size) * 2 * border - 100) / s.scale) * s.cell_size;
+    int wp_width = o.wp_width, wp_height = o.wp_height;
+    if (!o.has_wpw || wp_width > max_meters) wp_width = (int)max_meters;
+    if (!o.has_wph || wp_height > max_meters) wp_height = (int)max_meters;
+    Layout L;
+    L.minx = g.minx;
+    L.miny = g.miny;
+    L.cs = s.cell_size;
+    L.nrcols = nrcols;
+    L.nrrows = nrrows;
+    L.wp_nc = std::max(1, (int)(wp_width / s.cell_size));
+    L.wp_nr = std::max(1, (int)(wp_height / s.cell_size));
+    L.cols = (nrcols + L.wp_nc - 1) / L.wp_nc;
+    L.rows = (nrrows + L.wp_nr - 1) / L.wp_nr;
+    L.border = border;
+    std::fprintf(stderr, "Grid %d x %d cells; work packages %d x %d cells, %d x %d of them; border %g m\n",
+                 nrcols, nrrows, L.wp_nc, L.wp_nr, L.cols, L.rows, (double)border);
+    const double t_find = wall() - t_find0;
+
+    // 3. binning
+    double t_bin = 0;
+    const bool batched = o.engine == "batched";
+    Meta m;
+    m.frame = frame;
+    m.minx = L.minx;
+    m.miny = L.miny;
+    m.cs = L.cs;
+    m.border = L.border;
+    m.wp_nc = L.wp_nc;
+    m.wp_nr = L.wp_nr;
+    m.cols = L.cols;
+    m.rows = L.rows;
+    m.two_d = o.two_d ? 1 : 0;
+    if (!o.wp_computed) {
+        const double tb = wall();
+        const bool twoD = o.two_d;
+        if (batched) {
+            bin_points<RawPoint>(files, L, o.wp_path, ".raw",
+                                 [&frame, twoD](const nni_io::las_header &h, int32_t X, int32_t Y, int32_t Z,
+                                                uint16_t I, double x, double y) {
+                                     RawPoint r;
+                                     r.t = twoD ? 0 : (int)I;
+                                     const bool same = std::memcmp(h.scale, frame.scale, sizeof frame.scale) == 0 &&
+                                                       std::memcmp(h.offset, frame.offset, sizeof frame.offset) == 0;
+                                     if (same) {
+                                         r.X = X;
+                                         r.Y = Y;
+                                         r.Z = Z;
+                                     } else {
+                                         const double z = (double)Z * h.scale[2] + h.offset[2];
+                                         auto q = [](double v, double sc, double of) {
+                                             const double u = std::nearbyint((v - of) / sc);
+                                             if (!(std::fabs(u) < 2147483647.0)) die("a point does not fit the common LAS frame");
+                                             return (int32_t)u;
+                                         };
+                                         r.X = q(x, frame.scale[0], frame.offset[0]);
+                                         r.Y = q(y, frame.scale[1], frame.offset[1]);
+                                         r.Z = q(z, frame.scale[2], frame.offset[2]);
+                                     }
+                                     return r;
+                                 });
+            write_meta(o.wp_path, m);
+        } else {
+            bin_points<nni::Point4D>(files, L, o.wp_path, ".tpie",
+                                     [twoD](const nni_io::las_header &h, int32_t, int32_t, int32_t Z, uint16_t I,
+                                            double x, double y) {
+                                         return nni::Point4D(x, y, (double)Z * h.scale[2] + h.offset[2],
+                                                             twoD ? 0 : (int)I);
+                                     });
+        }
+        t_bin = wall() - tb;
+    } else if (batched) {
+        const Meta w = read_meta(o.wp_path);
+        if (w.wp_nc != m.wp_nc || w.wp_nr != m.wp_nr || w.cols != m.cols || w.rows != m.rows ||
+            w.minx != m.minx || w.miny != m.miny || w.cs != m.cs || w.two_d != m.two_d)
+            die("--wp-computed: the work packages in " + o.wp_path + " were binned for another grid");
+        m.frame = w.frame;
+    }
+    if (o.bin_only) {
+        std::cerr << "Total Find Tile time: " << t_find << "\nTotal Tile time: " << t_bin << "\n";
+        return EXIT_SUCCESS;
+    }
+
+    // 4. interpolation and output
+    const std::vector<float> slices = time_slices(o);
+    double t_nni = 0, t_out = 0;
+    if (batched) {
+        run_batched(o, L, s, m, slices, t_nni, t_out);
+    } else {
+        nni::init(s);
+        if (nni::lastStatus() != 0) die(std::string("nni::init: ") + nni::lastError());
+        if (o.accum == "fp32") nni::setAccumulationMode(TNNI_ACC_FP32_2SUM);
+        else if (o.accum == "s0") nni::setAccumulationMode(TNNI_ACC_S0);
+        else nni::setAccumulationMode(TNNI_ACC_FIXED64);
+        run_plain(o, L, s, slices, t_nni, t_out);
+        nni::cleanup();
+    }
+    std::cerr << "Total Find Tile time: " << t_find << "\n"
+              << "Total Tile time: " << t_bin << "\n"
+              << "Total NNI time: " << t_nni << "\n"
+              << "Total Print Output time: " << t_out << "\n";
+    return EXIT_SUCCESS;
+}
+
+int main(int argc, char **argv)
+{
+    try {
+        return run(argc, argv);
+    } catch (const std::exception &e) {
+        die(e.what());
+    }
+}

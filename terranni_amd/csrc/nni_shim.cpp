@@ -25,6 +25,8 @@ int g_mode = TNNI_ACC_FIXED64;
 int g_status = TNNI_OK;
 std::string g_err;
 tpie::cpu_timer g_t_read, g_t_tile, g_t_write;
+bool g_have_origin64 = false;
+double g_origin64_x = 0.0, g_origin64_y = 0.0;
 
 void set_error(int st, const std::string &msg)
 {
@@ -87,6 +89,13 @@ void printTimes()
 }
 
 void setAccumulationMode(int mode) { g_mode = mode; }
+
+void setOrigin64(double origin_x, double origin_y)
+{
+    g_have_origin64 = true;
+    g_origin64_x = origin_x;
+    g_origin64_y = origin_y;
+}
 int lastStatus() { return g_status; }
 const char *lastError() { return g_err.c_str(); }
 
@@ -96,6 +105,8 @@ void performNNI(const WorkPackage &wp, tpie::ami::stream<OutPoint> &out,
 {
     (void)debug;
     g_status = TNNI_OK;
+    const bool have64 = g_have_origin64;
+    g_have_origin64 = false;
     if (!g_ctx) {
         set_error(TNNI_E_NODEVICE, "performNNI before a successful nni::init");
         return;
@@ -126,8 +137,8 @@ void performNNI(const WorkPackage &wp, tpie::ami::stream<OutPoint> &out,
     //    relative to the tile origin, heights in millimetres
     tnni_tile_desc d;
     std::memset(&d, 0, sizeof d);
-    d.origin_x = (double)wp.origin_x;
-    d.origin_y = (double)wp.origin_y;
+    d.origin_x = have64 ? g_origin64_x : (double)wp.origin_x;
+    d.origin_y = have64 ? g_origin64_y : (double)wp.origin_y;
     d.cell_size = g_settings.cell_size;
     d.scale = g_settings.scale;
     d.nrows = wp.nrows;

@@ -219,3 +219,80 @@ def workload_batch(seed: int = 3, n_tiles: int = 256, scale_points: float = 1.0,
         tiles.append(make_tile(m["seed"], n, m["grid"], kind=m["kind"],
                                origin=(float(k % 16) * 1100.0, float(k // 16) * 1100.0)))
     return tiles
+
+
+# ---- LAS files ---------------------------------------------------------------
+
+# Point record lengths of the LAS point formats this writer produces; every
+# format starts with int32 X, Y, Z and uint16 intensity.
+LAS_RECORD_LENGTH = {0: 20, 1: 28, 2: 26, 3: 34, 6: 30, 7: 36, 8: 38}
+
+
+def write_las(path, X, Y, Z, intensity=None, scale=LAS_SCALE, offset=(0.0, 0.0, 0.0),
+              version=(1, 2), point_format=0):
+    """Write a LAS 1.2 or 1.4 file (no VLRs) of raw int32 coordinates; the
+    header's bounding box is that of the de-scaled points."""
+    X = np.ascontiguousarray(X, np.int32)
+    Y = np.ascontiguousarray(Y, np.int32)
+    Z = np.ascontiguousarray(Z, np.int32)
+    n = X.shape[0]
+    I = np.zeros(n, np.uint16) if intensity is None else np.ascontiguousarray(intensity, np.uint16)
+    rl = LAS_RECORD_LENGTH[point_format]
+    hsize = 375 if version >= (1, 4) else 227
+    h = bytearray(hsize)
+    h[0:4] = b"LASF"
+    h[24], h[25] = version
+    h[26:58] = b"terranni_amd synth".ljust(32, b"\0")
+    h[58:90] = b"terranni_amd".ljust(32, b"\0")
+    np.frombuffer(memoryview(h)[94:96], np.uint16)[:] = hsize
+    np.frombuffer(memoryview(h)[96:100], np.uint32)[:] = hsize
+    h[104] = point_format
+    np.frombuffer(memoryview(h)[105:107], np.uint16)[:] = rl
+    legacy = n if (point_format <= 5 and n < 2**32) else 0
+    np.frombuffer(memoryview(h)[107:111], np.uint32)[:] = legacy
+    np.frombuffer(memoryview(h)[131:155], np.float64)[:] = scale
+    np.frombuffer(memoryview(h)[155:179], np.float64)[:] = offset
+    if n:
+        xs = X.astype(np.float64) * scale[0] + offset[0]
+        ys = Y.astype(np.float64) * scale[1] + offset[1]
+        zs = Z.astype(np.float64) * scale[2] + offset[2]
+        box = [xs.max(), xs.min(), ys.max(), ys.min(), zs.max(), zs.min()]
+    else:
+        box = [0.0] * 6
+    np.frombuffer(memoryview(h)[179:227], np.float64)[:] = box
+    if hsize >= 375:
+        np.frombuffer(memoryview(h)[247:255], np.uint64)[:] = n
+    rec = np.zeros((n, rl), np.uint8)
+    rec[:, 0:4] = X.view(np.uint8).reshape(n, 4)
+    rec[:, 4:8] = Y.view(np.uint8).reshape(n, 4)
+    rec[:, 8:12] = Z.view(np.uint8).reshape(n, 4)
+    rec[:, 12:14] = I.view(np.uint8).reshape(n, 2)
+    with open(path, "wb") as f:
+        f.write(bytes(h))
+        f.write(rec.tobytes())
+
+
+def las_scene(seed: int, n: int, width: float, height: float, origin=(500000.0, 4200000.0),
+              n_files: int = 3, kind: str = "uniform"):
+    """A small LiDAR survey split into n_files strips along x: LAS int32 at
+    0.01 m with offset `origin`; returns [(X, Y, Z, intensity)] per strip in
+    file order (the points of strip k have x in its x range)."""
+    rng = np.random.default_rng(seed)
+    field = HeightField(rng)
+    if kind == "uniform":
+        x = rng.uniform(0, width, n)
+        y = rng.uniform(0, height, n)
+    else:
+        x, y = clustered_points(rng, n, max(width, height), max(4, n // 2000), border=0.0)
+        keep = (x < width) & (y < height)
+        x, y = x[keep], y[keep]
+    z = field(x, y) + rng.normal(0.0, 0.1, x.shape[0])
+    X = np.rint(x / LAS_SCALE[0]).astype(np.int32)
+    Y = np.rint(y / LAS_SCALE[1]).astype(np.int32)
+    Z = np.rint(z / LAS_SCALE[2]).astype(np.int32)
+    edges = np.linspace(0, width, n_files + 1)
+    out = []
+    for k in range(n_files):
+        m = (x >= edges[k]) & (x < edges[k + 1])
+        out.append((X[m], Y[m], Z[m], np.zeros(int(m.sum()), np.uint16)))
+    return out

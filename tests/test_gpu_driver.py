@@ -1,0 +1,107 @@
+"""The streaming driver on the GPU: LAS tiles in, ESRI grids out.
+
+Work packages are exact pieces of the whole grid (every point within
+`border` of a package is in it, and package origins sit on cell edges), so
+the driver's grid must equal the oracle's single tile over the whole grid:
+bit for bit from the batched engine (LAS integers de-scaled on the GPU) and
+in the ESRI ASCII text; within 1e-6 from the plain nni.h engine (which takes
+heights through nni::Point4D metres, quantised to millimetres)."""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import terranni_amd as T
+import terranni_amd.synth as S
+from helpers import REL_FLOOR
+from test_driver import DRIVER, GRID, OFFSET, run_driver, write_scene
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")]
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def oracle_grid(strips, t_keep=None):
+    """The oracle over the whole grid as one tile; t_keep: only the points
+    of that time."""
+    X = np.concatenate([s[0] for s in strips])
+    Y = np.concatenate([s[1] for s in strips])
+    Z = np.concatenate([s[2] for s in strips])
+    if t_keep is not None:
+        m = np.concatenate([s[3] for s in strips]) == t_keep
+        X, Y, Z = X[m], Y[m], Z[m]
+    g = GRID
+    p = T.TileParams(nrows=g["rows"], ncols=g["cols"], cell_size=g["cs"], scale=g["scale"],
+                     site_radius=g["sr"], origin_x=OFFSET[0], origin_y=OFFSET[1],
+                     las_scale=S.LAS_SCALE, las_offset=OFFSET)
+    return O.run(p, X, Y, Z)
+
+
+def read_flt(out, name):
+    hdr = dict(line.split() for line in open(os.path.join(out, name + ".hdr")))
+    nc, nr = int(hdr["ncols"]), int(hdr["nrows"])
+    return np.fromfile(os.path.join(out, name + ".flt"), np.float32).reshape(nr, nc)[::-1]
+
+
+def read_asc_lines(out, name):
+    with open(os.path.join(out, name + ".tpie.asc")) as f:
+        return f.read().split("\n")
+
+
+def asc_text(h):
+    """The grid as the reference writes it: rows top down, `%g ` per value."""
+    return ["".join("%g " % float(v) for v in row) for row in h[::-1]]
+
+
+def test_driver_batched_equals_whole_grid_oracle(tmp_path):
+    (tmp_path / "las").mkdir()
+    strips = write_scene(str(tmp_path / "las"))
+    _say("driver batched --2d")
+    _, out = run_driver(tmp_path, "batched", ["--2d", "--format=both"])
+    ref = oracle_grid(strips)
+    h = read_flt(out, "0.00000")
+    assert h.shape == (GRID["rows"], GRID["cols"])
+    assert np.array_equal(h.view(np.uint32), ref["h_fixed"].view(np.uint32))
+    lines = read_asc_lines(out, "0.00000")
+    assert lines[0] == f"NROWS {GRID['rows']}" and lines[1] == f"NCOLS {GRID['cols']}"
+    assert lines[5] == "NODATA_VALUE -9999"
+    assert lines[6:6 + GRID["rows"]] == asc_text(ref["h_fixed"])
+    info = (out / "tpie.info").read_text().split()
+    assert f"nrrows={GRID['rows']}" in info and f"nrcols={GRID['cols']}" in info
+
+
+def test_driver_plain_matches_batched_and_oracle(tmp_path):
+    (tmp_path / "las").mkdir()
+    strips = write_scene(str(tmp_path / "las"))
+    _say("driver plain --2d")
+    _, out_p = run_driver(tmp_path, "plain", ["--2d", "--format=both"])
+    ref = oracle_grid(strips)
+    h = read_flt(out_p, "0.00000").astype(np.float64)
+    h64 = ref["h64"]
+    valid = h64 != T.NODATA
+    assert np.array_equal(valid, h != T.NODATA)
+    rel = np.abs(h[valid] - h64[valid]) / np.maximum(np.abs(h64[valid]), REL_FLOOR)
+    assert rel.max() <= 1e-6, rel.max()
+    vals = np.array([[float(v) for v in line.split()] for line in read_asc_lines(out_p, "0.00000")[6:6 + GRID["rows"]]])
+    assert np.allclose(vals[::-1], ref["h_fixed"], rtol=1e-5, atol=0)
+
+
+def test_driver_time_slice_and_reuse_of_packages(tmp_path):
+    """Without --2d a slice keeps the points whose time (LAS intensity)
+    equals it; --wp-computed reruns from the binned packages."""
+    (tmp_path / "las").mkdir()
+    strips = write_scene(str(tmp_path / "las"))
+    _say("driver batched slice t=1")
+    _, out = run_driver(tmp_path, "batched", ["--time-start=1", "--format=flt"])
+    ref = oracle_grid(strips, t_keep=1)
+    h = read_flt(out, "1.00000")
+    assert np.array_equal(h.view(np.uint32), ref["h_fixed"].view(np.uint32))
+    os.remove(out / "1.00000.flt")
+    _say("driver batched slice t=1, --wp-computed")
+    _, out2 = run_driver(tmp_path, "batched", ["--time-start=1", "--format=flt", "--wp-computed"])
+    h2 = read_flt(out2, "1.00000")
+    assert np.array_equal(h2.view(np.uint32), ref["h_fixed"].view(np.uint32))
