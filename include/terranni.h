@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define TNNI_ABI_VERSION 2
+#define TNNI_ABI_VERSION 3
 
 typedef enum tnni_status {
     TNNI_OK = 0,
@@ -202,6 +202,24 @@ typedef struct tnni_serve_stats {
     uint64_t inflight_bytes;       /* footprint admitted right now       */
 } tnni_serve_stats;
 tnni_status tnni_serve_stats_read(tnni_ctx *ctx, tnni_serve_stats *out);
+
+/* ---- space-time (timeRadius > 0) ----------------------------------------
+ * One work package at time cur_time with time radius r, as performNNI with
+ * timeRadius > 0 (nni.cpp:870-944, replaces its slice loop :910-933 and the
+ * Delta render-buffer cache :321-379).  T[i] is point i's time (the LAS
+ * intensity, las_reader.inl:59).  Slices t = cur_time + j (float), j =
+ * -(int)r .. r: slice t's sites are the points with |t - T| <= r + 1e-6
+ * (nni.cpp:770), each at squared distance |x - s|^2 + (|t - T| in metres,
+ * rounded to the unit)^2, regions in the plane; the queries of slice j add
+ * j^2 (nni.cpp:810); every cell sums over all slices and divides once
+ * (nni.cpp:906,936).  Label outputs are per slice and not returned.  r = 0
+ * is the 2-D tile of the points with T == cur_time. */
+tnni_status tnni_tile_run_time(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *X,
+                               const int32_t *Y, const int32_t *Z, const int32_t *T, uint64_t n,
+                               float cur_time, float time_radius, float *out_h);
+tnni_status tnni_tile_run_time_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *dX,
+                                   const int32_t *dY, const int32_t *dZ, const int32_t *dT,
+                                   uint64_t n, float cur_time, float time_radius, float *d_out_h);
 
 /* ---- autotuning --------------------------------------------------------
  * New (the reference has fixed GL raster sizes, nni.cpp:601-668): the bucket

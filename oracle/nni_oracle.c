@@ -133,7 +133,7 @@ static int covers(const orc_frame *f, int alg, int64_t dx, int64_t dy, int site)
 
 /* Site-major pass.  Writes only rows [l_lo, l_hi). */
 static void raster_band(const orc_frame *f, int alg, const int32_t *xu, const int32_t *yu,
-                        const uint8_t *keep, const int64_t *ids, int64_t nid,
+                        const uint8_t *keep, const uint32_t *dt2, const int64_t *ids, int64_t nid,
                         int l_lo, int l_hi, uint32_t *d2, int32_t *label)
 {
     const int64_t ps = f->ps_u;
@@ -150,7 +150,7 @@ static void raster_band(const orc_frame *f, int alg, const int32_t *xu, const in
             for (int64_t k = k0; k <= k1; ++k) {
                 int64_t dx = f->x0 + k * ps - sx;
                 if (!covers(f, alg, dx, dy, 1)) continue;
-                uint32_t dd = (uint32_t)(dx * dx + dy * dy);
+                uint32_t dd = (uint32_t)(dx * dx + dy * dy + (dt2 ? (int64_t)dt2[i] : 0));
                 size_t p = (size_t)l * f->W + k;
                 /* depth test: nearer wins; equal depth keeps the lower index */
                 if (dd < d2[p] || (dd == d2[p] && (int32_t)i < label[p])) {
@@ -165,6 +165,13 @@ static void raster_band(const orc_frame *f, int alg, const int32_t *xu, const in
 void orc_labels_raster(const orc_frame *f, const int32_t *xu, const int32_t *yu,
                        const uint8_t *keep, int64_t n,
                        uint32_t *d2, int32_t *label, int nthreads)
+{
+    orc_labels_raster_t(f, xu, yu, keep, NULL, n, d2, label, nthreads);
+}
+
+void orc_labels_raster_t(const orc_frame *f, const int32_t *xu, const int32_t *yu,
+                         const uint8_t *keep, const uint32_t *dt2, int64_t n,
+                         uint32_t *d2, int32_t *label, int nthreads)
 {
     const int alg = f->algorithm;
     const size_t P = (size_t)f->W * f->H;
@@ -203,7 +210,7 @@ void orc_labels_raster(const orc_frame *f, const int32_t *xu, const int32_t *yu,
         int l_lo = b * band_h;
         int l_hi = l_lo + band_h < f->H ? l_lo + band_h : f->H;
         int bfirst = b - reach < 0 ? 0 : b - reach;
-        raster_band(f, alg, xu, yu, keep, ids + cnt[bfirst], cnt[b + 1] - cnt[bfirst],
+        raster_band(f, alg, xu, yu, keep, dt2, ids + cnt[bfirst], cnt[b + 1] - cnt[bfirst],
                     l_lo, l_hi, d2, label);
     }
     free(cur);
@@ -215,6 +222,13 @@ void orc_labels_brute(const orc_frame *f, const int32_t *xu, const int32_t *yu,
                       const uint8_t *keep, int64_t n,
                       uint32_t *d2, int32_t *label)
 {
+    orc_labels_brute_t(f, xu, yu, keep, NULL, n, d2, label);
+}
+
+void orc_labels_brute_t(const orc_frame *f, const int32_t *xu, const int32_t *yu,
+                        const uint8_t *keep, const uint32_t *dt2, int64_t n,
+                        uint32_t *d2, int32_t *label)
+{
     const int alg = f->algorithm;
     for (int64_t l = 0; l < f->H; ++l) {
         for (int64_t k = 0; k < f->W; ++k) {
@@ -225,7 +239,7 @@ void orc_labels_brute(const orc_frame *f, const int32_t *xu, const int32_t *yu,
                 if (!keep[i]) continue;
                 int64_t dx = px - xu[i], dy = py - yu[i];
                 if (!covers(f, alg, dx, dy, 1)) continue;
-                uint32_t dd = (uint32_t)(dx * dx + dy * dy);
+                uint32_t dd = (uint32_t)(dx * dx + dy * dy + (dt2 ? (int64_t)dt2[i] : 0));
                 if (dd < best) { best = dd; bi = (int32_t)i; }
             }
             d2[l * f->W + k] = best;
@@ -244,10 +258,24 @@ uint32_t orc_s0_value(const orc_desc *d, int32_t Z)
     return (uint32_t)(int64_t)v;
 }
 
-void orc_heights(const orc_desc *d, const orc_frame *f,
-                 const uint32_t *d2, const int32_t *label, const int32_t *Z,
-                 float *h_fixed, double *h64, float *h_s0, uint32_t *count,
-                 uint64_t *pairs, int nthreads)
+void orc_slice_sites(const orc_frame *f, const int32_t *T, const uint8_t *keep, int64_t n,
+                     float t, float time_radius, uint8_t *keep_slice, uint32_t *dt2)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        /* abs(t - p.t) <= timeRadius + 0.000001 (nni.cpp:770): float
+         * difference, compared in double */
+        float dt = fabsf(t - (float)T[i]);
+        int in = keep[i] && (double)dt <= (double)time_radius + 0.000001;
+        keep_slice[i] = (uint8_t)in;
+        /* addPoint(wp, p, abs(t - p.t)) -> timeOffset^2 (nni.cpp:197,779) */
+        uint32_t u = in ? (uint32_t)rint((double)dt * (double)f->upm) : 0u;
+        dt2[i] = u * u;
+    }
+}
+
+void orc_heights_accumulate(const orc_desc *d, const orc_frame *f,
+                            const uint32_t *d2, const int32_t *label, const int32_t *Z,
+                            int64_t jj2, orc_acc *acc, uint64_t *pairs, int nthreads)
 {
     const int alg = f->algorithm;
     const int64_t ps = f->ps_u;
@@ -266,9 +294,7 @@ void orc_heights(const orc_desc *d, const orc_frame *f,
             int64_t k1 = imin64(f->W - 1, floordiv64(qx + f->Rqb - f->x0, ps));
             int64_t l0 = imax64(0, ceildiv64(qy - f->Rqb - f->y0, ps));
             int64_t l1 = imin64(f->H - 1, floordiv64(qy + f->Rqb - f->y0, ps));
-            int64_t sumZ = 0;
-            double sum64 = 0.0;
-            uint32_t cnt = 0, numer0 = 0, denom0 = 0;
+            orc_acc a = acc[(size_t)j * d->ncols + i];
             for (int64_t l = l0; l <= l1; ++l) {
                 int64_t dy = f->y0 + l * ps - qy;
                 for (int64_t k = k0; k <= k1; ++k) {
@@ -278,35 +304,58 @@ void orc_heights(const orc_desc *d, const orc_frame *f,
                     int64_t dx = f->x0 + k * ps - qx;
                     if (!covers(f, alg, dx, dy, 0)) continue;
                     K++;
-                    /* query depth <= site depth (GL_LEQUAL, nni.cpp:804) */
-                    if ((uint64_t)(dx * dx + dy * dy) > (uint64_t)d2[p]) continue;
+                    /* query depth <= site depth (GL_LEQUAL, nni.cpp:804); a
+                     * space-time query at time offset j adds j^2 (nni.cpp:810,
+                     * 197) */
+                    if ((uint64_t)(dx * dx + dy * dy + jj2) > (uint64_t)d2[p]) continue;
                     S++;
-                    cnt++;
-                    sumZ += Z[s];
-                    sum64 += fma((double)Z[s], sz, oz);
-                    if (h_s0) {
-                        uint32_t v = orc_s0_value(d, Z[s]);
-                        if (v > 0) { numer0 += v; denom0++; } /* cuda_nni.cu:423,447-448 */
-                    }
+                    a.cnt++;
+                    a.sumZ += Z[s];
+                    a.sum64 += fma((double)Z[s], sz, oz);
+                    uint32_t v = orc_s0_value(d, Z[s]);
+                    if (v > 0) { a.numer0 += v; a.denom0++; } /* cuda_nni.cu:423,447-448 */
                 }
             }
-            size_t o = (size_t)j * d->ncols + i;
-            if (count) count[o] = cnt;
-            if (h_fixed)
-                h_fixed[o] = cnt ? (float)fma((double)sumZ / (double)cnt, sz, oz) : -9999.0f;
-            if (h64) h64[o] = cnt ? sum64 / (double)cnt : -9999.0;
-            if (h_s0) {
-                /* divide, cuda_nni.cu:497-500, then de-scale nni.cpp:841-846 */
-                int32_t res;
-                if (denom0 == 0)
-                    res = (int32_t)((uint32_t)(-9999) * (uint32_t)d->multiplier);
-                else
-                    res = (int32_t)numer0 / (int32_t)denom0;
-                double z = 1.0 * res / d->multiplier;
-                if (z != -9999) z = z - (-d->point_min) - 1;
-                h_s0[o] = (float)z;
-            }
+            acc[(size_t)j * d->ncols + i] = a;
         }
     }
-    if (pairs) { pairs[0] = K; pairs[1] = S; }
+    if (pairs) { pairs[0] += K; pairs[1] += S; }
+}
+
+void orc_heights_finish(const orc_desc *d, const orc_acc *acc, float *h_fixed, double *h64,
+                        float *h_s0, uint32_t *count)
+{
+    const double sz = d->las_scale[2], oz = d->las_offset[2];
+    const size_t C = (size_t)d->nrows * d->ncols;
+    for (size_t o = 0; o < C; ++o) {
+        const orc_acc a = acc[o];
+        if (count) count[o] = a.cnt;
+        if (h_fixed)
+            h_fixed[o] = a.cnt ? (float)fma((double)a.sumZ / (double)a.cnt, sz, oz) : -9999.0f;
+        if (h64) h64[o] = a.cnt ? a.sum64 / (double)a.cnt : -9999.0;
+        if (h_s0) {
+            /* divide, cuda_nni.cu:497-500, then de-scale nni.cpp:841-846 */
+            int32_t res;
+            if (a.denom0 == 0)
+                res = (int32_t)((uint32_t)(-9999) * (uint32_t)d->multiplier);
+            else
+                res = (int32_t)a.numer0 / (int32_t)a.denom0;
+            double z = 1.0 * res / d->multiplier;
+            if (z != -9999) z = z - (-d->point_min) - 1;
+            h_s0[o] = (float)z;
+        }
+    }
+}
+
+void orc_heights(const orc_desc *d, const orc_frame *f,
+                 const uint32_t *d2, const int32_t *label, const int32_t *Z,
+                 float *h_fixed, double *h64, float *h_s0, uint32_t *count,
+                 uint64_t *pairs, int nthreads)
+{
+    const size_t C = (size_t)d->nrows * d->ncols;
+    orc_acc *acc = (orc_acc *)calloc(C ? C : 1, sizeof(orc_acc));
+    if (pairs) { pairs[0] = 0; pairs[1] = 0; }
+    orc_heights_accumulate(d, f, d2, label, Z, 0, acc, pairs, nthreads);
+    orc_heights_finish(d, acc, h_fixed, h64, h_s0, count);
+    free(acc);
 }

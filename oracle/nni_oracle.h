@@ -106,6 +106,49 @@ void orc_heights(const orc_desc *d, const orc_frame *f,
 /* S0 site value floor((z + zOffset + 1) * zMult), nni.cpp:539. */
 uint32_t orc_s0_value(const orc_desc *d, int32_t Z);
 
+/* ---- space-time (timeRadius > 0), nni.cpp:902-936 ----------------------
+ * performNNI runs the slices t = curTime + j, j = -(int)timeRadius ..
+ * timeRadius.  Slice t's sites are the points with |t - T| <= timeRadius +
+ * 1e-6 (nni.cpp:770), each at squared time distance dt2 = rint(|t - T| *
+ * upm)^2 (units^2: one time unit is one metre, drawPlane's timeOffset term,
+ * nni.cpp:197) added to its squared distance; the region tests stay in the
+ * plane.  The queries of slice j add (j * upm)^2 (performNNIQueries with
+ * timeOffset j, nni.cpp:931,810).  Every cell sums over all slices, then
+ * divides once (cudaSetup zeroes once, nni.cpp:906; completeBufferAnalysis
+ * after the loop, :936). */
+
+/* The slice's sites (keep_slice) and their dt2 (0 where not a site). */
+void orc_slice_sites(const orc_frame *f, const int32_t *T, const uint8_t *keep, int64_t n,
+                     float t, float time_radius, uint8_t *keep_slice, uint32_t *dt2);
+
+/* The site passes with per-site squared time distances (dt2 NULL: none). */
+void orc_labels_raster_t(const orc_frame *f, const int32_t *xu, const int32_t *yu,
+                         const uint8_t *keep, const uint32_t *dt2, int64_t n,
+                         uint32_t *d2, int32_t *label, int nthreads);
+void orc_labels_brute_t(const orc_frame *f, const int32_t *xu, const int32_t *yu,
+                        const uint8_t *keep, const uint32_t *dt2, int64_t n,
+                        uint32_t *d2, int32_t *label);
+
+/* A cell's running sums over the slices. */
+typedef struct {
+    int64_t sumZ;     /* sum of the stolen sites' LAS Z integers          */
+    double sum64;     /* sum of their FP64 heights                         */
+    uint32_t numer0;  /* S0: sum of the zMultiplier values (u32 wrap)      */
+    uint32_t denom0;  /* S0: their count (colour 0 skipped)                */
+    uint32_t cnt;     /* stolen pixels                                     */
+    uint32_t pad;
+} orc_acc;
+
+/* One slice's query pass into acc (cells row-major), the queries at time
+ * offset jj2 = (j * upm)^2; pairs (nullable) accumulates K and S. */
+void orc_heights_accumulate(const orc_desc *d, const orc_frame *f,
+                            const uint32_t *d2, const int32_t *label, const int32_t *Z,
+                            int64_t jj2, orc_acc *acc, uint64_t *pairs, int nthreads);
+
+/* The divide of every cell (outputs as orc_heights, any may be NULL). */
+void orc_heights_finish(const orc_desc *d, const orc_acc *acc, float *h_fixed, double *h64,
+                        float *h_s0, uint32_t *count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,19 @@ def lib():
     L.orc_heights.restype = None
     L.orc_s0_value.argtypes = [ctypes.POINTER(OrcDesc), ctypes.c_int32]
     L.orc_s0_value.restype = ctypes.c_uint32
+    L.orc_slice_sites.argtypes = [ctypes.POINTER(OrcFrame), P, P, ctypes.c_int64, ctypes.c_float,
+                                  ctypes.c_float, P, P]
+    L.orc_slice_sites.restype = None
+    L.orc_labels_raster_t.argtypes = [ctypes.POINTER(OrcFrame), P, P, P, P, ctypes.c_int64, P, P,
+                                      ctypes.c_int]
+    L.orc_labels_raster_t.restype = None
+    L.orc_labels_brute_t.argtypes = [ctypes.POINTER(OrcFrame), P, P, P, P, ctypes.c_int64, P, P]
+    L.orc_labels_brute_t.restype = None
+    L.orc_heights_accumulate.argtypes = [ctypes.POINTER(OrcDesc), ctypes.POINTER(OrcFrame), P, P,
+                                         P, ctypes.c_int64, P, P, ctypes.c_int]
+    L.orc_heights_accumulate.restype = None
+    L.orc_heights_finish.argtypes = [ctypes.POINTER(OrcDesc), P, P, P, P, P]
+    L.orc_heights_finish.restype = None
     _lib = L
     return L
 
@@ -182,3 +195,65 @@ def run(params, X, Y, Z, nthreads: int = 0, s0: bool = False, brute: bool = Fals
 def s0_value(params, Z: int) -> int:
     d = desc_from(params)
     return int(lib().orc_s0_value(ctypes.byref(d), int(Z)))
+
+
+# one orc_acc per cell: sumZ int64, sum64 double, numer0, denom0, cnt, pad u32
+ACC = np.dtype([("sumZ", "<i8"), ("sum64", "<f8"), ("numer0", "<u4"), ("denom0", "<u4"),
+                ("cnt", "<u4"), ("pad", "<u4")])
+
+
+def time_slices(time_radius: float):
+    """j of performNNI's slice loop: -(int)timeRadius .. timeRadius
+    (nni.cpp:910)."""
+    j = -int(time_radius)
+    out = []
+    while j <= time_radius:
+        out.append(j)
+        j += 1
+    return out
+
+
+def run_time(params, X, Y, Z, T, cur_time: float, time_radius: float, nthreads: int = 0,
+             s0: bool = False, brute: bool = False, keep_slices: bool = False):
+    """Space-time tile (nni.cpp:902-936): slices t = cur_time + j (float),
+    sites of each filtered by time and at their squared time distance, the
+    queries of slice j at (j * upm)^2, sums over all slices, one divide.
+    Returns the dict of heights(); keep_slices adds per-slice (labels, d2)."""
+    d, f, xu, yu, keep = _prep(params, X, Y)
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    T = np.ascontiguousarray(T, dtype=np.int32)
+    n = xu.shape[0]
+    acc = np.zeros(params.nrows * params.ncols, ACC)
+    pairs = np.zeros(2, np.uint64)
+    ks = np.empty(n, np.uint8)
+    dt2 = np.empty(n, np.uint32)
+    slices = []
+    for j in time_slices(time_radius):
+        t = np.float32(np.float32(cur_time) + np.float32(j))
+        lib().orc_slice_sites(ctypes.byref(f), _p(T), _p(keep), n, float(t), float(time_radius),
+                              _p(ks), _p(dt2))
+        d2 = np.empty((f.H, f.W), np.uint32)
+        lab = np.empty((f.H, f.W), np.int32)
+        if brute:
+            lib().orc_labels_brute_t(ctypes.byref(f), _p(xu), _p(yu), _p(ks), _p(dt2), n, _p(d2),
+                                     _p(lab))
+        else:
+            lib().orc_labels_raster_t(ctypes.byref(f), _p(xu), _p(yu), _p(ks), _p(dt2), n, _p(d2),
+                                      _p(lab), nthreads)
+        jj2 = (j * f.upm) ** 2
+        lib().orc_heights_accumulate(ctypes.byref(d), ctypes.byref(f), _p(d2), _p(lab), _p(Z), jj2,
+                                     _p(acc), _p(pairs), nthreads)
+        if keep_slices:
+            slices.append((lab, d2))
+    shp = (params.nrows, params.ncols)
+    hf = np.empty(shp, np.float32)
+    h64 = np.empty(shp, np.float64)
+    hs0 = np.empty(shp, np.float32) if s0 else None
+    cnt = np.empty(shp, np.uint32)
+    lib().orc_heights_finish(ctypes.byref(d), _p(acc), _p(hf), _p(h64), _p(hs0), _p(cnt))
+    out = {"h_fixed": hf, "h64": h64, "count": cnt, "K": int(pairs[0]), "S": int(pairs[1])}
+    if s0:
+        out["h_s0"] = hs0
+    if keep_slices:
+        out["slices"] = slices
+    return out

@@ -197,12 +197,17 @@ def lib():
                                  ctypes.POINTER(ctypes.c_int)]
     L.tnni_tune_load.argtypes = [P, ctypes.POINTER(TuneEntry), ctypes.c_int]
     L.tnni_tune_clear.argtypes = [P]
+    L.tnni_tile_run_time.argtypes = [P, ctypes.POINTER(TileDesc), P, P, P, P, ctypes.c_uint64,
+                                     ctypes.c_float, ctypes.c_float, P]
+    L.tnni_tile_run_time_dev.argtypes = [P, ctypes.POINTER(TileDesc), P, P, P, P, ctypes.c_uint64,
+                                         ctypes.c_float, ctypes.c_float, P]
     for name in ("tnni_frame", "tnni_footprint", "tnni_ctx_create", "tnni_tile_run",
                  "tnni_tile_run_dev", "tnni_batch_run_dev", "tnni_dev_alloc", "tnni_dev_free",
                  "tnni_h2d", "tnni_d2h", "tnni_d2d", "tnni_profile_enable", "tnni_profile_read",
                  "tnni_submit", "tnni_submit_dev", "tnni_poll", "tnni_wait",
                  "tnni_serve_stats_read", "tnni_shape_class", "tnni_autotune",
-                 "tnni_tune_read", "tnni_tune_load", "tnni_tune_clear"):
+                 "tnni_tune_read", "tnni_tune_load", "tnni_tune_clear", "tnni_tile_run_time",
+                 "tnni_tile_run_time_dev"):
         getattr(L, name).restype = ctypes.c_int
     del i32p
     _lib = L
@@ -351,6 +356,26 @@ class Context:
             self.h, k, descs, arr(dXs), arr(dYs), arr(dZs), nn, arr(d_hs),
             arr(d_labs) if d_labs is not None else None,
             arr(d_d2s) if d_d2s is not None else None))
+
+    # ---- space-time ----
+    def tile_run_time(self, p: TileParams, X, Y, Z, T, cur_time: float, time_radius: float):
+        """Heights of a space-time tile (host buffers): slices cur_time + j,
+        j = -(int)r..r, sites within the time radius at their time distance."""
+        X = np.ascontiguousarray(X, dtype=np.int32)
+        Y = np.ascontiguousarray(Y, dtype=np.int32)
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        T = np.ascontiguousarray(T, dtype=np.int32)
+        h = np.empty((p.nrows, p.ncols), dtype=np.float32)
+        d = p.to_desc()
+        self._chk(lib().tnni_tile_run_time(self.h, ctypes.byref(d), _ptr(X), _ptr(Y), _ptr(Z), _ptr(T),
+                                           X.shape[0], cur_time, time_radius, _ptr(h)))
+        return h
+
+    def tile_run_time_dev(self, p: TileParams, dX, dY, dZ, dT, n: int, cur_time: float,
+                          time_radius: float, d_h):
+        d = p.to_desc()
+        self._chk(lib().tnni_tile_run_time_dev(self.h, ctypes.byref(d), dX.ptr, dY.ptr, dZ.ptr,
+                                               dT.ptr, n, cur_time, time_radius, d_h.ptr))
 
     # ---- autotuning ----
     def autotune(self, params, dXs, dYs, dZs, ns, batched: bool = True, reps: int = 3):

@@ -503,7 +503,7 @@ Meta read_meta(const std::string &dir)
 
 // One package of a row in flight: its points of the slice and its raster.
 struct Flight {
-    std::vector<int32_t> X, Y, Z;
+    std::vector<int32_t> X, Y, Z, T;
     std::vector<float> h;
     uint64_t ticket = 0;
 };
@@ -548,24 +548,47 @@ void run_batched(const Options &o, const Layout &L, const nni::Settings &s, cons
                 tpie::ami::stream<RawPoint> st(wp_file(o.wp_path, row, col, ".raw"), tpie::ami::READ_STREAM);
                 if (!st.is_valid()) die("missing work package " + wp_file(o.wp_path, row, col, ".raw"));
                 RawPoint *p;
-                // the slice's points (nni.cpp:770 filter; the stream is sorted
-                // by time, so they are one run of it)
-                while (st.read_item(&p) == tpie::ami::NO_ERROR) {
-                    if (std::fabs(t - (float)p->t) <= tr + 0.000001f) {
+                if (tr <= 0) {
+                    // the slice's points (nni.cpp:770 filter; the stream is
+                    // sorted by time, so they are one run of it)
+                    while (st.read_item(&p) == tpie::ami::NO_ERROR) {
+                        if ((double)std::fabs(t - (float)p->t) <= (double)tr + 0.000001) {
+                            f->X.push_back(p->X);
+                            f->Y.push_back(p->Y);
+                            f->Z.push_back(p->Z);
+                        } else if ((float)p->t > t + tr) {
+                            break;
+                        }
+                    }
+                } else {
+                    // space-time: every point within 2r + 1 of t; the engine
+                    // keeps each slice's own
+                    const double w = 2.0 * (double)tr + 1.0;
+                    while (st.read_item(&p) == tpie::ami::NO_ERROR) {
+                        const double dt = (double)p->t - (double)t;
+                        if (dt > w) break;
+                        if (dt < -w) continue;
                         f->X.push_back(p->X);
                         f->Y.push_back(p->Y);
                         f->Z.push_back(p->Z);
-                    } else if ((float)p->t > t + tr) {
-                        break;
+                        f->T.push_back(p->t);
                     }
                 }
                 f->h.resize(cells);
                 tnni_tile_desc d = base;
                 d.origin_x = L.origin_x(col);
                 d.origin_y = L.origin_y(row);
-                const tnni_status stt = tnni_submit(ctx, &d, f->X.data(), f->Y.data(), f->Z.data(),
-                                                    f->X.size(), f->h.data(), nullptr, nullptr, &f->ticket);
-                if (stt != TNNI_OK) die(std::string("submit: ") + tnni_last_error(ctx));
+                tnni_status stt;
+                if (tr <= 0) {
+                    stt = tnni_submit(ctx, &d, f->X.data(), f->Y.data(), f->Z.data(), f->X.size(),
+                                      f->h.data(), nullptr, nullptr, &f->ticket);
+                } else {
+                    // the slices of one package run in order on the GPU
+                    stt = tnni_tile_run_time(ctx, &d, f->X.data(), f->Y.data(), f->Z.data(), f->T.data(),
+                                             f->X.size(), t, tr, f->h.data());
+                    f->ticket = 0;
+                }
+                if (stt != TNNI_OK) die(std::string("tile: ") + tnni_last_error(ctx));
                 fl.push_back(std::move(f));
             }
             return fl;
@@ -577,7 +600,8 @@ void run_batched(const Options &o, const Layout &L, const nni::Settings &s, cons
             std::vector<std::unique_ptr<Flight>> next;
             if (row > 0) next = submit_row(row - 1);
             for (auto &f : pending)
-                if (tnni_wait(ctx, f->ticket, nullptr) != TNNI_OK) die(std::string("tile: ") + tnni_last_error(ctx));
+                if (f->ticket && tnni_wait(ctx, f->ticket, nullptr) != TNNI_OK)
+                    die(std::string("tile: ") + tnni_last_error(ctx));
             const double tw = wall();
             // rows of this band inside the grid, top down
             const int j_hi = std::min(L.nrrows, (row + 1) * L.wp_nr) - 1, j_lo = row * L.wp_nr;
@@ -628,7 +652,6 @@ int run(int argc, char **argv)
         std::cerr << "Cannot set timeStep = 0.  Resetting it to 1.\n";
         o.time_step = 1;
     }
-    if (o.time_radius > 0) die("space-time interpolation (--time-radius > 0) is not provided by this engine");
     tpie::tempname::set_default_path(o.tmp_path);
 
     // 1. the tiles and their bounding box

@@ -111,27 +111,55 @@ void performNNI(const WorkPackage &wp, tpie::ami::stream<OutPoint> &out,
         set_error(TNNI_E_NODEVICE, "performNNI before a successful nni::init");
         return;
     }
-    if (timeRadius > 0) {
-        set_error(TNNI_E_INVALID, "space-time interpolation (timeRadius > 0) is not provided by this engine");
-        return;
-    }
-    // 1. the slice's points (renderInputPoints, nni.cpp:741-788; one slice)
+    // 1. the points (renderInputPoints, nni.cpp:741-788)
     g_t_read.start();
     const float t = curTime;
-    TPIE_OS_OFFSET seekto = vec.tell(), finalSeekTo = vec.tell();
-    bool found = false;
+    const bool space_time = timeRadius > 0;
     std::vector<Point4D> pts;
     Point4D *p = nullptr;
-    while (vec.read_item(&p) == tpie::ami::NO_ERROR) {
-        if (!found && p->t >= t + 1 - timeRadius) {
-            seekto = vec.tell() - 1;
-            finalSeekTo = seekto;
-            found = true;
+    if (!space_time) {
+        // one slice: the reference's filter and seeks
+        TPIE_OS_OFFSET seekto = vec.tell(), finalSeekTo = vec.tell();
+        bool found = false;
+        while (vec.read_item(&p) == tpie::ami::NO_ERROR) {
+            if (!found && p->t >= t + 1 - timeRadius) {
+                seekto = vec.tell() - 1;
+                finalSeekTo = seekto;
+                found = true;
+            }
+            if ((double)std::fabs(t - (float)p->t) <= (double)timeRadius + 0.000001) pts.push_back(*p);
+            else if (p->t > t + timeRadius) break;
         }
-        if (std::fabs(t - p->t) <= timeRadius + 0.000001f) pts.push_back(*p);
-        else if (p->t > t + timeRadius) break;
+        vec.seek(useDelta ? seekto : finalSeekTo);
+    } else {
+        // slices curTime + j, j = -(int)r..r, each with the points within r
+        // of it: every point within 2r + 1 of curTime is read (the engine
+        // filters each slice exactly).  The stream is left at the first
+        // point of that window, where a later call with a later curTime
+        // finds all of its points.  (The reference leaves it after the
+        // slices it drew, relying on its render-buffer cache for the rest,
+        // nni.cpp:321-379,782-788; the engine recomputes a slice instead,
+        // and a slice's sites do not depend on curTime.)
+        const double w = 2.0 * (double)timeRadius + 1.0;
+        TPIE_OS_OFFSET start = -1;
+        for (;;) {
+            const TPIE_OS_OFFSET at = vec.tell();
+            if (vec.read_item(&p) != tpie::ami::NO_ERROR) {
+                if (start < 0) start = at;
+                break;
+            }
+            const double dt = (double)p->t - (double)t;
+            if (dt > w) {
+                if (start < 0) start = at;
+                break;
+            }
+            if (dt >= -w) {
+                if (start < 0) start = at;
+                pts.push_back(*p);
+            }
+        }
+        vec.seek(start);
     }
-    vec.seek(useDelta ? seekto : finalSeekTo);
 
     // 2. the tile on the engine's grid: coordinates in its own unit
     //    relative to the tile origin, heights in millimetres
@@ -164,17 +192,22 @@ void performNNI(const WorkPackage &wp, tpie::ami::stream<OutPoint> &out,
     d.las_offset[1] = d.origin_y;
     d.las_offset[2] = 0.0;
     const size_t n = pts.size();
-    std::vector<int32_t> X(n), Y(n), Z(n);
+    std::vector<int32_t> X(n), Y(n), Z(n), T(space_time ? n : 0);
     for (size_t i = 0; i < n; ++i) {
         X[i] = clamp_i32(std::llround((pts[i].p[0] - d.origin_x) * upm));
         Y[i] = clamp_i32(std::llround((pts[i].p[1] - d.origin_y) * upm));
         Z[i] = clamp_i32(std::llround(pts[i].p[2] * 1e3));
+        if (space_time) T[i] = pts[i].t;
     }
     g_t_read.stop();
 
     g_t_tile.start();
     std::vector<float> h((size_t)wp.nrows * wp.ncols);
-    st = tnni_tile_run(g_ctx, &d, X.data(), Y.data(), Z.data(), n, h.data(), nullptr, nullptr);
+    if (space_time)
+        st = tnni_tile_run_time(g_ctx, &d, X.data(), Y.data(), Z.data(), T.data(), n, curTime,
+                                timeRadius, h.data());
+    else
+        st = tnni_tile_run(g_ctx, &d, X.data(), Y.data(), Z.data(), n, h.data(), nullptr, nullptr);
     g_t_tile.stop();
     if (st != TNNI_OK) {
         set_error(st, tnni_last_error(g_ctx));

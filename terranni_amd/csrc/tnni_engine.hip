@@ -59,6 +59,7 @@ struct Arena {
     hipEvent_t done = nullptr;
     DevBuf tiles, chunks, batch, hist, partials, banded, sorted, offs, hdist, edist;
     DevBuf sx, sy, sz, sh, sl, sd;  // device staging of host-pointer calls
+    DevBuf st, acc;                 // space-time: point times, per-cell sums
     HostBuf h_desc;                 // pinned descriptors
     HostBuf h_in, h_out;            // pinned staging of the served form
     hipGraphExec_t gexec = nullptr;
@@ -271,7 +272,8 @@ void arena_free(tnni_ctx *ctx, Arena &a)
 {
     if (a.stream) (void)hipStreamSynchronize(a.stream);
     DevBuf *bufs[] = {&a.tiles, &a.chunks, &a.batch, &a.hist, &a.partials, &a.banded, &a.sorted,
-                      &a.offs, &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd};
+                      &a.offs, &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd,
+                      &a.st, &a.acc};
     for (DevBuf *b : bufs) release(ctx, *b);
     HostBuf *hb[] = {&a.h_desc, &a.h_in, &a.h_out};
     for (HostBuf *b : hb) {
@@ -303,6 +305,8 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
 {
     if (tiles.empty()) return TNNI_OK;
     const int alg = tiles[0].t.algorithm, mode = tiles[0].t.mode;
+    const bool time = tiles[0].t.T != nullptr;
+    if (time && how == Launch::Graph) return fail(ctx, TNNI_E_INVALID, "space-time tiles run direct");
     int64_t pts = 0, bks = 0, items = 0, rows = 0, bands = 0, hents = 0;
     size_t lds = 0;
     int wg_cap = 0;       // tuned workgroups per CU of the tile kernel
@@ -310,8 +314,8 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     std::vector<Chunk> chunks;
     for (size_t k = 0; k < tiles.size(); ++k) {
         TileDev &t = tiles[k].t;
-        if (t.algorithm != alg || t.mode != mode)
-            return fail(ctx, TNNI_E_INVALID, "a batch must share algorithm and accum_mode");
+        if (t.algorithm != alg || t.mode != mode || (t.T != nullptr) != time)
+            return fail(ctx, TNNI_E_INVALID, "a batch must share algorithm, accum_mode and time mode");
         t.pt_base = pts;
         t.bk_base = bks;
         t.item_base = items;
@@ -384,6 +388,7 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     cfg.main_lds = lds;
     cfg.algorithm = alg;
     cfg.mode = mode;
+    cfg.time = time ? 1 : 0;
     DevBuffers db;
     db.batch = (BatchDev *)a.batch.p;
     db.tiles = (const TileDev *)a.tiles.p;
@@ -476,7 +481,8 @@ void finish_request(tnni_ctx *ctx, const std::shared_ptr<Request> &r, tnni_statu
 void arena_trim(tnni_ctx *ctx, Arena &a)
 {
     DevBuf *bufs[] = {&a.tiles, &a.chunks, &a.batch, &a.hist, &a.partials, &a.banded, &a.sorted,
-                      &a.offs, &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd};
+                      &a.offs, &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd,
+                      &a.st, &a.acc};
     for (DevBuf *b : bufs) release(ctx, *b);
     if (a.gexec) (void)hipGraphExecDestroy(a.gexec);
     a.gexec = nullptr;
@@ -890,6 +896,111 @@ tnni_status tnni_tile_run(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32
     if (out_labels)
         HIP_TRY(ctx, hipMemcpyAsync(out_labels, a.sl.p, pix * 4, hipMemcpyDeviceToHost, a.stream));
     if (out_d2) HIP_TRY(ctx, hipMemcpyAsync(out_d2, a.sd.p, pix * 4, hipMemcpyDeviceToHost, a.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(a.stream));
+    return TNNI_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The slices of one space-time tile (performNNI's loop, nni.cpp:910-933):
+// j = -(int)r .. r, slice time t = cur_time + j in float, each a full
+// pipeline run over the tile's points that keeps the points within the time
+// radius of t as sites; the cells' sums stay in the arena between the runs
+// and the last run divides.  p.t.X/Y/Z/T and out_h are set by the caller.
+tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_time,
+                          float time_radius)
+{
+    const TileDev &t = p.t;
+    if (!(time_radius >= 0.0f) || time_radius > 1.0e4f)
+        return fail(ctx, TNNI_E_INVALID, "time_radius must be in [0, 1e4]");
+    // squared distances plus time terms stay below the "no site" key
+    const uint64_t u = (uint64_t)std::ceil((double)time_radius * t.upm) + 1;
+    const uint64_t dxy = t.algorithm == TNNI_ALG_PLANES ? 2ull * t.Rs * t.Rs : (uint64_t)t.Rs2;
+    const uint64_t dq = t.algorithm == TNNI_ALG_PLANES ? 2ull * t.Rq * t.Rq : (uint64_t)t.Rq2;
+    if (dxy + u * u >= (uint64_t)kNoD2 || dq + u * u >= (uint64_t)kNoD2)
+        return fail(ctx, TNNI_E_GEOMETRY, "time radius too large for 32-bit squared distances");
+    const size_t cells = (size_t)t.nrows * t.ncols;
+    tnni_status st = ensure(ctx, a.acc, cells * sizeof(uint4));
+    if (st != TNNI_OK) return st;
+    const int j0 = -(int)time_radius;
+    int j1 = j0;
+    while ((float)(j1 + 1) <= time_radius) ++j1;
+    for (int j = j0; j <= j1; ++j) {
+        std::vector<Prepared> v(1, p);
+        TileDev &s = v[0].t;
+        s.slice_t = cur_time + (float)j;
+        s.time_radius = time_radius;
+        const int64_t ju = (int64_t)j * t.upm;
+        s.jj2 = (uint32_t)(ju * ju);
+        s.acc_load = j != j0;
+        s.acc_final = j == j1;
+        s.acc = (uint4 *)a.acc.p;
+        if ((st = run_prepared(ctx, a, v, false, Launch::Direct)) != TNNI_OK) return st;
+    }
+    return TNNI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tnni_status tnni_tile_run_time_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *dX,
+                                   const int32_t *dY, const int32_t *dZ, const int32_t *dT,
+                                   uint64_t n, float cur_time, float time_radius, float *d_out_h)
+{
+    if (!ctx || !desc || !d_out_h || (n && (!dX || !dY || !dZ || !dT)))
+        return fail(ctx, TNNI_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    Prepared p;
+    std::string err;
+    tnni_status st = prepare_tile(*desc, n, ctx->n_cu, false, &ctx->tuned, p.t, p.info, err);
+    if (st != TNNI_OK) return fail(ctx, st, err);
+    p.t.X = dX;
+    p.t.Y = dY;
+    p.t.Z = dZ;
+    // with no points T is never read: any non-null pointer marks the time mode
+    p.t.T = n ? dT : (const int32_t *)d_out_h;
+    p.t.out_h = d_out_h;
+    if ((st = run_time_tile(ctx, ctx->main, p, cur_time, time_radius)) != TNNI_OK) return st;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->main.stream));
+    return TNNI_OK;
+}
+
+tnni_status tnni_tile_run_time(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *X,
+                               const int32_t *Y, const int32_t *Z, const int32_t *T, uint64_t n,
+                               float cur_time, float time_radius, float *out_h)
+{
+    if (!ctx || !desc || !out_h || (n && (!X || !Y || !Z || !T)))
+        return fail(ctx, TNNI_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    Prepared p;
+    std::string err;
+    tnni_status st = prepare_tile(*desc, n, ctx->n_cu, false, &ctx->tuned, p.t, p.info, err);
+    if (st != TNNI_OK) return fail(ctx, st, err);
+    Arena &a = ctx->main;
+    const size_t cells = (size_t)p.t.nrows * p.t.ncols;
+    if ((st = ensure(ctx, a.sx, n * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.sy, n * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.sz, n * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.st, n * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.sh, cells * 4)) != TNNI_OK) return st;
+    if (n) {
+        HIP_TRY(ctx, hipMemcpyAsync(a.sx.p, X, n * 4, hipMemcpyHostToDevice, a.stream));
+        HIP_TRY(ctx, hipMemcpyAsync(a.sy.p, Y, n * 4, hipMemcpyHostToDevice, a.stream));
+        HIP_TRY(ctx, hipMemcpyAsync(a.sz.p, Z, n * 4, hipMemcpyHostToDevice, a.stream));
+        HIP_TRY(ctx, hipMemcpyAsync(a.st.p, T, n * 4, hipMemcpyHostToDevice, a.stream));
+    }
+    p.t.X = (const int32_t *)a.sx.p;
+    p.t.Y = (const int32_t *)a.sy.p;
+    p.t.Z = (const int32_t *)a.sz.p;
+    p.t.T = n ? (const int32_t *)a.st.p : (const int32_t *)a.sh.p;  // non-null: time mode
+    p.t.out_h = (float *)a.sh.p;
+    if ((st = run_time_tile(ctx, a, p, cur_time, time_radius)) != TNNI_OK) return st;
+    HIP_TRY(ctx, hipMemcpyAsync(out_h, a.sh.p, cells * 4, hipMemcpyDeviceToHost, a.stream));
     HIP_TRY(ctx, hipStreamSynchronize(a.stream));
     return TNNI_OK;
 }

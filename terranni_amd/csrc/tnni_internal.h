@@ -57,6 +57,16 @@ struct TileDev {
     int tc, ntx, nty;           // cell-tile side and tiles along x / y
     int pad0;
     int64_t item_base;          // first work item of this tile
+    // space-time slice (T != nullptr): the sites are the points with
+    // |slice_t - T[i]| <= time_radius + 1e-6, at squared time distance dt2
+    // (units^2) added to their squared distance; queries add jj2; acc holds
+    // every cell's sums between the slices of one tile
+    const int32_t *T;
+    float slice_t, time_radius;
+    uint32_t jj2;
+    int acc_load, acc_final;    // start from acc / divide into out_h
+    int pad2;
+    uint4 *acc;
     float *out_h;
     int32_t *out_labels;
     uint32_t *out_d2;
@@ -136,6 +146,7 @@ struct LaunchCfg {
     size_t main_lds;    // dynamic LDS bytes of the tile kernel
     int algorithm;      // TNNI_ALG_* of every tile in the batch
     int mode;           // TNNI_ACC_* of every tile in the batch
+    int time;           // space-time slices (every tile has T)
 };
 
 struct DevBuffers {

@@ -47,6 +47,20 @@ __device__ __forceinline__ int32_t site_value(const TileDev &t, int32_t Z)
     return (int32_t)(uint32_t)(int64_t)v;
 }
 
+// Space-time: whether a point of time Ti is a site of the slice at
+// t.slice_t (renderInputPoints' filter, nni.cpp:770: |t - p.t| in float
+// against timeRadius + 1e-6 in double) and its squared time distance in
+// units^2 (drawPlane's timeOffset term, nni.cpp:197: one time unit is one
+// metre), the distance rounded to the unit.
+__device__ __forceinline__ bool slice_site(const TileDev &t, int32_t Ti, uint32_t &dt2)
+{
+    const float dt = fabsf(t.slice_t - (float)Ti);
+    const bool in = (double)dt <= (double)t.time_radius + 0.000001;
+    const uint32_t u = in ? (uint32_t)rint((double)dt * (double)t.upm) : 0u;
+    dt2 = u * u;
+    return in;
+}
+
 // v / d for 0 <= v < 2^30: FP64 reciprocal, then at most one correction.
 __device__ __forceinline__ int div_pos(int v, int d, double inv_d)
 {
@@ -90,17 +104,20 @@ k_band_count(BatchDev *b, const TileDev *tiles, const Chunk *chunks, uint32_t *h
         for (int i = threadIdx.x; i < nb; i += kBlock) h[i] = 0u;
         __syncthreads();
         for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kBlock * kPtsPerStep) {
-            int32_t X[kPtsPerStep], Y[kPtsPerStep];
+            int32_t X[kPtsPerStep], Y[kPtsPerStep], Tm[kPtsPerStep];
 #pragma unroll
             for (int q = 0; q < kPtsPerStep; ++q) {
                 const int64_t i = i0 + (int64_t)q * kBlock;
                 X[q] = i < ch.end ? t.X[i] : 0;
                 Y[q] = i < ch.end ? t.Y[i] : 0;
+                Tm[q] = (t.T && i < ch.end) ? t.T[i] : 0;
             }
 #pragma unroll
             for (int q = 0; q < kPtsPerStep; ++q) {
                 int xu, yu;
-                if (i0 + (int64_t)q * kBlock < ch.end && point_units(t, X[q], Y[q], xu, yu))
+                uint32_t dt2;
+                if (i0 + (int64_t)q * kBlock < ch.end && point_units(t, X[q], Y[q], xu, yu) &&
+                    (!t.T || slice_site(t, Tm[q], dt2)))
                     atomicAdd(&h[point_band(t, yu)], 1u);
             }
         }
@@ -135,15 +152,18 @@ k_band_scatter(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
                 const int64_t i = i0 + (int64_t)q * kBlock;
                 X[q] = i < ch.end ? t.X[i] : 0;
                 Y[q] = i < ch.end ? t.Y[i] : 0;
-                Z[q] = i < ch.end ? t.Z[i] : 0;
+                // space-time sites carry their time instead of their value
+                Z[q] = i < ch.end ? (t.T ? t.T[i] : t.Z[i]) : 0;
             }
 #pragma unroll
             for (int q = 0; q < kPtsPerStep; ++q) {
                 const int64_t i = i0 + (int64_t)q * kBlock;
                 int xu, yu;
-                if (i < ch.end && point_units(t, X[q], Y[q], xu, yu)) {
+                uint32_t dt2 = 0;
+                if (i < ch.end && point_units(t, X[q], Y[q], xu, yu) &&
+                    (!t.T || slice_site(t, Z[q], dt2))) {
                     const uint32_t pos = atomicAdd(&cur[point_band(t, yu)], 1u);
-                    banded[pos] = make_int4(xu, yu, (int)i, site_value(t, Z[q]));
+                    banded[pos] = make_int4(xu, yu, (int)i, t.T ? (int)dt2 : site_value(t, Z[q]));
                 }
             }
         }
@@ -366,32 +386,36 @@ k_edist_cols(const BatchDev *b, const TileDev *tiles, const uint8_t *__restrict_
 // pixel, so the region test is skipped.
 constexpr int kAllCover = 2;
 
-template <int ALG>
+// TIME (space-time slice): st.w is the site's squared time distance, added
+// to the distance; the region test stays in the plane (nni.cpp:117-157,
+// 207-225), and the value comes from the label afterwards.
+template <int ALG, bool TIME>
 __device__ __forceinline__ void visit_site(const int4 st, int px, int py, int R, uint32_t R2,
                                            uint64_t &best, int &bval)
 {
     const int dx = px - st.x, dy = py - st.y;
-    const uint32_t dd = (uint32_t)__mul24(dx, dx) + (uint32_t)__mul24(dy, dy);
+    const uint32_t dxy = (uint32_t)__mul24(dx, dx) + (uint32_t)__mul24(dy, dy);
+    const uint32_t dd = TIME ? dxy + (uint32_t)st.w : dxy;
     if (ALG == kAllCover) {
         const uint64_t key = ((uint64_t)dd << 32) | (uint32_t)st.z;
         const bool better = key < best;
         best = better ? key : best;
-        bval = better ? st.w : bval;
+        if (!TIME) bval = better ? st.w : bval;
         return;
     }
     bool cov = max((uint32_t)(dx + R), (uint32_t)(dy + R)) <= 2u * (uint32_t)R;
-    if (ALG == TNNI_ALG_CONES) cov = cov && dd <= R2;
+    if (ALG == TNNI_ALG_CONES) cov = cov && dxy <= R2;
     // a site that does not cover the pixel gets the key of "no site"
     const uint32_t hi = cov ? dd : 0xFFFFFFFFu;
     const uint32_t lo = cov ? (uint32_t)st.z : 0xFFFFFFFFu;
     const uint64_t key = ((uint64_t)hi << 32) | lo;
     const bool better = key < best;
     best = better ? key : best;
-    bval = better ? st.w : bval;
+    if (!TIME) bval = better ? st.w : bval;
 }
 
 // Visit the sites stored at [s, e) of a site array (global or LDS).
-template <int ALG>
+template <int ALG, bool TIME>
 __device__ __forceinline__ void scan_range(uint32_t s, uint32_t e, const int4 *__restrict__ sites,
                                            int px, int py, int R, uint32_t R2,
                                            uint64_t &best, int &bval)
@@ -401,14 +425,14 @@ __device__ __forceinline__ void scan_range(uint32_t s, uint32_t e, const int4 *_
     for (; p + 1 < e; p += 2) {
         const int4 a = sites[p];
         const int4 b = sites[p + 1];
-        visit_site<ALG>(a, px, py, R, R2, best, bval);
-        visit_site<ALG>(b, px, py, R, R2, best, bval);
+        visit_site<ALG, TIME>(a, px, py, R, R2, best, bval);
+        visit_site<ALG, TIME>(b, px, py, R, R2, best, bval);
     }
-    if (p < e) visit_site<ALG>(sites[p], px, py, R, R2, best, bval);
+    if (p < e) visit_site<ALG, TIME>(sites[p], px, py, R, R2, best, bval);
 }
 
 // Ring r of the bucket square around (bx, by), from the global arrays.
-template <int ALG>
+template <int ALG, bool TIME>
 __device__ __forceinline__ void ring_global(const TileDev &t, const uint32_t *__restrict__ offs,
                                             const int4 *__restrict__ sorted, int bx, int by, int r,
                                             int px, int py, int R, uint64_t &best, int &bval)
@@ -435,8 +459,8 @@ __device__ __forceinline__ void ring_global(const TileDev &t, const uint32_t *__
                 b1 = rs[bx + r + 1];
             }
         }
-        scan_range<ALG>(a0, a1, sorted, px, py, R, t.Rs2, best, bval);
-        scan_range<ALG>(b0, b1, sorted, px, py, R, t.Rs2, best, bval);
+        scan_range<ALG, TIME>(a0, a1, sorted, px, py, R, t.Rs2, best, bval);
+        scan_range<ALG, TIME>(b0, b1, sorted, px, py, R, t.Rs2, best, bval);
     }
 }
 
@@ -469,7 +493,7 @@ __device__ __forceinline__ void wave_sync()
 }
 
 // Ring r (0 or 1) of the bucket square around (bx, by), from the stage.
-template <int ALG>
+template <int ALG, bool TIME>
 __device__ __forceinline__ void ring_staged(const TileDev &t, const WaveStage &ws, int cx0, int cy0,
                                             int wp, int bx, int by, int r, int px, int py, int R,
                                             uint64_t &best, int &bval)
@@ -497,8 +521,8 @@ __device__ __forceinline__ void ring_staged(const TileDev &t, const WaveStage &w
                 b1 = wr[bx + r + 1 - cx0] + rel;
             }
         }
-        scan_range<ALG>(a0, a1, ws.sites, px, py, R, t.Rs2, best, bval);
-        scan_range<ALG>(b0, b1, ws.sites, px, py, R, t.Rs2, best, bval);
+        scan_range<ALG, TIME>(a0, a1, ws.sites, px, py, R, t.Rs2, best, bval);
+        scan_range<ALG, TIME>(b0, b1, ws.sites, px, py, R, t.Rs2, best, bval);
     }
 }
 
@@ -510,7 +534,7 @@ __device__ __forceinline__ void ring_staged(const TileDev &t, const WaveStage &w
 // every unvisited site is provably farther than the best so far (strictly,
 // so an equal distance still reaches the lower index) or outside the site
 // region.
-template <int ALG>
+template <int ALG, bool TIME>
 __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__restrict__ offs,
                                             const uint8_t *__restrict__ edist,
                                             const int4 *__restrict__ sorted, WaveStage &ws,
@@ -588,12 +612,12 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
         if (staged && r <= 1) {
             // the sites of rings 0 and 1 lie within 2*bs (L-inf) of the pixel
             if (t.ring1_covered)
-                ring_staged<kAllCover>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
+                ring_staged<kAllCover, TIME>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
             else
-                ring_staged<ALG>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
+                ring_staged<ALG, TIME>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
         }
         else
-            ring_global<ALG>(t, offs, sorted, bx, by, r, px, py, R, best, bval);
+            ring_global<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, best, bval);
     }
 }
 
@@ -627,6 +651,35 @@ struct Acc {
         }
     }
 
+    // The running sums as 16 bytes, carried between the slices of a
+    // space-time tile.
+    __device__ __forceinline__ void load(const uint4 &v)
+    {
+        if (MODE == TNNI_ACC_FIXED64) s64 = (int64_t)(((uint64_t)v.y << 32) | v.x);
+        else if (MODE == TNNI_ACC_FP32_2SUM) {
+            hi = __uint_as_float(v.x);
+            lo = __uint_as_float(v.y);
+        } else numer = v.x;
+        cnt = v.z;
+    }
+    __device__ __forceinline__ uint4 save() const
+    {
+        uint4 v;
+        if (MODE == TNNI_ACC_FIXED64) {
+            v.x = (uint32_t)(uint64_t)s64;
+            v.y = (uint32_t)((uint64_t)s64 >> 32);
+        } else if (MODE == TNNI_ACC_FP32_2SUM) {
+            v.x = __float_as_uint(hi);
+            v.y = __float_as_uint(lo);
+        } else {
+            v.x = numer;
+            v.y = 0u;
+        }
+        v.z = cnt;
+        v.w = 0u;
+        return v;
+    }
+
     __device__ __forceinline__ float result(const TileDev &t) const
     {
         if (MODE == TNNI_ACC_FIXED64) {
@@ -649,7 +702,7 @@ struct Acc {
     }
 };
 
-template <int ALG, int MODE>
+template <int ALG, int MODE, bool TIME>
 __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, uint32_t *smem,
                                               const uint32_t *__restrict__ offs,
                                               const uint8_t *__restrict__ edist,
@@ -671,8 +724,10 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
         const bool active = k <= kB && l <= lB;
         uint64_t best;
         int bval;
-        label_block<ALG>(t, offs, edist, sorted, ws, kA, lA, kB, lB, k, l, active, best, bval);
+        label_block<ALG, TIME>(t, offs, edist, sorted, ws, kA, lA, kB, lB, k, l, active, best, bval);
         if (!active) continue;
+        // a space-time label's value: the winning point's height
+        if (TIME) bval = best == ~0ull ? 0 : site_value(t, t.Z[(uint32_t)best]);
         const uint32_t d2 = (uint32_t)(best >> 32);
         s_d2[(l - g.rl0) * RW + (k - g.rk0)] = d2;
         s_val[(l - g.rl0) * RW + (k - g.rk0)] = bval;
@@ -693,23 +748,28 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
         const int ma = max(qa, -i * t.scale), mb = min(qb, t.W - 1 - i * t.scale);
         const int na = max(qa, -j * t.scale), nb = min(qb, t.H - 1 - j * t.scale);
         Acc<MODE> acc;
+        const int64_t cell = (int64_t)j * t.ncols + i;
+        if (TIME && t.acc_load) acc.load(t.acc[cell]);
         for (int n = na; n <= nb; ++n) {
             const uint32_t uy = (uint32_t)(t.y0 + n * t.ps_u);
             const int row = (j * t.scale + n - g.rl0) * RW - g.rk0 + i * t.scale;
             for (int m = ma; m <= mb; ++m) {
                 const uint32_t ux = (uint32_t)(t.x0 + m * t.ps_u);
+                // the query region is in the plane; a space-time query adds
+                // its time offset to the distance only
                 const uint32_t dq = ux * ux + uy * uy;
                 const uint32_t d2 = s_d2[row + m];
-                bool steal = d2 != kNoD2 && dq <= d2;
+                bool steal = d2 != kNoD2 && dq + (TIME ? t.jj2 : 0u) <= d2;
                 if (ALG == TNNI_ALG_CONES) steal = steal && dq <= t.Rq2;
                 if (steal) acc.add(s_val[row + m]);
             }
         }
-        t.out_h[(int64_t)j * t.ncols + i] = acc.result(t);
+        if (!TIME || t.acc_final) t.out_h[cell] = acc.result(t);
+        else t.acc[cell] = acc.save();
     }
 }
 
-template <int ALG, int MODE>
+template <int ALG, int MODE, bool TIME>
 __global__ void __launch_bounds__(kBlock)
 k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
         const uint8_t *__restrict__ edist, const int4 *__restrict__ sorted)
@@ -735,15 +795,15 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
         const TileDev t = tiles[lo];
         const int64_t local = item - t.item_base;
         const int tj = (int)(local / t.ntx), ti = (int)(local % t.ntx);
-        process_block<ALG, MODE>(t, ti, tj, smem, offs, edist, sorted);
+        process_block<ALG, MODE, TIME>(t, ti, tj, smem, offs, edist, sorted);
         __syncthreads();
     }
 }
 
-template <int ALG, int MODE>
+template <int ALG, int MODE, bool TIME>
 hipError_t launch_tiles(const DevBuffers &d, const LaunchCfg &cfg, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_tiles<ALG, MODE>), dim3(cfg.main_grid), dim3(kBlock), cfg.main_lds, s,
+    hipLaunchKernelGGL((k_tiles<ALG, MODE, TIME>), dim3(cfg.main_grid), dim3(kBlock), cfg.main_lds, s,
                        d.batch, d.tiles, d.offs, d.edist, d.sorted);
     return hipGetLastError();
 }
@@ -769,14 +829,20 @@ hipError_t enqueue_pipeline(const DevBuffers &d, const LaunchCfg &cfg, hipStream
                        d.hdist, d.edist);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
-    const int sel = cfg.algorithm * 3 + cfg.mode;
+    const int sel = (cfg.time ? 6 : 0) + cfg.algorithm * 3 + cfg.mode;
     switch (sel) {
-    case 0: e = launch_tiles<0, 0>(d, cfg, s); break;
-    case 1: e = launch_tiles<0, 1>(d, cfg, s); break;
-    case 2: e = launch_tiles<0, 2>(d, cfg, s); break;
-    case 3: e = launch_tiles<1, 0>(d, cfg, s); break;
-    case 4: e = launch_tiles<1, 1>(d, cfg, s); break;
-    case 5: e = launch_tiles<1, 2>(d, cfg, s); break;
+    case 0: e = launch_tiles<0, 0, false>(d, cfg, s); break;
+    case 1: e = launch_tiles<0, 1, false>(d, cfg, s); break;
+    case 2: e = launch_tiles<0, 2, false>(d, cfg, s); break;
+    case 3: e = launch_tiles<1, 0, false>(d, cfg, s); break;
+    case 4: e = launch_tiles<1, 1, false>(d, cfg, s); break;
+    case 5: e = launch_tiles<1, 2, false>(d, cfg, s); break;
+    case 6: e = launch_tiles<0, 0, true>(d, cfg, s); break;
+    case 7: e = launch_tiles<0, 1, true>(d, cfg, s); break;
+    case 8: e = launch_tiles<0, 2, true>(d, cfg, s); break;
+    case 9: e = launch_tiles<1, 0, true>(d, cfg, s); break;
+    case 10: e = launch_tiles<1, 1, true>(d, cfg, s); break;
+    case 11: e = launch_tiles<1, 2, true>(d, cfg, s); break;
     default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
@@ -789,7 +855,7 @@ size_t tile_kernel_stage_bytes() { return (size_t)kWaves * sizeof(WaveStage); }
 int tile_kernel_occupancy(size_t lds_bytes)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tiles<0, 0>, kBlock, lds_bytes) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tiles<0, 0, false>, kBlock, lds_bytes) !=
         hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;

@@ -105,3 +105,34 @@ def test_driver_time_slice_and_reuse_of_packages(tmp_path):
     _, out2 = run_driver(tmp_path, "batched", ["--time-start=1", "--format=flt", "--wp-computed"])
     h2 = read_flt(out2, "1.00000")
     assert np.array_equal(h2.view(np.uint32), ref["h_fixed"].view(np.uint32))
+
+
+def test_driver_space_time_both_engines(tmp_path):
+    """--time-radius 1 at time 1: the slices 0, 1, 2 with their time terms,
+    summed (nni.cpp:902-936), over the whole grid as one oracle tile."""
+    (tmp_path / "las").mkdir()
+    strips = write_scene(str(tmp_path / "las"))
+    X = np.concatenate([s[0] for s in strips])
+    Y = np.concatenate([s[1] for s in strips])
+    Z = np.concatenate([s[2] for s in strips])
+    times = np.concatenate([s[3] for s in strips]).astype(np.int32)
+    # the packages are sorted by time (stably), so the points' order, which
+    # breaks equal-distance ties, is the time order
+    o = np.argsort(times, kind="stable")
+    X, Y, Z, times = X[o], Y[o], Z[o], times[o]
+    g = GRID
+    p = T.TileParams(nrows=g["rows"], ncols=g["cols"], cell_size=g["cs"], scale=g["scale"],
+                     site_radius=g["sr"], origin_x=OFFSET[0], origin_y=OFFSET[1],
+                     las_scale=S.LAS_SCALE, las_offset=OFFSET)
+    ref = O.run_time(p, X, Y, Z, times, 1.0, 1.0)
+    _say("driver batched --time-radius 1")
+    _, out_b = run_driver(tmp_path, "batched", ["--time-start=1", "--time-radius=1", "--format=flt"])
+    hb = read_flt(out_b, "1.00000")
+    assert np.array_equal(hb.view(np.uint32), ref["h_fixed"].view(np.uint32))
+    _say("driver plain --time-radius 1")
+    _, out_p = run_driver(tmp_path, "plain", ["--time-start=1", "--time-radius=1", "--format=flt"])
+    hp = read_flt(out_p, "1.00000").astype(np.float64)
+    valid = ref["h64"] != T.NODATA
+    assert np.array_equal(valid, hp != T.NODATA)
+    rel = np.abs(hp[valid] - ref["h64"][valid]) / np.maximum(np.abs(ref["h64"][valid]), REL_FLOOR)
+    assert rel.max() <= 1e-6

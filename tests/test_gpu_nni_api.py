@@ -30,9 +30,12 @@ def _run_api(tmp_path, pts, args):
     return np.fromfile(pout, dtype=OUT)
 
 
-@pytest.mark.parametrize("grid,n,cs,scale,slices,gox", [
-    (64, 3000, 0.5, 1, 1, 0), (40, 2000, 2.0, 5, 3, 0), (50, 4000, 1.0, 1, 2, 7)])
-def test_perform_nni_matches_oracle(tmp_path, grid, n, cs, scale, slices, gox):
+@pytest.mark.parametrize("grid,n,cs,scale,slices,gox,tr", [
+    (64, 3000, 0.5, 1, 1, 0, 0.0), (40, 2000, 2.0, 5, 3, 0, 0.0), (50, 4000, 1.0, 1, 2, 7, 0.0),
+    (48, 6000, 0.5, 1, 4, 0, 1.0), (40, 4000, 1.0, 2, 3, 3, 2.0)])
+def test_perform_nni_matches_oracle(tmp_path, grid, n, cs, scale, slices, gox, tr):
+    """tr > 0: space-time slices (performNNI's timeRadius), one call per
+    time t = 0..slices-1 on the same stream, as binRun's time loop."""
     tile = S.make_tile(40 + grid, n, grid, cell_size=cs, scale=scale, kind="clustered",
                        origin=(1000.0, 2000.0))
     p = tile.params
@@ -46,13 +49,13 @@ def test_perform_nni_matches_oracle(tmp_path, grid, n, cs, scale, slices, gox):
     total = grid + gox - 3  # the total grid clips the tile's last columns/rows
     args = dict(scale=scale, cell_size=cs, nrows=total, ncols=total, origin_x=p.origin_x,
                 origin_y=p.origin_y, grid_origin_x=gox, grid_origin_y=gox, wp_nrows=grid,
-                wp_ncols=grid, slices=slices)
+                wp_ncols=grid, slices=slices, time_radius=tr)
     out = _run_api(tmp_path, pts, args)
     f = T.frame(p)
     upm = f["upm"]
     exp = []
     for s in range(slices):
-        sel = t == s
+        sel = (t == s) if tr == 0 else np.ones(tile.n, bool)
         X = np.rint((x[sel] - p.origin_x) * upm).astype(np.int32)
         Y = np.rint((y[sel] - p.origin_y) * upm).astype(np.int32)
         Z = np.rint(z[sel] * 1e3).astype(np.int32)
@@ -60,7 +63,10 @@ def test_perform_nni_matches_oracle(tmp_path, grid, n, cs, scale, slices, gox):
                          origin_x=p.origin_x, origin_y=p.origin_y,
                          las_scale=(1.0 / upm, 1.0 / upm, 1e-3),
                          las_offset=(p.origin_x, p.origin_y, 0.0))
-        ref = O.run(q, X, Y, Z)["h_fixed"]
+        if tr == 0:
+            ref = O.run(q, X, Y, Z)["h_fixed"]
+        else:
+            ref = O.run_time(q, X, Y, Z, t, float(s), tr)["h_fixed"]
         for i in range(grid):          # column-major, as nni.cpp:836-860
             for j in range(grid):
                 if i + gox < total and j + gox < total:
