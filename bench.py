@@ -142,7 +142,8 @@ def bench_served(T, tiles, rates, per_rate, resident: bool, seed=4, tune_table=N
     out = []
     rng = np.random.default_rng(seed)
     ring = 192  # output rasters in flight (resident)
-    with T.Context(0, n_slots=3, max_batch_tiles=64) as c:
+    # admission by HBM footprint within a 200 GB budget
+    with T.Context(0, n_slots=3, max_batch_tiles=64, hbm_budget=200 << 30) as c:
         if tune_table:
             c.tune_load(tune_table)
         if resident:
@@ -255,6 +256,126 @@ def bench_batched_only(T, S, args):
     print(json.dumps(out))
 
 
+def bench_stream(T, S, args):
+    """--workload stream: the streamed-terrain pipeline end to end, down-scaled
+    (fBm terrain, Hurst 0.8, LAS tiles of 1 km with 5 % overlap duplicates):
+    the driver (LAS reader, binning with border copies, batched engine,
+    GridFloat output) with int64 fixed-point and with FP32 two-sum heights;
+    the CPU oracle on about 1 % of the work packages is the CPU baseline and
+    the error check."""
+    import shutil
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    driver = os.path.join(ROOT, "terranni_amd", "_lib", "terranni")
+    work = tempfile.mkdtemp(prefix="tnni_stream_")
+    ext = args.stream_km * 1000.0
+    cs = 0.5
+    grid = int(round(ext / cs))
+    origin = (500000.0, 4200000.0)
+    try:
+        las = os.path.join(work, "las")
+        os.makedirs(las)
+        t0 = time.perf_counter()
+        files = S.streamed_terrain_las(las, seed=5, extent_m=ext, n_points=args.stream_points,
+                                       origin=origin)
+        n_recs = sum((os.path.getsize(f) - 227) // 20 for f in files)
+        log(f"streamed terrain: {len(files)} LAS tiles, {n_recs} records, generated in "
+            f"{time.perf_counter() - t0:.1f}s")
+
+        def run(tag, extra):
+            out = os.path.join(work, "out_" + tag)
+            wp = os.path.join(work, "wp")
+            os.makedirs(out, exist_ok=True)
+            os.makedirs(wp, exist_ok=True)
+            cmd = [driver, f"--tilepath={las}", f"--output={out}", f"--wp-path={wp}",
+                   f"--tmp-path={work}", f"--cell-size={cs}", "--scaling=1", "--site-radius=5",
+                   f"--origin-x={origin[0]}", f"--origin-y={origin[1]}", f"--grid-cols={grid}",
+                   f"--grid-rows={grid}", f"--wp-width={args.stream_wp_m}",
+                   f"--wp-height={args.stream_wp_m}", "--engine=batched", "--2d", "--format=flt",
+                   *extra]
+            a = time.perf_counter()
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            wall = time.perf_counter() - a
+            if r.returncode != 0:
+                raise RuntimeError(f"driver failed: {r.stderr[-2000:]}")
+            ph = {}
+            for line in r.stderr.splitlines():
+                if line.startswith("Total ") and ":" in line:
+                    k, v = line.split(":", 1)
+                    ph[k.replace("Total ", "").replace(" time", "").strip().lower()] = float(v)
+            log(f"driver {tag}: {wall:.2f}s {ph}")
+            h = np.fromfile(os.path.join(out, "0.00000.flt"), np.float32).reshape(grid, grid)[::-1]
+            return wall, ph, h
+
+        wall64, ph64, h64 = run("fixed64", ["--accum=fixed64"])
+        wall32, ph32, h32 = run("fp32", ["--accum=fp32", "--wp-computed"])
+        cells = grid * grid
+        valid = h64 != T.NODATA
+        same_nodata = bool(np.array_equal(valid, h32 != T.NODATA))
+        d = np.abs(h32[valid].astype(np.float64) - h64[valid]) / np.maximum(np.abs(h64[valid]), 1e-6)
+
+        # CPU oracle on about 1 % of the work packages (at least one)
+        import pyoracle as O
+        meta = dict(line.split("=", 1) for line in open(os.path.join(work, "wp", "wp.meta")).read().split())
+        nc, nr = int(meta["wp_ncols"]), int(meta["wp_nrows"])
+        cols, rows = int(meta["cols"]), int(meta["rows"])
+        rng = np.random.default_rng(7)
+        k = max(1, int(round(0.01 * cols * rows)))
+        picks = rng.choice(cols * rows, k, replace=False)
+        raw = np.dtype([("X", "<i4"), ("Y", "<i4"), ("Z", "<i4"), ("t", "<i4")])
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        o_s, o_cells, emax_fixed, emax_64, bits = 0.0, 0, 0.0, 0.0, 0
+        for w in picks:
+            r, c = divmod(int(w), cols)
+            pts = np.fromfile(os.path.join(work, "wp", f"r{r}c{c}.raw"), dtype=raw)
+            p = T.TileParams(nrows=nr, ncols=nc, cell_size=cs, scale=1, site_radius=5.0,
+                             origin_x=origin[0] + c * nc * cs, origin_y=origin[1] + r * nr * cs,
+                             las_scale=S.LAS_SCALE,
+                             las_offset=(float(meta["offset0"]), float(meta["offset1"]), float(meta["offset2"])))
+            a = time.perf_counter()
+            ref = O.run(p, pts["X"], pts["Y"], pts["Z"], nthreads=threads)
+            o_s += time.perf_counter() - a
+            j1, i1 = min(grid, (r + 1) * nr), min(grid, (c + 1) * nc)
+            g64 = h64[r * nr:j1, c * nc:i1]
+            g32 = h32[r * nr:j1, c * nc:i1]
+            rf = ref["h_fixed"][:j1 - r * nr, :i1 - c * nc]
+            r64 = ref["h64"][:j1 - r * nr, :i1 - c * nc]
+            o_cells += g64.size
+            bits += int(np.count_nonzero(g64.view(np.uint32) != rf.view(np.uint32)))
+            v = r64 != T.NODATA
+            if v.any():
+                den = np.maximum(np.abs(r64[v]), 1e-6)
+                emax_fixed = max(emax_fixed, float((np.abs(g64[v] - r64[v]) / den).max()))
+                emax_64 = max(emax_64, float((np.abs(g32[v] - r64[v]) / den).max()))
+        nni_s = ph64.get("nni", wall64)
+        out = {
+            "metric": "DEM cells/s (streamed terrain, driver end to end, batched engine)",
+            "value": cells / nni_s, "unit": "cells/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": nni_s * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "int32",
+            "data": "synthetic (seeded fBm terrain, Hurst 0.8, LAS int32 at 0.01 m)",
+            "config": {"workload": f"streamed terrain {args.stream_km} x {args.stream_km} km, "
+                                   f"{grid}^2 DEM at 0.5 m, {len(files)} LAS tiles of 1 km with 5 % "
+                                   f"overlap, work packages of {args.stream_wp_m} m",
+                       "points": int(n_recs), "cells": cells, "form": "driver, batched engine"},
+            "points_per_s": n_recs / nni_s,
+            "end_to_end_cells_per_s": cells / wall64,
+            "phases_s": {"fixed64": ph64, "fp32": ph32},
+            "fp32_vs_fixed64": {"max_rel": float(d.max()) if d.size else 0.0,
+                                "rms_rel": float(np.sqrt(np.mean(d * d))) if d.size else 0.0,
+                                "nodata_match": same_nodata},
+            "cpu_baseline": {"value": o_cells / o_s, "unit": "cells/s", "cores": threads,
+                             "kind": "port",
+                             "sample": f"{k} of {cols * rows} work packages ({o_cells} cells)"},
+            "accuracy_sampled": {"fixed64_bits_differ": bits, "fixed64_max_rel_vs_fp64": emax_fixed,
+                                 "fp32_max_rel_vs_fp64": emax_64},
+        }
+        print(json.dumps(out))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def load_traffic(kernel="k_tiles<0, 0>"):
     """HBM bytes per k_tiles launch on this workload, from the committed
     FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_single_4m.json)."""
@@ -277,9 +398,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-cells", type=int, default=0, help="cell-tile side (0 = auto)")
     ap.add_argument("--bucket-px", type=int, default=0, help="bucket side in pixels (0 = auto)")
-    ap.add_argument("--workload", choices=("single", "batch"), default="single",
+    ap.add_argument("--workload", choices=("single", "batch", "stream"), default="single",
                     help="single: the 4096^2 tile (default line, plus the batched and served "
-                         "comparisons); batch: only the batched 256-tile form")
+                         "comparisons); batch: only the batched 256-tile form; stream: the "
+                         "streamed-terrain driver pipeline (down-scaled)")
+    ap.add_argument("--stream-km", type=float, default=4.0, help="streamed terrain side (km)")
+    ap.add_argument("--stream-points", type=int, default=12_000_000)
+    ap.add_argument("--stream-wp-m", type=int, default=1000, help="work package side (m)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (built-in bucket and cell-tile sizes)")
@@ -301,6 +426,9 @@ def main():
 
     if args.workload == "batch":
         bench_batched_only(T, S, args)
+        return
+    if args.workload == "stream":
+        bench_stream(T, S, args)
         return
     t0 = time.perf_counter()
     tile = S.workload_single_4m(seed=1, n=args.points, grid=args.grid,

@@ -200,6 +200,7 @@ typedef struct tnni_serve_stats {
     uint64_t max_inflight_tiles;   /* most tiles on the device at once   */
     uint64_t max_inflight_batches; /* most batches on the device at once */
     uint64_t inflight_bytes;       /* footprint admitted right now       */
+    uint64_t budget_retries;       /* launches put back to wait for HBM  */
 } tnni_serve_stats;
 tnni_status tnni_serve_stats_read(tnni_ctx *ctx, tnni_serve_stats *out);
 

@@ -102,6 +102,23 @@ public:
         return NO_ERROR;
     }
 
+    // Up to *len items from the current position into mm_space; *len is set
+    // to the number read (TPIE's read_array).
+    err read_array(T *mm_space, TPIE_OS_SIZE_T *len)
+    {
+        if (!f_ || !mm_space || !len) return IO_ERROR;
+        flush();
+        const TPIE_OS_OFFSET want = std::min<TPIE_OS_OFFSET>((TPIE_OS_OFFSET)*len, length() - pos_);
+        *len = 0;
+        if (want <= 0) return END_OF_STREAM;
+        if (fseeko(f_, (off_t)pos_ * (off_t)sizeof(T), SEEK_SET) != 0) return IO_ERROR;
+        const size_t got = std::fread(mm_space, sizeof(T), (size_t)want, f_);
+        pos_ += (TPIE_OS_OFFSET)got;
+        *len = (TPIE_OS_SIZE_T)got;
+        rbuf_.clear();
+        return got == (size_t)want ? NO_ERROR : IO_ERROR;
+    }
+
     err write_item(const T &item)
     {
         if (!f_) return IO_ERROR;

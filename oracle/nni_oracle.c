@@ -273,9 +273,35 @@ void orc_slice_sites(const orc_frame *f, const int32_t *T, const uint8_t *keep, 
     }
 }
 
-void orc_heights_accumulate(const orc_desc *d, const orc_frame *f,
-                            const uint32_t *d2, const int32_t *label, const int32_t *Z,
-                            int64_t jj2, orc_acc *acc, uint64_t *pairs, int nthreads)
+/* One cell's divide (orc_heights' outputs, any may be NULL). */
+static void finish_cell(const orc_desc *d, const orc_acc *a, size_t o, float *h_fixed,
+                        double *h64, float *h_s0, uint32_t *count)
+{
+    const double sz = d->las_scale[2], oz = d->las_offset[2];
+    if (count) count[o] = a->cnt;
+    if (h_fixed)
+        h_fixed[o] = a->cnt ? (float)fma((double)a->sumZ / (double)a->cnt, sz, oz) : -9999.0f;
+    if (h64) h64[o] = a->cnt ? a->sum64 / (double)a->cnt : -9999.0;
+    if (h_s0) {
+        /* divide, cuda_nni.cu:497-500, then de-scale nni.cpp:841-846 */
+        int32_t res;
+        if (a->denom0 == 0)
+            res = (int32_t)((uint32_t)(-9999) * (uint32_t)d->multiplier);
+        else
+            res = (int32_t)a->numer0 / (int32_t)a->denom0;
+        double z = 1.0 * res / d->multiplier;
+        if (z != -9999) z = z - (-d->point_min) - 1;
+        h_s0[o] = (float)z;
+    }
+}
+
+/* The query pass over every cell, query-major.  acc (nullable): the cells'
+ * running sums, read and written back; without it each cell starts from
+ * zero and is divided at once into the outputs. */
+static void heights_cells(const orc_desc *d, const orc_frame *f, const uint32_t *d2,
+                          const int32_t *label, const int32_t *Z, int64_t jj2, orc_acc *acc,
+                          int want_s0, float *h_fixed, double *h64, float *h_s0, uint32_t *count,
+                          uint64_t *pairs, int nthreads)
 {
     const int alg = f->algorithm;
     const int64_t ps = f->ps_u;
@@ -294,7 +320,10 @@ void orc_heights_accumulate(const orc_desc *d, const orc_frame *f,
             int64_t k1 = imin64(f->W - 1, floordiv64(qx + f->Rqb - f->x0, ps));
             int64_t l0 = imax64(0, ceildiv64(qy - f->Rqb - f->y0, ps));
             int64_t l1 = imin64(f->H - 1, floordiv64(qy + f->Rqb - f->y0, ps));
-            orc_acc a = acc[(size_t)j * d->ncols + i];
+            const size_t o = (size_t)j * d->ncols + i;
+            orc_acc a;
+            if (acc) a = acc[o];
+            else memset(&a, 0, sizeof a);
             for (int64_t l = l0; l <= l1; ++l) {
                 int64_t dy = f->y0 + l * ps - qy;
                 for (int64_t k = k0; k <= k1; ++k) {
@@ -312,39 +341,31 @@ void orc_heights_accumulate(const orc_desc *d, const orc_frame *f,
                     a.cnt++;
                     a.sumZ += Z[s];
                     a.sum64 += fma((double)Z[s], sz, oz);
-                    uint32_t v = orc_s0_value(d, Z[s]);
-                    if (v > 0) { a.numer0 += v; a.denom0++; } /* cuda_nni.cu:423,447-448 */
+                    if (want_s0) {
+                        uint32_t v = orc_s0_value(d, Z[s]);
+                        if (v > 0) { a.numer0 += v; a.denom0++; } /* cuda_nni.cu:423,447-448 */
+                    }
                 }
             }
-            acc[(size_t)j * d->ncols + i] = a;
+            if (acc) acc[o] = a;
+            else finish_cell(d, &a, o, h_fixed, h64, h_s0, count);
         }
     }
     if (pairs) { pairs[0] += K; pairs[1] += S; }
 }
 
+void orc_heights_accumulate(const orc_desc *d, const orc_frame *f,
+                            const uint32_t *d2, const int32_t *label, const int32_t *Z,
+                            int64_t jj2, orc_acc *acc, uint64_t *pairs, int nthreads)
+{
+    heights_cells(d, f, d2, label, Z, jj2, acc, 1, NULL, NULL, NULL, NULL, pairs, nthreads);
+}
+
 void orc_heights_finish(const orc_desc *d, const orc_acc *acc, float *h_fixed, double *h64,
                         float *h_s0, uint32_t *count)
 {
-    const double sz = d->las_scale[2], oz = d->las_offset[2];
     const size_t C = (size_t)d->nrows * d->ncols;
-    for (size_t o = 0; o < C; ++o) {
-        const orc_acc a = acc[o];
-        if (count) count[o] = a.cnt;
-        if (h_fixed)
-            h_fixed[o] = a.cnt ? (float)fma((double)a.sumZ / (double)a.cnt, sz, oz) : -9999.0f;
-        if (h64) h64[o] = a.cnt ? a.sum64 / (double)a.cnt : -9999.0;
-        if (h_s0) {
-            /* divide, cuda_nni.cu:497-500, then de-scale nni.cpp:841-846 */
-            int32_t res;
-            if (a.denom0 == 0)
-                res = (int32_t)((uint32_t)(-9999) * (uint32_t)d->multiplier);
-            else
-                res = (int32_t)a.numer0 / (int32_t)a.denom0;
-            double z = 1.0 * res / d->multiplier;
-            if (z != -9999) z = z - (-d->point_min) - 1;
-            h_s0[o] = (float)z;
-        }
-    }
+    for (size_t o = 0; o < C; ++o) finish_cell(d, &acc[o], o, h_fixed, h64, h_s0, count);
 }
 
 void orc_heights(const orc_desc *d, const orc_frame *f,
@@ -352,10 +373,7 @@ void orc_heights(const orc_desc *d, const orc_frame *f,
                  float *h_fixed, double *h64, float *h_s0, uint32_t *count,
                  uint64_t *pairs, int nthreads)
 {
-    const size_t C = (size_t)d->nrows * d->ncols;
-    orc_acc *acc = (orc_acc *)calloc(C ? C : 1, sizeof(orc_acc));
     if (pairs) { pairs[0] = 0; pairs[1] = 0; }
-    orc_heights_accumulate(d, f, d2, label, Z, 0, acc, pairs, nthreads);
-    orc_heights_finish(d, acc, h_fixed, h64, h_s0, count);
-    free(acc);
+    heights_cells(d, f, d2, label, Z, 0, NULL, h_s0 != NULL, h_fixed, h64, h_s0, count, pairs,
+                  nthreads);
 }

@@ -79,7 +79,7 @@ class CtxOpts(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "submitted", "completed", "batches", "max_inflight_tiles", "max_inflight_batches",
-        "inflight_bytes")]
+        "inflight_bytes", "budget_retries")]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -508,15 +508,9 @@ struct Flight {
     uint64_t ticket = 0;
 };
 
-void run_batched(const Options &o, const Layout &L, const nni::Settings &s, const Meta &m,
-                 const std::vector<float> &slices, double &t_nni, double &t_out)
+void run_batched(tnni_ctx *ctx, const Options &o, const Layout &L, const nni::Settings &s,
+                 const Meta &m, const std::vector<float> &slices, double &t_nni, double &t_out)
 {
-    tnni_ctx_opts co;
-    std::memset(&co, 0, sizeof co);
-    if (const char *dev = std::getenv("TERRANNI_DEVICE")) co.device = std::atoi(dev);
-    co.n_slots = 3;
-    tnni_ctx *ctx = nullptr;
-    if (tnni_ctx_create(&co, &ctx) != TNNI_OK) die(std::string("engine: ") + tnni_last_error(nullptr));
     tnni_tile_desc base;
     std::memset(&base, 0, sizeof base);
     for (int k = 0; k < 3; ++k) {
@@ -547,32 +541,39 @@ void run_batched(const Options &o, const Layout &L, const nni::Settings &s, cons
                 std::unique_ptr<Flight> f(new Flight());
                 tpie::ami::stream<RawPoint> st(wp_file(o.wp_path, row, col, ".raw"), tpie::ami::READ_STREAM);
                 if (!st.is_valid()) die("missing work package " + wp_file(o.wp_path, row, col, ".raw"));
-                RawPoint *p;
+                // the whole package in one read; it is sorted by time, so a
+                // slice's points are one run of it
+                std::vector<RawPoint> all((size_t)st.stream_len());
+                TPIE_OS_SIZE_T got = all.size();
+                if (!all.empty() && st.read_array(all.data(), &got) != tpie::ami::NO_ERROR)
+                    die("cannot read " + wp_file(o.wp_path, row, col, ".raw"));
+                size_t b = 0, e = 0;
                 if (tr <= 0) {
-                    // the slice's points (nni.cpp:770 filter; the stream is
-                    // sorted by time, so they are one run of it)
-                    while (st.read_item(&p) == tpie::ami::NO_ERROR) {
-                        if ((double)std::fabs(t - (float)p->t) <= (double)tr + 0.000001) {
-                            f->X.push_back(p->X);
-                            f->Y.push_back(p->Y);
-                            f->Z.push_back(p->Z);
-                        } else if ((float)p->t > t + tr) {
-                            break;
-                        }
-                    }
+                    // the slice's points (nni.cpp:770 filter)
+                    while (b < all.size() && !((double)std::fabs(t - (float)all[b].t) <= (double)tr + 0.000001) &&
+                           !((float)all[b].t > t + tr))
+                        ++b;
+                    e = b;
+                    while (e < all.size() && (double)std::fabs(t - (float)all[e].t) <= (double)tr + 0.000001) ++e;
                 } else {
                     // space-time: every point within 2r + 1 of t; the engine
                     // keeps each slice's own
                     const double w = 2.0 * (double)tr + 1.0;
-                    while (st.read_item(&p) == tpie::ami::NO_ERROR) {
-                        const double dt = (double)p->t - (double)t;
-                        if (dt > w) break;
-                        if (dt < -w) continue;
-                        f->X.push_back(p->X);
-                        f->Y.push_back(p->Y);
-                        f->Z.push_back(p->Z);
-                        f->T.push_back(p->t);
-                    }
+                    while (b < all.size() && (double)all[b].t - (double)t < -w) ++b;
+                    e = b;
+                    while (e < all.size() && (double)all[e].t - (double)t <= w) ++e;
+                }
+                const size_t k = e - b;
+                f->X.resize(k);
+                f->Y.resize(k);
+                f->Z.resize(k);
+                if (tr > 0) f->T.resize(k);
+                for (size_t q = 0; q < k; ++q) {
+                    const RawPoint &p = all[b + q];
+                    f->X[q] = p.X;
+                    f->Y[q] = p.Y;
+                    f->Z[q] = p.Z;
+                    if (tr > 0) f->T[q] = p.t;
                 }
                 f->h.resize(cells);
                 tnni_tile_desc d = base;
@@ -622,7 +623,6 @@ void run_batched(const Options &o, const Layout &L, const nni::Settings &s, cons
         }
         nni_s += wall() - ts;
     }
-    tnni_ctx_destroy(ctx);
     t_nni = nni_s - out_s;
     t_out = out_s;
 }
@@ -822,7 +822,16 @@ int run(int argc, char **argv)
     const std::vector<float> slices = time_slices(o);
     double t_nni = 0, t_out = 0;
     if (batched) {
-        run_batched(o, L, s, m, slices, t_nni, t_out);
+        // the engine context is made before the timed part, like nni::init
+        // in the reference (terranni.cpp:694)
+        tnni_ctx_opts co;
+        std::memset(&co, 0, sizeof co);
+        if (const char *dev = std::getenv("TERRANNI_DEVICE")) co.device = std::atoi(dev);
+        co.n_slots = 3;
+        tnni_ctx *ctx = nullptr;
+        if (tnni_ctx_create(&co, &ctx) != TNNI_OK) die(std::string("engine: ") + tnni_last_error(nullptr));
+        run_batched(ctx, o, L, s, m, slices, t_nni, t_out);
+        tnni_ctx_destroy(ctx);
     } else {
         nni::init(s);
         if (nni::lastStatus() != 0) die(std::string("nni::init: ") + nni::lastError());

@@ -85,6 +85,7 @@ struct Request {
     PrepInfo info;
     uint64_t footprint = 0;
     double t_submit = 0, t_done = 0;
+    std::string err;  // why it failed
     tnni_status status = TNNI_PENDING;
     // host requests: pinned copies of the inputs (X|Y|Z) and of the outputs
     // (h|labels|d2), filled by tnni_submit and drained by tnni_wait
@@ -189,6 +190,10 @@ struct tnni_ctx {
     uint64_t next_id = 1;
     uint64_t serve_budget = 0;
     uint64_t inflight_bytes = 0;
+    // a launch found the HBM budget taken by batches in flight and cached
+    // buffers: admit nothing until a batch retires
+    bool budget_blocked = false;
+    bool budget_full_trim = false;  // the last retry ran with nothing in flight
     tnni_serve_stats stats{};
     PinnedPool pinned;
 };
@@ -586,6 +591,7 @@ void scheduler(tnni_ctx *ctx)
             s.reqs.clear();
             s.bytes = 0;
             s.busy = false;
+            ctx->budget_blocked = false;
             ctx->dcv.notify_all();
         }
         if (ctx->shutdown && !any_busy) break;
@@ -594,7 +600,7 @@ void scheduler(tnni_ctx *ctx)
         bool launched = false;
         for (auto &sp : ctx->slots) {
             Slot &s = *sp;
-            if (s.busy || ctx->pending.empty() || ctx->shutdown) continue;
+            if (s.busy || ctx->pending.empty() || ctx->shutdown || ctx->budget_blocked) continue;
             if (ctx->batch_window_us > 0 && !any_busy &&
                 (int)ctx->pending.size() < ctx->max_batch_tiles &&
                 now_s() - ctx->pending.front()->t_submit < ctx->batch_window_us * 1e-6)
@@ -620,8 +626,35 @@ void scheduler(tnni_ctx *ctx)
             lk.unlock();
             const tnni_status st = launch_batch(ctx, s);
             lk.lock();
+            bool others_busy = false;
+            for (auto &q : ctx->slots)
+                if (q.get() != &s && q->busy) others_busy = true;
+            if (st == TNNI_E_BUDGET && (others_busy || !ctx->budget_full_trim)) {
+                // The budget is held by batches in flight or by buffers cached
+                // from earlier, larger batches: drop this slot's and the idle
+                // slots' caches, put the requests back at the head of the
+                // queue, and admit again once a batch retires (or at once if
+                // none is in flight: everything is trimmed now).
+                (void)hipStreamSynchronize(s.a.stream);
+                arena_trim(ctx, s.a);
+                for (auto &q : ctx->slots)
+                    if (q.get() != &s && !q->busy) arena_trim(ctx, q->a);
+                for (auto it = s.reqs.rbegin(); it != s.reqs.rend(); ++it) ctx->pending.push_front(*it);
+                ctx->inflight_bytes -= s.bytes;
+                s.reqs.clear();
+                s.bytes = 0;
+                s.busy = false;
+                ctx->budget_full_trim = !others_busy;
+                ctx->budget_blocked = others_busy;
+                ctx->stats.budget_retries++;
+                continue;
+            }
+            if (st == TNNI_OK) ctx->budget_full_trim = false;
             if (st != TNNI_OK) {
-                for (auto &r : s.reqs) finish_request(ctx, r, st);
+                for (auto &r : s.reqs) {
+                    r->err = ctx->err;
+                    finish_request(ctx, r, st);
+                }
                 ctx->inflight_bytes -= s.bytes;
                 s.reqs.clear();
                 s.bytes = 0;
@@ -1159,7 +1192,7 @@ tnni_status tnni_wait(tnni_ctx *ctx, uint64_t ticket, double *latency_s)
         r->t_done = std::max(r->t_done, now_s());  // results in the caller's buffers
     }
     if (latency_s) *latency_s = r->t_done - r->t_submit;
-    if (r->status != TNNI_OK) ctx->err = "request failed";
+    if (r->status != TNNI_OK) ctx->err = "request failed" + (r->err.empty() ? std::string() : ": " + r->err);
     return r->status;
 }
 

@@ -296,3 +296,85 @@ def las_scene(seed: int, n: int, width: float, height: float, origin=(500000.0, 
         m = (x >= edges[k]) & (x < edges[k + 1])
         out.append((X[m], Y[m], Z[m], np.zeros(int(m.sum()), np.uint16)))
     return out
+
+
+# ---- streamed terrain (fBm) --------------------------------------------------
+
+def _hash01(ix, iy, octave, seed):
+    """Deterministic value in [0, 1) per lattice node (SplitMix64-style mix)."""
+    h = (ix.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+         ^ iy.astype(np.uint64) * np.uint64(0xC2B2AE3D27D4EB4F)
+         ^ np.uint64((octave * 0x165667B19E3779F9 + seed * 0x27D4EB2F165667C5) & 0xFFFFFFFFFFFFFFFF))
+    h ^= h >> np.uint64(30)
+    h *= np.uint64(0xBF58476D1CE4E5B9)
+    h ^= h >> np.uint64(27)
+    h *= np.uint64(0x94D049BB133111EB)
+    h ^= h >> np.uint64(31)
+    return (h >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def fbm_height(x, y, seed: int, hurst: float = 0.8, octaves: int = 12, base_m: float = 4096.0,
+               amp_m: float = 300.0):
+    """Fractional Brownian terrain evaluated point by point: octaves of
+    seeded value noise (smoothstep-interpolated lattice values), octave o at
+    wavelength base_m / 2^o and amplitude amp_m * 2^(-hurst * o)."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    z = np.zeros_like(x)
+    with np.errstate(over="ignore"):
+        for o in range(octaves):
+            f = (2.0 ** o) / base_m
+            u, v = x * f, y * f
+            ix, iy = np.floor(u).astype(np.int64), np.floor(v).astype(np.int64)
+            fx, fy = u - ix, v - iy
+            sx, sy = fx * fx * (3 - 2 * fx), fy * fy * (3 - 2 * fy)
+            a = _hash01(ix, iy, o, seed)
+            b = _hash01(ix + 1, iy, o, seed)
+            c = _hash01(ix, iy + 1, o, seed)
+            d = _hash01(ix + 1, iy + 1, o, seed)
+            n = a + (b - a) * sx + (c - a) * sy + (a - b - c + d) * sx * sy
+            z += amp_m * 2.0 ** (-hurst * o) * (2.0 * n - 1.0)
+    return z
+
+
+def streamed_terrain_las(out_dir: str, seed: int = 5, extent_m: float = 5000.0,
+                         n_points: int = 20_000_000, tile_m: float = 1000.0,
+                         overlap: float = 0.05, origin=(500000.0, 4200000.0), chunk: int = 4_000_000):
+    """The streamed-terrain workload: n_points uniform over an extent_m
+    square of fBm terrain (Hurst 0.8), cut into LAS tiles of tile_m with
+    `overlap` of the tile side shared with each neighbour (points there are
+    written to both files, exact duplicates).  Returns the list of files."""
+    import os
+    rng = np.random.default_rng(seed)
+    nt = int(np.ceil(extent_m / tile_m))
+    margin = overlap * tile_m
+    per = [[[] for _ in range(nt)] for _ in range(nt)]
+    left = n_points
+    while left > 0:
+        k = min(chunk, left)
+        left -= k
+        x = rng.uniform(0, extent_m, k)
+        y = rng.uniform(0, extent_m, k)
+        z = fbm_height(x, y, seed) + rng.normal(0.0, 0.05, k)
+        X = np.rint(x / LAS_SCALE[0]).astype(np.int32)
+        Y = np.rint(y / LAS_SCALE[1]).astype(np.int32)
+        Z = np.rint(z / LAS_SCALE[2]).astype(np.int32)
+        for r in range(nt):
+            for c in range(nt):
+                m = ((x >= c * tile_m - margin) & (x < (c + 1) * tile_m + margin) &
+                     (y >= r * tile_m - margin) & (y < (r + 1) * tile_m + margin))
+                if m.any():
+                    per[r][c].append((X[m], Y[m], Z[m]))
+    files = []
+    for r in range(nt):
+        for c in range(nt):
+            if not per[r][c]:
+                continue
+            X = np.concatenate([p[0] for p in per[r][c]])
+            Y = np.concatenate([p[1] for p in per[r][c]])
+            Z = np.concatenate([p[2] for p in per[r][c]])
+            path = os.path.join(out_dir, f"tile_r{r:03d}_c{c:03d}.las")
+            write_las(path, X, Y, Z, None, offset=(origin[0], origin[1], 0.0))
+            files.append(path)
+            per[r][c] = None
+    return files

@@ -47,6 +47,17 @@ int main(int argc, char **argv)
         CHECK(s.read_item(&p) == tpie::ami::NO_ERROR && p->key == 99);
         CHECK(s.read_item(&p) == tpie::ami::NO_ERROR && p->key == -1);
         CHECK(s.read_item(&p) == tpie::ami::NO_ERROR && p->key == 101);
+        // bulk read from a position, clipped at the end
+        std::vector<Item> buf(1000);
+        s.seek(49500);
+        TPIE_OS_SIZE_T len = buf.size();
+        CHECK(s.read_array(buf.data(), &len) == tpie::ami::NO_ERROR && len == 500);
+        CHECK(buf[0].key == 49500 && buf[499].key == 49999 && s.tell() == 50000);
+        s.seek(98);
+        len = 4;
+        CHECK(s.read_array(buf.data(), &len) == tpie::ami::NO_ERROR && len == 4);
+        CHECK(buf[0].key == 98 && buf[2].key == -1 && buf[3].key == 101);
+        CHECK(s.read_item(&p) == tpie::ami::NO_ERROR && p->key == 102);
     }
     // named streams persist; APPEND_STREAM starts at the end
     {
