@@ -161,6 +161,7 @@ def bench_served(T, tiles, rates, per_rate, resident: bool, seed=4, tune_table=N
         for p in [submit(k, k) for k in range(16)]:  # warm the slots and graphs
             p.wait()
         for rate in rates:
+            c.serve_stats_reset()
             picks = rng.integers(0, len(tiles), per_rate)
             gaps = rng.exponential(1.0 / rate, per_rate)
             pend, lat = [], []
@@ -376,7 +377,7 @@ def bench_stream(T, S, args):
         shutil.rmtree(work, ignore_errors=True)
 
 
-def load_traffic(kernel="k_tiles<0, 0>"):
+def load_traffic(kernel="k_tiles<0, 0, false>"):
     """HBM bytes per k_tiles launch on this workload, from the committed
     FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_single_4m.json)."""
     path = os.path.join(ROOT, "profiles", "traffic_single_4m.json")
@@ -386,6 +387,21 @@ def load_traffic(kernel="k_tiles<0, 0>"):
         d = json.load(f)
     k = d.get("kernels", {}).get(kernel)
     return None if k is None else k["bytes"]
+
+
+VALU_PEAK = 256 * 2.4e9  # wave64 VALU instructions per second: 1 per CU per cycle
+
+
+def load_valu(kernel="k_tiles<0, 0, false>"):
+    """VALU wave-instructions per k_tiles launch on this workload, from the
+    committed SQ_INSTS_VALU summary (profiles/sq_counters_single_4m.json)."""
+    path = os.path.join(ROOT, "profiles", "sq_counters_single_4m.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("SQ_INSTS_VALU")
 
 
 def main():
@@ -503,6 +519,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
                      "kernel": "k_tiles", "kernel_ms": k_ms, "alg_bytes": alg_bytes},
     }
+    valu = load_valu()
+    if valu:
+        # the bound that matters for this kernel: integer VALU issue
+        rate = valu / (k_ms * 1e-3)
+        out["roofline"]["valu"] = {"instructions": valu, "achieved_per_s": rate,
+                                   "peak_per_s": VALU_PEAK, "frac": rate / VALU_PEAK}
     for d in (dX, dY, dZ, dH):
         d.free()
     pool, table = None, None

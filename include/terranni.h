@@ -203,6 +203,8 @@ typedef struct tnni_serve_stats {
     uint64_t budget_retries;       /* launches put back to wait for HBM  */
 } tnni_serve_stats;
 tnni_status tnni_serve_stats_read(tnni_ctx *ctx, tnni_serve_stats *out);
+/* Restart the max_inflight_* high-water marks (the counts are kept). */
+tnni_status tnni_serve_stats_reset(tnni_ctx *ctx);
 
 /* ---- space-time (timeRadius > 0) ----------------------------------------
  * One work package at time cur_time with time radius r, as performNNI with

@@ -188,6 +188,7 @@ def lib():
     L.tnni_poll.argtypes = [P, ctypes.c_uint64]
     L.tnni_wait.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
     L.tnni_serve_stats_read.argtypes = [P, ctypes.POINTER(ServeStats)]
+    L.tnni_serve_stats_reset.argtypes = [P]
     L.tnni_shape_class.argtypes = [ctypes.POINTER(TileDesc), ctypes.c_uint64, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_uint64)]
     L.tnni_autotune.argtypes = [P, ctypes.c_int, ctypes.POINTER(TileDesc), P, P, P,
@@ -205,7 +206,7 @@ def lib():
                  "tnni_tile_run_dev", "tnni_batch_run_dev", "tnni_dev_alloc", "tnni_dev_free",
                  "tnni_h2d", "tnni_d2h", "tnni_d2d", "tnni_profile_enable", "tnni_profile_read",
                  "tnni_submit", "tnni_submit_dev", "tnni_poll", "tnni_wait",
-                 "tnni_serve_stats_read", "tnni_shape_class", "tnni_autotune",
+                 "tnni_serve_stats_read", "tnni_serve_stats_reset", "tnni_shape_class", "tnni_autotune",
                  "tnni_tune_read", "tnni_tune_load", "tnni_tune_clear", "tnni_tile_run_time",
                  "tnni_tile_run_time_dev"):
         getattr(L, name).restype = ctypes.c_int
@@ -441,6 +442,10 @@ class Context:
         s = ServeStats()
         self._chk(lib().tnni_serve_stats_read(self.h, ctypes.byref(s)))
         return s.as_dict()
+
+    def serve_stats_reset(self):
+        """Restart the in-flight high-water marks."""
+        self._chk(lib().tnni_serve_stats_reset(self.h))
 
     # ---- profiling ----
     def profile(self, enable: bool):

@@ -1205,6 +1205,15 @@ tnni_status tnni_serve_stats_read(tnni_ctx *ctx, tnni_serve_stats *out)
     return TNNI_OK;
 }
 
+tnni_status tnni_serve_stats_reset(tnni_ctx *ctx)
+{
+    if (!ctx) return fail(ctx, TNNI_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->smu);
+    ctx->stats.max_inflight_tiles = 0;
+    ctx->stats.max_inflight_batches = 0;
+    return TNNI_OK;
+}
+
 tnni_status tnni_profile_enable(tnni_ctx *ctx, int enable)
 {
     if (!ctx) return fail(ctx, TNNI_E_INVALID, "null argument");

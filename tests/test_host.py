@@ -22,7 +22,10 @@ def test_nni_api_library_exports_the_reference_functions():
                 "nni::setRenderbuffers(int, int, int)", "nni::clearRenderbuffers()",
                 "nni::setupRun(nni::WorkPackage const&, int, int)",
                 "nni::performNNI(nni::WorkPackage const&, tpie::ami::stream<nni::OutPoint>&, "
-                "tpie::ami::stream<nni::Point4D>&, float, float, bool, bool)"):
+                "tpie::ami::stream<nni::Point4D>&, float, float, bool, bool)",
+                # this engine's additions
+                "nni::setAccumulationMode(int)", "nni::setOrigin64(double, double)",
+                "nni::lastStatus()", "nni::lastError()"):
         assert sym in out, sym
 
 
