@@ -163,7 +163,8 @@ def bench_served(T, tiles, rates, per_rate, resident: bool, seed=4, tune_table=N
         for rate in rates:
             c.serve_stats_reset()
             picks = rng.integers(0, len(tiles), per_rate)
-            gaps = rng.exponential(1.0 / rate, per_rate)
+            # rate 0: saturation, every request submitted as soon as it can be
+            gaps = rng.exponential(1.0 / rate, per_rate) if rate > 0 else np.zeros(per_rate)
             pend, lat = [], []
             cv = threading.Condition()
             state = {"waited": 0}
@@ -425,7 +426,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (built-in bucket and cell-tile sizes)")
     ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--served-rates", default="50,200,500,1000,2000")
+    ap.add_argument("--served-rates", default="50,200,500,1000,2000,0",
+                    help="offered tiles/s (0: saturated, back-to-back submits)")
     ap.add_argument("--served-requests", type=int, default=300,
                     help="requests per arrival rate")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -538,8 +540,8 @@ def main():
             "workload": "Poisson arrivals of tiles drawn from the 256 uneven tiles",
             "hbm_resident": bench_served(T, pool, rates, args.served_requests, True,
                                          tune_table=table),
-            "host_buffers": bench_served(T, pool, rates[:2], args.served_requests, False,
-                                         tune_table=table),
+            "host_buffers": bench_served(T, pool, [r for r in rates if r > 0][:2] + [0.0],
+                                         args.served_requests, False, tune_table=table),
         }
     if not args.no_cpu:
         c_cells, threads, per, reps, ref = cpu_baseline(tile)

@@ -146,24 +146,26 @@ k_band_scatter(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
             cur[i] = hist[t.hist_base + (int64_t)i * t.nchunks + col];
         __syncthreads();
         for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kBlock * kPtsPerStep) {
-            int32_t X[kPtsPerStep], Y[kPtsPerStep], Z[kPtsPerStep];
+            int32_t X[kPtsPerStep], Y[kPtsPerStep], W[kPtsPerStep];
 #pragma unroll
             for (int q = 0; q < kPtsPerStep; ++q) {
                 const int64_t i = i0 + (int64_t)q * kBlock;
                 X[q] = i < ch.end ? t.X[i] : 0;
                 Y[q] = i < ch.end ? t.Y[i] : 0;
-                // space-time sites carry their time instead of their value
-                Z[q] = i < ch.end ? (t.T ? t.T[i] : t.Z[i]) : 0;
+                W[q] = i < ch.end ? (t.T ? t.T[i] : t.Z[i]) : 0;
             }
+            // a site record {index, x, y, w}: the index first (the low half
+            // of the search key); w the site's value, or for a space-time
+            // slice its squared time distance
 #pragma unroll
             for (int q = 0; q < kPtsPerStep; ++q) {
                 const int64_t i = i0 + (int64_t)q * kBlock;
                 int xu, yu;
                 uint32_t dt2 = 0;
                 if (i < ch.end && point_units(t, X[q], Y[q], xu, yu) &&
-                    (!t.T || slice_site(t, Z[q], dt2))) {
+                    (!t.T || slice_site(t, W[q], dt2))) {
                     const uint32_t pos = atomicAdd(&cur[point_band(t, yu)], 1u);
-                    banded[pos] = make_int4(xu, yu, (int)i, t.T ? (int)dt2 : site_value(t, Z[q]));
+                    banded[pos] = make_int4((int)i, xu, yu, t.T ? (int)dt2 : site_value(t, W[q]));
                 }
             }
         }
@@ -301,8 +303,8 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
         __syncthreads();
         for (uint32_t p = s0 + threadIdx.x; p < s1; p += kBlock) {
             const int4 rec = banded[p];
-            const int bx = div_pos(rec.x - t.gx0, t.bs, t.inv_bs);
-            const int by = div_pos(rec.y - t.gy0, t.bs, t.inv_bs);
+            const int bx = div_pos(rec.y - t.gx0, t.bs, t.inv_bs);
+            const int by = div_pos(rec.z - t.gy0, t.bs, t.inv_bs);
             atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
         }
         __syncthreads();
@@ -337,8 +339,8 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
         __syncthreads();
         for (uint32_t p = s0 + threadIdx.x; p < s1; p += kBlock) {
             const int4 rec = banded[p];
-            const int bx = div_pos(rec.x - t.gx0, t.bs, t.inv_bs);
-            const int by = div_pos(rec.y - t.gy0, t.bs, t.inv_bs);
+            const int bx = div_pos(rec.y - t.gx0, t.bs, t.inv_bs);
+            const int by = div_pos(rec.z - t.gy0, t.bs, t.inv_bs);
             const uint32_t pos = atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
             sorted[s0 + pos] = rec;
         }
@@ -386,18 +388,19 @@ k_edist_cols(const BatchDev *b, const TileDev *tiles, const uint8_t *__restrict_
 // pixel, so the region test is skipped.
 constexpr int kAllCover = 2;
 
-// TIME (space-time slice): st.w is the site's squared time distance, added
-// to the distance; the region test stays in the plane (nni.cpp:117-157,
-// 207-225), and the value comes from the label afterwards.
+// A record is {index, x, y, w}: w is the site's value, or for a space-time
+// slice (TIME) its squared time distance, added to the distance (the region
+// test stays in the plane, nni.cpp:117-157, 207-225; the winner's value is
+// then read through its index).
 template <int ALG, bool TIME>
 __device__ __forceinline__ void visit_site(const int4 st, int px, int py, int R, uint32_t R2,
                                            uint64_t &best, int &bval)
 {
-    const int dx = px - st.x, dy = py - st.y;
+    const int dx = px - st.y, dy = py - st.z;
     const uint32_t dxy = (uint32_t)__mul24(dx, dx) + (uint32_t)__mul24(dy, dy);
     const uint32_t dd = TIME ? dxy + (uint32_t)st.w : dxy;
     if (ALG == kAllCover) {
-        const uint64_t key = ((uint64_t)dd << 32) | (uint32_t)st.z;
+        const uint64_t key = ((uint64_t)dd << 32) | (uint32_t)st.x;
         const bool better = key < best;
         best = better ? key : best;
         if (!TIME) bval = better ? st.w : bval;
@@ -407,7 +410,7 @@ __device__ __forceinline__ void visit_site(const int4 st, int px, int py, int R,
     if (ALG == TNNI_ALG_CONES) cov = cov && dxy <= R2;
     // a site that does not cover the pixel gets the key of "no site"
     const uint32_t hi = cov ? dd : 0xFFFFFFFFu;
-    const uint32_t lo = cov ? (uint32_t)st.z : 0xFFFFFFFFu;
+    const uint32_t lo = cov ? (uint32_t)st.x : 0xFFFFFFFFu;
     const uint64_t key = ((uint64_t)hi << 32) | lo;
     const bool better = key < best;
     best = better ? key : best;
@@ -420,15 +423,18 @@ __device__ __forceinline__ void scan_range(uint32_t s, uint32_t e, const int4 *_
                                            int px, int py, int R, uint32_t R2,
                                            uint64_t &best, int &bval)
 {
-    uint32_t p = s;
-    // two records in flight per step
-    for (; p + 1 < e; p += 2) {
-        const int4 a = sites[p];
-        const int4 b = sites[p + 1];
+    // two records in flight per step; the loop runs on the record pointer
+    const int4 *q = sites + s;
+    const int4 *const qe = sites + e;
+    if (q >= qe) return;
+    const int4 *const qlast = qe - 1;
+    for (; q < qlast; q += 2) {
+        const int4 a = q[0];
+        const int4 b = q[1];
         visit_site<ALG, TIME>(a, px, py, R, R2, best, bval);
         visit_site<ALG, TIME>(b, px, py, R, R2, best, bval);
     }
-    if (p < e) visit_site<ALG, TIME>(sites[p], px, py, R, R2, best, bval);
+    if (q < qe) visit_site<ALG, TIME>(*q, px, py, R, R2, best, bval);
 }
 
 // Ring r of the bucket square around (bx, by), from the global arrays.
@@ -727,6 +733,7 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
         label_block<ALG, TIME>(t, offs, edist, sorted, ws, kA, lA, kB, lB, k, l, active, best, bval);
         if (!active) continue;
         // a space-time label's value: the winning point's height
+        // a space-time winner's value, through its index
         if (TIME) bval = best == ~0ull ? 0 : site_value(t, t.Z[(uint32_t)best]);
         const uint32_t d2 = (uint32_t)(best >> 32);
         s_d2[(l - g.rl0) * RW + (k - g.rk0)] = d2;
