@@ -710,6 +710,7 @@ struct Acc {
 
 template <int ALG, int MODE, bool TIME>
 __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, uint32_t *smem,
+                                              uint32_t *s_next,
                                               const uint32_t *__restrict__ offs,
                                               const uint8_t *__restrict__ edist,
                                               const int4 *__restrict__ sorted)
@@ -721,9 +722,11 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
     uint32_t *s_d2 = smem + kWaves * (sizeof(WaveStage) / 4);
     int32_t *s_val = reinterpret_cast<int32_t *>(s_d2 + RW * RH);
 
-    // 1. discrete Voronoi of the block's pixels, 8x8 pixels per wave
+    // 1. discrete Voronoi of the block's pixels, 8x8 pixels per wave; the
+    //    waves take blocks as they finish (an LDS counter), so one wave on
+    //    dense blocks does not hold the others at the barrier below
     const int nbx8 = (RW + 7) >> 3, nby8 = (RH + 7) >> 3;
-    for (int blk = wave; blk < nbx8 * nby8; blk += kWaves) {
+    for (int blk = wave; blk < nbx8 * nby8;) {
         const int kA = g.rk0 + (blk % nbx8) * 8, lA = g.rl0 + (blk / nbx8) * 8;
         const int kB = min(kA + 7, g.rk1), lB = min(lA + 7, g.rl1);
         const int k = kA + (lane & 7), l = lA + (lane >> 3);
@@ -731,18 +734,21 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
         uint64_t best;
         int bval;
         label_block<ALG, TIME>(t, offs, edist, sorted, ws, kA, lA, kB, lB, k, l, active, best, bval);
-        if (!active) continue;
-        // a space-time label's value: the winning point's height
-        // a space-time winner's value, through its index
-        if (TIME) bval = best == ~0ull ? 0 : site_value(t, t.Z[(uint32_t)best]);
-        const uint32_t d2 = (uint32_t)(best >> 32);
-        s_d2[(l - g.rl0) * RW + (k - g.rk0)] = d2;
-        s_val[(l - g.rl0) * RW + (k - g.rk0)] = bval;
-        if (k >= g.ok0 && k <= g.ok1 && l >= g.ol0 && l <= g.ol1) {
-            const int64_t o = (int64_t)l * t.W + k;
-            if (t.out_labels) t.out_labels[o] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
-            if (t.out_d2) t.out_d2[o] = d2;
+        if (active) {
+            // a space-time winner's value, through its index
+            if (TIME) bval = best == ~0ull ? 0 : site_value(t, t.Z[(uint32_t)best]);
+            const uint32_t d2 = (uint32_t)(best >> 32);
+            s_d2[(l - g.rl0) * RW + (k - g.rk0)] = d2;
+            s_val[(l - g.rl0) * RW + (k - g.rk0)] = bval;
+            if (k >= g.ok0 && k <= g.ok1 && l >= g.ol0 && l <= g.ol1) {
+                const int64_t o = (int64_t)l * t.W + k;
+                if (t.out_labels) t.out_labels[o] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
+                if (t.out_d2) t.out_d2[o] = d2;
+            }
         }
+        uint32_t nxt = 0;
+        if (lane == 0) nxt = atomicAdd(s_next, 1u);
+        blk = __shfl((int)nxt, 0, 64);
     }
     __syncthreads();
 
@@ -750,8 +756,18 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
     //    |x - q|^2 <= d2(x); accumulate the label's value, then divide.
     const int qa = query_lo(t), qb = query_hi(t);
     const int ncx = g.i1 - g.i0, ncy = g.j1 - g.j0;
+    // cell c = (ci, cj) of the block, stepped by kBlock cells without a
+    // division per cell
+    const int step_j = kBlock / ncx, step_i = kBlock - step_j * ncx;
+    int ci = (int)threadIdx.x % ncx, cj = (int)threadIdx.x / ncx;
     for (int c = threadIdx.x; c < ncx * ncy; c += kBlock) {
-        const int i = g.i0 + c % ncx, j = g.j0 + c / ncx;
+        const int i = g.i0 + ci, j = g.j0 + cj;
+        ci += step_i;
+        cj += step_j;
+        if (ci >= ncx) {
+            ci -= ncx;
+            ++cj;
+        }
         const int ma = max(qa, -i * t.scale), mb = min(qb, t.W - 1 - i * t.scale);
         const int na = max(qa, -j * t.scale), nb = min(qb, t.H - 1 - j * t.scale);
         Acc<MODE> acc;
@@ -783,10 +799,14 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
 {
     extern __shared__ __align__(16) uint32_t smem[];
     __shared__ uint32_t s_item;
+    __shared__ uint32_t s_next;  // next label block of the current item
     const int64_t n_items = b->n_items;
     const int n_tiles = b->n_tiles;
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(&b->work_counter, 1u);
+        if (threadIdx.x == 0) {
+            s_item = atomicAdd(&b->work_counter, 1u);
+            s_next = kWaves;  // blocks 0..3 go to waves 0..3
+        }
         __syncthreads();
         const int64_t item = (int64_t)__builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
@@ -802,7 +822,7 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
         const TileDev t = tiles[lo];
         const int64_t local = item - t.item_base;
         const int tj = (int)(local / t.ntx), ti = (int)(local % t.ntx);
-        process_block<ALG, MODE, TIME>(t, ti, tj, smem, offs, edist, sorted);
+        process_block<ALG, MODE, TIME>(t, ti, tj, smem, &s_next, offs, edist, sorted);
         __syncthreads();
     }
 }
