@@ -115,7 +115,7 @@ uint64_t shape_class(const tnni_tile_desc &d, uint64_t n, const tnni_frame_info 
 }
 
 // Set the cell-tile size of t to tc and return the LDS its items need: the
-// per-wave stages plus the label tile of the widest item (the first and
+// per-wave stages plus the label tile (and search list) of the widest item (the first and
 // last items of a row also own the raster border, interior ones are alike).
 size_t tile_lds_bytes(TileDev &t, int tc)
 {
@@ -133,7 +133,8 @@ size_t tile_lds_bytes(TileDev &t, int tc)
         const Region g = region_of(t, 0, tj);
         rh = std::max(rh, g.rl1 - g.rl0 + 1);
     }
-    return (size_t)rw * rh * 8 + tile_kernel_stage_bytes();
+    // per pixel: d2 and value (4 + 4 bytes), a list slot (2 bytes)
+    return ((size_t)rw * rh * 10 + 3) / 4 * 4 + tile_kernel_stage_bytes();
 }
 
 // Useful share of the label lanes of an interior item of tc cells: it labels

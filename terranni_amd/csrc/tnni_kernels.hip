@@ -470,6 +470,32 @@ __device__ __forceinline__ void ring_global(const TileDev &t, const uint32_t *__
     }
 }
 
+// The full square of buckets within R of (bx, by), from the global arrays:
+// one contiguous range per bucket row.  Used where every bucket closer than
+// ring R - 1 is empty, so that the square holds exactly rings R-1 and R.
+template <int ALG, bool TIME>
+__device__ __forceinline__ void square_global(const TileDev &t, const uint32_t *__restrict__ offs,
+                                              const int4 *__restrict__ sorted, int bx, int by, int R,
+                                              int px, int py, int Rr, uint64_t &best, int &bval)
+{
+    const uint32_t *base = offs + t.bk_base;
+    const int c0 = max(bx - R, 0), c1 = min(bx + R, t.nbx - 1);
+    const int r0 = max(by - R, 0), r1 = min(by + R, t.nby - 1);
+    // four rows' bucket offsets in flight at a time, then their ranges
+    for (int row = r0; row <= r1; row += 4) {
+        uint32_t a[4], e[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t *rs = base + (int64_t)min(row + q, r1) * t.nbx;
+            a[q] = row + q <= r1 ? rs[c0] : 0u;
+            e[q] = row + q <= r1 ? rs[c1 + 1] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            scan_range<ALG, TIME>(a[q], e[q], sorted, px, py, Rr, t.Rs2, best, bval);
+    }
+}
+
 // k / bm for a pixel index 0 <= k < 2^20: multiply by ceil(2^32 / bm)
 // (exact since k * (ceil(2^32/bm) * bm - 2^32) < 2^32); bm == 1 has no
 // 32-bit multiplier and is taken as is.
@@ -545,11 +571,12 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
                                             const uint8_t *__restrict__ edist,
                                             const int4 *__restrict__ sorted, WaveStage &ws,
                                             int kA, int lA, int kB, int lB, int k, int l,
-                                            bool active, uint64_t &best, int &bval)
+                                            bool active, uint64_t &best, int &bval, bool &pending)
 {
     const int lane = threadIdx.x & 63;
     best = ~0ull;
     bval = 0;
+    pending = false;
     const int bs = t.bs;
     const int bx = t.bK + div_bm(t, k), by = t.bK + div_bm(t, l);
     const int r0 = active ? (int)edist[t.bk_base + (int64_t)by * t.nbx + bx] : 1 << 20;
@@ -615,15 +642,55 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
             if (gap > t.Rsb) break;
             if ((uint64_t)gap * (uint64_t)gap > (best >> 32)) break;
         }
-        if (staged && r <= 1) {
+        if (r >= 2) {
+            // rings 2 and beyond: the pixel joins its item's list and is
+            // finished later with the other such pixels, on full waves
+            pending = true;
+            break;
+        }
+        if (staged) {
             // the sites of rings 0 and 1 lie within 2*bs (L-inf) of the pixel
             if (t.ring1_covered)
                 ring_staged<kAllCover, TIME>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
             else
                 ring_staged<ALG, TIME>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
-        }
-        else
+        } else {
             ring_global<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, best, bval);
+        }
+    }
+}
+
+// The rest of a pixel's search from ring max(r0, 2) on, from the global
+// arrays: the pixels of an item that get there are gathered first, so that
+// these long searches (sparse stretches) run side by side on full waves
+// instead of one or two lanes of a wave each.  best / bval hold what rings
+// 0-1 found.
+template <int ALG, bool TIME>
+__device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *__restrict__ offs,
+                                             const uint8_t *__restrict__ edist,
+                                             const int4 *__restrict__ sorted, int k, int l,
+                                             uint64_t &best, int &bval)
+{
+    const int bs = t.bs;
+    const int bx = t.bK + div_bm(t, k), by = t.bK + div_bm(t, l);
+    const int r0 = (int)edist[t.bk_base + (int64_t)by * t.nbx + bx];
+    const int px = t.x0 + k * t.ps_u, py = t.y0 + l * t.ps_u;
+    const int R = (ALG == TNNI_ALG_PLANES) ? t.Rs : t.Rsb;
+    const int el = px - (t.gx0 + bx * bs) + 1, er = t.gx0 + (bx + 1) * bs - px;
+    const int eb = py - (t.gy0 + by * bs) + 1, et = t.gy0 + (by + 1) * bs - py;
+    const int e0 = min(min(el, er), min(eb, et));
+    for (int r = max(r0, 2);; ++r) {
+        const int gap = e0 + (r - 1) * bs;
+        if (gap > t.Rsb) break;
+        if ((uint64_t)gap * (uint64_t)gap > (best >> 32)) break;
+        if (r == r0) {
+            // a sparse stretch: every ring inside r0 is empty, so rings r0
+            // and r0 + 1 are the square of radius r0 + 1, one range per row
+            square_global<ALG, TIME>(t, offs, sorted, bx, by, r + 1, px, py, R, best, bval);
+            ++r;
+        } else {
+            ring_global<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, best, bval);
+        }
     }
 }
 
@@ -710,7 +777,7 @@ struct Acc {
 
 template <int ALG, int MODE, bool TIME>
 __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, uint32_t *smem,
-                                              uint32_t *s_next,
+                                              uint32_t *s_next, uint32_t *s_npend,
                                               const uint32_t *__restrict__ offs,
                                               const uint8_t *__restrict__ edist,
                                               const int4 *__restrict__ sorted)
@@ -721,6 +788,7 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
     WaveStage &ws = reinterpret_cast<WaveStage *>(smem)[wave];
     uint32_t *s_d2 = smem + kWaves * (sizeof(WaveStage) / 4);
     int32_t *s_val = reinterpret_cast<int32_t *>(s_d2 + RW * RH);
+    uint16_t *s_pend = reinterpret_cast<uint16_t *>(s_val + RW * RH);  // pixels still searching
 
     // 1. discrete Voronoi of the block's pixels, 8x8 pixels per wave; the
     //    waves take blocks as they finish (an LDS counter), so one wave on
@@ -733,8 +801,24 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
         const bool active = k <= kB && l <= lB;
         uint64_t best;
         int bval;
-        label_block<ALG, TIME>(t, offs, edist, sorted, ws, kA, lA, kB, lB, k, l, active, best, bval);
-        if (active) {
+        bool pending;
+        label_block<ALG, TIME>(t, offs, edist, sorted, ws, kA, lA, kB, lB, k, l, active, best, bval,
+                               pending);
+        // the pixels that still search: one list slot each, one LDS atomic
+        // per wave
+        const uint64_t pm = __ballot(active && pending);
+        uint32_t base = 0;
+        if (pm) {
+            if (lane == 0) base = atomicAdd(s_npend, (uint32_t)__popcll(pm));
+            base = (uint32_t)__shfl((int)base, 0, 64);
+        }
+        if (active && pending) {
+            // what rings 0-1 found: d2 and, until the search ends, the index
+            const int p = (l - g.rl0) * RW + (k - g.rk0);
+            s_d2[p] = (uint32_t)(best >> 32);
+            s_val[p] = (int32_t)(uint32_t)best;
+            s_pend[base + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull))] = (uint16_t)p;
+        } else if (active) {
             // a space-time winner's value, through its index
             if (TIME) bval = best == ~0ull ? 0 : site_value(t, t.Z[(uint32_t)best]);
             const uint32_t d2 = (uint32_t)(best >> 32);
@@ -749,6 +833,26 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
         uint32_t nxt = 0;
         if (lane == 0) nxt = atomicAdd(s_next, 1u);
         blk = __shfl((int)nxt, 0, 64);
+    }
+    __syncthreads();
+
+    // 1b. the pixels whose search goes past ring 1, one per thread
+    const uint32_t npend = *s_npend;
+    for (uint32_t q = threadIdx.x; q < npend; q += kBlock) {
+        const int p = s_pend[q];
+        const int k = g.rk0 + p % RW, l = g.rl0 + p / RW;
+        uint64_t best = ((uint64_t)s_d2[p] << 32) | (uint32_t)s_val[p];
+        int bval = (!TIME && best != ~0ull) ? site_value(t, t.Z[(uint32_t)best]) : 0;
+        finish_pixel<ALG, TIME>(t, offs, edist, sorted, k, l, best, bval);
+        if (TIME) bval = best == ~0ull ? 0 : site_value(t, t.Z[(uint32_t)best]);
+        const uint32_t d2 = (uint32_t)(best >> 32);
+        s_d2[p] = d2;
+        s_val[p] = bval;
+        if (k >= g.ok0 && k <= g.ok1 && l >= g.ol0 && l <= g.ol1) {
+            const int64_t o = (int64_t)l * t.W + k;
+            if (t.out_labels) t.out_labels[o] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
+            if (t.out_d2) t.out_d2[o] = d2;
+        }
     }
     __syncthreads();
 
@@ -799,13 +903,15 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
 {
     extern __shared__ __align__(16) uint32_t smem[];
     __shared__ uint32_t s_item;
-    __shared__ uint32_t s_next;  // next label block of the current item
+    __shared__ uint32_t s_next;   // next label block of the current item
+    __shared__ uint32_t s_npend;  // its pixels still searching past ring 1
     const int64_t n_items = b->n_items;
     const int n_tiles = b->n_tiles;
     for (;;) {
         if (threadIdx.x == 0) {
             s_item = atomicAdd(&b->work_counter, 1u);
             s_next = kWaves;  // blocks 0..3 go to waves 0..3
+            s_npend = 0u;
         }
         __syncthreads();
         const int64_t item = (int64_t)__builtin_amdgcn_readfirstlane(s_item);
@@ -822,7 +928,7 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
         const TileDev t = tiles[lo];
         const int64_t local = item - t.item_base;
         const int tj = (int)(local / t.ntx), ti = (int)(local % t.ntx);
-        process_block<ALG, MODE, TIME>(t, ti, tj, smem, &s_next, offs, edist, sorted);
+        process_block<ALG, MODE, TIME>(t, ti, tj, smem, &s_next, &s_npend, offs, edist, sorted);
         __syncthreads();
     }
 }
