@@ -312,7 +312,7 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     const int alg = tiles[0].t.algorithm, mode = tiles[0].t.mode;
     const bool time = tiles[0].t.T != nullptr;
     if (time && how == Launch::Graph) return fail(ctx, TNNI_E_INVALID, "space-time tiles run direct");
-    int64_t pts = 0, bks = 0, items = 0, rows = 0, bands = 0, hents = 0;
+    int64_t pts = 0, bks = 0, ebytes = 0, items = 0, rows = 0, bands = 0, hents = 0;
     size_t lds = 0;
     int wg_cap = 0;       // tuned workgroups per CU of the tile kernel
     bool wg_any = false;  // some tile has no tuned cap
@@ -323,6 +323,9 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
             return fail(ctx, TNNI_E_INVALID, "a batch must share algorithm, accum_mode and time mode");
         t.pt_base = pts;
         t.bk_base = bks;
+        t.nbx4 = (t.nbx + 3) & ~3;
+        t.e_base = ebytes;
+        ebytes += (int64_t)t.nbx4 * t.nby;
         t.item_base = items;
         t.row_base = rows;
         t.band_base = bands;
@@ -359,8 +362,8 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     if ((st = ensure(ctx, a.banded, (size_t)pts * 16)) != TNNI_OK) return st;
     if ((st = ensure(ctx, a.sorted, (size_t)pts * 16)) != TNNI_OK) return st;
     if ((st = ensure(ctx, a.offs, (size_t)(bks + 1) * 4)) != TNNI_OK) return st;
-    if ((st = ensure(ctx, a.hdist, (size_t)bks)) != TNNI_OK) return st;
-    if ((st = ensure(ctx, a.edist, (size_t)bks)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.hdist, (size_t)ebytes)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.edist, (size_t)ebytes)) != TNNI_OK) return st;
 
     BatchDev bd;
     std::memset(&bd, 0, sizeof bd);
@@ -469,7 +472,11 @@ double tile_cost(const TileDev &t) { return (double)t.W * t.H + 8.0 * (double)t.
 uint64_t request_bytes(const TileDev &t, bool labels, bool d2)
 {
     const uint64_t cells = (uint64_t)t.nrows * t.ncols, pix = (uint64_t)t.W * t.H;
-    uint64_t b = (uint64_t)t.n * (12 + 16 + 16) + ((uint64_t)t.nbx * t.nby + 1) * 6 + cells * 4;
+    // points: staging, band order, bucket order; buckets: offsets and the
+    // two distance maps (rows padded to 4 bytes); cells: heights
+    const uint64_t nbx4 = ((uint64_t)t.nbx + 3) & ~3ull;
+    uint64_t b = (uint64_t)t.n * (12 + 16 + 16) + ((uint64_t)t.nbx * t.nby + 1) * 4 +
+                 nbx4 * t.nby * 2 + cells * 4;
     if (labels) b += pix * 4;
     if (d2) b += pix * 4;
     return b + sizeof(TileDev) + ((uint64_t)t.n / kChunkPts + 1) * sizeof(Chunk);

@@ -48,6 +48,9 @@ struct TileDev {
     int pad1;
     double inv_bs;              // 1 / bs
     int64_t bk_base;            // first bucket of this tile in the batch
+    int64_t e_base;             // first byte of its distance maps (4-aligned)
+    int nbx4;                   // bytes per row of the distance maps (nbx rounded up to 4)
+    int pad3;
     int64_t row_base;           // first bucket row of this tile in the batch
     int br, nbands;             // bucket rows per band, bands
     double inv_br;              // 1 / br
@@ -160,6 +163,7 @@ struct DevBuffers {
     uint32_t *offs;     // first site of every bucket, + 1 sentinel
     uint8_t *hdist;     // per bucket: distance to a non-empty bucket in its row
     uint8_t *edist;     // per bucket: Chebyshev distance to a non-empty bucket
+                        // (both: rows of nbx4 bytes from the tile's e_base)
 };
 
 // Enqueue the whole pipeline on `stream`.  ev_main0/ev_main1 (nullable)

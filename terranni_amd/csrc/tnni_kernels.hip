@@ -311,14 +311,14 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
         // row distances to a bucket holding a site, capped at ecap
         const int cap = t.ecap;
         for (int i = threadIdx.x; i < nloc; i += kBlock) {
-            const int x = i % nbx;
+            const int rr = i / nbx, x = i - rr * nbx;
             int d = cap;
             for (int q = 0; q < cap; ++q)
                 if ((x - q >= 0 && cnt[i - q]) || (x + q < nbx && cnt[i + q])) {
                     d = q;
                     break;
                 }
-            hdist[obase + i] = (uint8_t)d;
+            hdist[t.e_base + (int64_t)(r0 + rr) * t.nbx4 + x] = (uint8_t)d;
         }
         __syncthreads();
         // exclusive scan of the counters: each thread a contiguous segment
@@ -358,22 +358,29 @@ k_edist_cols(const BatchDev *b, const TileDev *tiles, const uint8_t *__restrict_
         const TileDev &t = tiles[tile_of_row(tiles, n_tiles, item)];
         const int row = (int)(item - t.row_base);
         const int nbx = t.nbx, nby = t.nby;
-        const uint8_t *h = hdist + t.bk_base;
-        for (int x = threadIdx.x; x < nbx; x += kBlock) {
-            int e = t.ecap;
-            for (int q0 = 0; q0 < e; q0 += 4) {
-                int up[4], dn[4];
+        // rows of nbx4 bytes from a 4-aligned base: four columns per thread
+        // with aligned 4-byte loads and stores (the padding columns compute
+        // values nothing reads)
+        const int nbx4 = t.nbx4;
+        const uint8_t *h = hdist + t.e_base;
+        const int cap = t.ecap;
+        (void)nbx;
+        for (int g = threadIdx.x; g < nbx4 / 4; g += kBlock) {
+            const int x = 4 * g;
+            int e[4] = {cap, cap, cap, cap};
+            for (int q = 0; q < cap; ++q) {
+                const int a = row - q, c = row + q;
+                const uint32_t up = a >= 0 ? *(const uint32_t *)(h + (int64_t)a * nbx4 + x) : 0xFFFFFFFFu;
+                const uint32_t dn = c < nby ? *(const uint32_t *)(h + (int64_t)c * nbx4 + x) : 0xFFFFFFFFu;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int a = row - q0 - q, c = row + q0 + q;
-                    up[q] = a >= 0 ? (int)h[(int64_t)a * nbx + x] : 255;
-                    dn[q] = c < nby ? (int)h[(int64_t)c * nbx + x] : 255;
+                for (int k = 0; k < 4; ++k) {
+                    const int v = min((int)((up >> (8 * k)) & 0xFFu), (int)((dn >> (8 * k)) & 0xFFu));
+                    e[k] = min(e[k], max(q, v));
                 }
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    e = min(e, max(q0 + q, min(up[q], dn[q])));
+                if (max(max(e[0], e[1]), max(e[2], e[3])) <= q) break;
             }
-            edist[t.bk_base + (int64_t)row * nbx + x] = (uint8_t)e;
+            *(uint32_t *)(edist + t.e_base + (int64_t)row * nbx4 + x) =
+                (uint32_t)e[0] | ((uint32_t)e[1] << 8) | ((uint32_t)e[2] << 16) | ((uint32_t)e[3] << 24);
         }
     }
 }
@@ -579,7 +586,7 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
     pending = false;
     const int bs = t.bs;
     const int bx = t.bK + div_bm(t, k), by = t.bK + div_bm(t, l);
-    const int r0 = active ? (int)edist[t.bk_base + (int64_t)by * t.nbx + bx] : 1 << 20;
+    const int r0 = active ? (int)edist[t.e_base + (int64_t)by * t.nbx4 + bx] : 1 << 20;
 
     // the window: the block's buckets and one more on each side
     const int cx0 = max(t.bK + div_bm(t, kA) - 1, 0), cx1 = min(t.bK + div_bm(t, kB) + 1, t.nbx - 1);
@@ -673,7 +680,7 @@ __device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *_
 {
     const int bs = t.bs;
     const int bx = t.bK + div_bm(t, k), by = t.bK + div_bm(t, l);
-    const int r0 = (int)edist[t.bk_base + (int64_t)by * t.nbx + bx];
+    const int r0 = (int)edist[t.e_base + (int64_t)by * t.nbx4 + bx];
     const int px = t.x0 + k * t.ps_u, py = t.y0 + l * t.ps_u;
     const int R = (ALG == TNNI_ALG_PLANES) ? t.Rs : t.Rsb;
     const int el = px - (t.gx0 + bx * bs) + 1, er = t.gx0 + (bx + 1) * bs - px;
