@@ -301,11 +301,21 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
         const int64_t obase = t.bk_base + (int64_t)r0 * nbx;
         for (int i = threadIdx.x; i < nloc; i += kBlock) cnt[i] = 0u;
         __syncthreads();
-        for (uint32_t p = s0 + threadIdx.x; p < s1; p += kBlock) {
-            const int4 rec = banded[p];
-            const int bx = div_pos(rec.y - t.gx0, t.bs, t.inv_bs);
-            const int by = div_pos(rec.z - t.gy0, t.bs, t.inv_bs);
-            atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
+        // kPtsPerStep records in flight per thread
+        for (uint32_t p0 = s0 + threadIdx.x; p0 < s1; p0 += kBlock * kPtsPerStep) {
+            int4 rec[kPtsPerStep];
+#pragma unroll
+            for (int q = 0; q < kPtsPerStep; ++q) {
+                const uint32_t p = p0 + (uint32_t)q * kBlock;
+                rec[q] = p < s1 ? banded[p] : make_int4(0, t.gx0, t.gy0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < kPtsPerStep; ++q) {
+                if (p0 + (uint32_t)q * kBlock >= s1) break;
+                const int bx = div_pos(rec[q].y - t.gx0, t.bs, t.inv_bs);
+                const int by = div_pos(rec[q].z - t.gy0, t.bs, t.inv_bs);
+                atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
+            }
         }
         __syncthreads();
         // row distances to a bucket holding a site, capped at ecap
@@ -337,12 +347,21 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
         for (int i = threadIdx.x; i < nloc; i += kBlock) offs[obase + i] = s0 + cnt[i];
         if (r1 == t.nby && threadIdx.x == 0) offs[t.bk_base + (int64_t)t.nby * nbx] = s1;
         __syncthreads();
-        for (uint32_t p = s0 + threadIdx.x; p < s1; p += kBlock) {
-            const int4 rec = banded[p];
-            const int bx = div_pos(rec.y - t.gx0, t.bs, t.inv_bs);
-            const int by = div_pos(rec.z - t.gy0, t.bs, t.inv_bs);
-            const uint32_t pos = atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
-            sorted[s0 + pos] = rec;
+        for (uint32_t p0 = s0 + threadIdx.x; p0 < s1; p0 += kBlock * kPtsPerStep) {
+            int4 rec[kPtsPerStep];
+#pragma unroll
+            for (int q = 0; q < kPtsPerStep; ++q) {
+                const uint32_t p = p0 + (uint32_t)q * kBlock;
+                rec[q] = p < s1 ? banded[p] : make_int4(0, t.gx0, t.gy0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < kPtsPerStep; ++q) {
+                if (p0 + (uint32_t)q * kBlock >= s1) break;
+                const int bx = div_pos(rec[q].y - t.gx0, t.bs, t.inv_bs);
+                const int by = div_pos(rec[q].z - t.gy0, t.bs, t.inv_bs);
+                const uint32_t pos = atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
+                sorted[s0 + pos] = rec[q];
+            }
         }
         __syncthreads();
     }
