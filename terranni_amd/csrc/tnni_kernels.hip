@@ -705,19 +705,21 @@ __device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *_
     const int el = px - (t.gx0 + bx * bs) + 1, er = t.gx0 + (bx + 1) * bs - px;
     const int eb = py - (t.gy0 + by * bs) + 1, et = t.gy0 + (by + 1) * bs - py;
     const int e0 = min(min(el, er), min(eb, et));
-    for (int r = max(r0, 2);; ++r) {
-        const int gap = e0 + (r - 1) * bs;
-        if (gap > t.Rsb) break;
-        if ((uint64_t)gap * (uint64_t)gap > (best >> 32)) break;
-        if (r == r0) {
-            // a sparse stretch: every ring inside r0 is empty, so rings r0
-            // and r0 + 1 are the square of radius r0 + 1, one range per row
-            square_global<ALG, TIME>(t, offs, sorted, bx, by, r + 1, px, py, R, best, bval);
-            ++r;
-        } else {
-            ring_global<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, best, bval);
-        }
+    // the last ring whose sites could still beat the best so far or lie in
+    // the site region: the ring loop of label_block would stop before the
+    // first ring past it
+    const uint64_t bd = best >> 32;
+    int rl = max(r0, 2) - 1;
+    for (;;) {
+        const int gap = e0 + rl * bs;  // of ring rl + 1
+        if (gap > t.Rsb || (uint64_t)gap * (uint64_t)gap > bd) break;
+        ++rl;
     }
+    // rings max(r0, 2) .. rl as the one square of radius rl (one range per
+    // bucket row, the row offsets loaded four rows at a time): the rings
+    // inside it are empty (below r0) or already visited (0 and 1), and a
+    // site seen twice changes no minimum
+    if (rl >= max(r0, 2)) square_global<ALG, TIME>(t, offs, sorted, bx, by, rl, px, py, R, best, bval);
 }
 
 template <int MODE>
