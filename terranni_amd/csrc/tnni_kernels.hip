@@ -463,42 +463,8 @@ __device__ __forceinline__ void scan_range(uint32_t s, uint32_t e, const int4 *_
     if (q < qe) visit_site<ALG, TIME>(*q, px, py, R, R2, best, bval);
 }
 
-// Ring r of the bucket square around (bx, by), from the global arrays.
-template <int ALG, bool TIME>
-__device__ __forceinline__ void ring_global(const TileDev &t, const uint32_t *__restrict__ offs,
-                                            const int4 *__restrict__ sorted, int bx, int by, int r,
-                                            int px, int py, int R, uint64_t &best, int &bval)
-{
-    const uint32_t *base = offs + t.bk_base;
-    for (int dy = -r; dy <= r; ++dy) {
-        const int row = by + dy;
-        if (row < 0 || row >= t.nby) continue;
-        const uint32_t *rs = base + (int64_t)row * t.nbx;
-        uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
-        if (dy == -r || dy == r) {
-            const int c0 = max(bx - r, 0), c1 = min(bx + r, t.nbx - 1);
-            if (c0 <= c1) {
-                a0 = rs[c0];
-                a1 = rs[c1 + 1];
-            }
-        } else {
-            if (bx - r >= 0) {
-                a0 = rs[bx - r];
-                a1 = rs[bx - r + 1];
-            }
-            if (bx + r < t.nbx) {
-                b0 = rs[bx + r];
-                b1 = rs[bx + r + 1];
-            }
-        }
-        scan_range<ALG, TIME>(a0, a1, sorted, px, py, R, t.Rs2, best, bval);
-        scan_range<ALG, TIME>(b0, b1, sorted, px, py, R, t.Rs2, best, bval);
-    }
-}
-
 // The full square of buckets within R of (bx, by), from the global arrays:
-// one contiguous range per bucket row.  Used where every bucket closer than
-// ring R - 1 is empty, so that the square holds exactly rings R-1 and R.
+// one contiguous range per bucket row, the row offsets four rows at a time.
 template <int ALG, bool TIME>
 __device__ __forceinline__ void square_global(const TileDev &t, const uint32_t *__restrict__ offs,
                                               const int4 *__restrict__ sorted, int bx, int by, int R,
@@ -550,48 +516,30 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Ring r (0 or 1) of the bucket square around (bx, by), from the stage.
+// The 3x3 square of buckets around (bx, by) (rings 0 and 1), from the stage.
 template <int ALG, bool TIME>
-__device__ __forceinline__ void ring_staged(const TileDev &t, const WaveStage &ws, int cx0, int cy0,
-                                            int wp, int bx, int by, int r, int px, int py, int R,
-                                            uint64_t &best, int &bval)
+__device__ __forceinline__ void square1_staged(const TileDev &t, const WaveStage &ws, int cx0, int cy0,
+                                               int wp, int bx, int by, int px, int py, int R,
+                                               uint64_t &best, int &bval)
 {
-    for (int dy = -r; dy <= r; ++dy) {
-        const int row = by + dy;
-        if (row < 0 || row >= t.nby) continue;
+    const int c0 = max(bx - 1, 0) - cx0, c1 = min(bx + 1, t.nbx - 1) + 1 - cx0;
+    for (int row = max(by - 1, 0); row <= min(by + 1, t.nby - 1); ++row) {
         const int j = row - cy0;
         const uint32_t *wr = ws.woff + j * wp;
         const uint32_t rel = ws.rowpre[j] - wr[0];  // staged slot = offset + rel
-        uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
-        if (dy == -r || dy == r) {
-            const int c0 = max(bx - r, 0), c1 = min(bx + r, t.nbx - 1);
-            if (c0 <= c1) {
-                a0 = wr[c0 - cx0] + rel;
-                a1 = wr[c1 + 1 - cx0] + rel;
-            }
-        } else {
-            if (bx - r >= 0) {
-                a0 = wr[bx - r - cx0] + rel;
-                a1 = wr[bx - r + 1 - cx0] + rel;
-            }
-            if (bx + r < t.nbx) {
-                b0 = wr[bx + r - cx0] + rel;
-                b1 = wr[bx + r + 1 - cx0] + rel;
-            }
-        }
-        scan_range<ALG, TIME>(a0, a1, ws.sites, px, py, R, t.Rs2, best, bval);
-        scan_range<ALG, TIME>(b0, b1, ws.sites, px, py, R, t.Rs2, best, bval);
+        scan_range<ALG, TIME>(wr[c0] + rel, wr[c1] + rel, ws.sites, px, py, R, t.Rs2, best, bval);
     }
 }
 
 // Nearest covering site of every pixel of one 8x8 block (one wave, one
 // pixel per lane; inactive lanes still take part in the staging).  Each
-// lane runs a ring search over the bucket grid: rings closer than its
-// bucket's empty distance hold no site and are skipped; rings 0 and 1 come
-// from the wave's LDS stage when the window fits it; the search stops when
-// every unvisited site is provably farther than the best so far (strictly,
-// so an equal distance still reaches the lower index) or outside the site
-// region.
+// lane searches the bucket grid in rings around its pixel's bucket: rings
+// closer than the bucket's empty distance r0 hold no site; rings 0 and 1
+// are visited as one 3x3 square, from the wave's LDS stage when the window
+// fits it; the search ends when every unvisited site is provably farther
+// than the best so far (strictly, so an equal distance still reaches the
+// lower index) or outside the site region, and otherwise goes on later
+// (pending) from ring max(r0, 2).
 template <int ALG, bool TIME>
 __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__restrict__ offs,
                                             const uint8_t *__restrict__ edist,
@@ -661,29 +609,26 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
     const int el = px - (t.gx0 + bx * bs) + 1, er = t.gx0 + (bx + 1) * bs - px;
     const int eb = py - (t.gy0 + by * bs) + 1, et = t.gy0 + (by + 1) * bs - py;
     const int e0 = min(min(el, er), min(eb, et));
-    for (int r = r0;; ++r) {
-        if (r > 0) {
-            // every site outside the visited (2r-1)^2 square is at least gap away
-            const int gap = e0 + (r - 1) * bs;
-            if (gap > t.Rsb) break;
-            if ((uint64_t)gap * (uint64_t)gap > (best >> 32)) break;
-        }
-        if (r >= 2) {
-            // rings 2 and beyond: the pixel joins its item's list and is
-            // finished later with the other such pixels, on full waves
-            pending = true;
-            break;
-        }
+    if (r0 <= 1) {
+        // rings 0 and 1 as one 3x3 square of buckets (three row ranges).
+        // The ring loop would skip ring 1 only when e0 > Rsb (no site of it
+        // covers the pixel) or e0^2 > d2 after ring 0 (none of it is nearer),
+        // so visiting it anyway leaves the minimum as it is.
         if (staged) {
-            // the sites of rings 0 and 1 lie within 2*bs (L-inf) of the pixel
             if (t.ring1_covered)
-                ring_staged<kAllCover, TIME>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
+                square1_staged<kAllCover, TIME>(t, ws, cx0, cy0, wp, bx, by, px, py, R, best, bval);
             else
-                ring_staged<ALG, TIME>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
+                square1_staged<ALG, TIME>(t, ws, cx0, cy0, wp, bx, by, px, py, R, best, bval);
         } else {
-            ring_global<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, best, bval);
+            square_global<ALG, TIME>(t, offs, sorted, bx, by, 1, px, py, R, best, bval);
         }
     }
+    // every site not yet visited lies in ring max(r0, 2) or beyond, at least
+    // gap away: if one may still cover the pixel and be nearer, the pixel
+    // joins its item's list and is finished later with the other such
+    // pixels, on full waves
+    const int gap = e0 + (max(r0, 2) - 1) * bs;
+    pending = gap <= t.Rsb && (uint64_t)gap * (uint64_t)gap <= (best >> 32);
 }
 
 // The rest of a pixel's search from ring max(r0, 2) on, from the global
