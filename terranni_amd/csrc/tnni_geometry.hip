@@ -122,17 +122,28 @@ size_t tile_lds_bytes(TileDev &t, int tc)
     t.tc = tc;
     t.ntx = (t.ncols + tc - 1) / tc;
     t.nty = (t.nrows + tc - 1) / tc;
-    int rw = 0, rh = 0;
-    const int xs[3] = {0, std::min(1, t.ntx - 1), t.ntx - 1};
-    const int ys[3] = {0, std::min(1, t.nty - 1), t.nty - 1};
-    for (int ti : xs) {
+    // an item's pixel width depends on its column only, its height on its
+    // row only.  A query halo wider than tc cells clips items other than
+    // the first at the raster edge, so every item within the halo's reach
+    // of either edge is measured; the full items past them (neither first
+    // nor last, halo inside the raster) all share the width of item `near`.
+    const int near = (std::abs(query_lo(t)) + std::abs(query_hi(t))) / std::max(1, tc * t.scale) + 2;
+    auto widest = [near](int n, auto extent) {
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            if (i > near && i < n - 1 - near) i = n - 1 - near;
+            m = std::max(m, extent(i));
+        }
+        return m;
+    };
+    const int rw = widest(t.ntx, [&](int ti) {
         const Region g = region_of(t, ti, 0);
-        rw = std::max(rw, g.rk1 - g.rk0 + 1);
-    }
-    for (int tj : ys) {
+        return g.rk1 - g.rk0 + 1;
+    });
+    const int rh = widest(t.nty, [&](int tj) {
         const Region g = region_of(t, 0, tj);
-        rh = std::max(rh, g.rl1 - g.rl0 + 1);
-    }
+        return g.rl1 - g.rl0 + 1;
+    });
     // per pixel: d2 and value (4 + 4 bytes), a list slot (2 bytes)
     return ((size_t)rw * rh * 10 + 3) / 4 * 4 + tile_kernel_stage_bytes();
 }

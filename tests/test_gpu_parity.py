@@ -54,11 +54,16 @@ def test_bucket_size_does_not_change_results(ctx, bucket):
     check_tile(ctx, p, t.X, t.Y, t.Z)
 
 
-@pytest.mark.parametrize("tc", [1, 3, 16, 48])
-def test_cell_tile_size_does_not_change_results(ctx, tc):
-    _say("tile cells", tc)
-    t = S.make_tile(22, 4000, 70, cell_size=2.0, scale=1, kind="uniform")
-    p = dataclasses.replace(t.params, tile_cells=tc)
+@pytest.mark.parametrize("tc,scale,po", [(1, 1, -1), (3, 1, -1), (16, 1, -1), (48, 1, -1),
+                                         (1, 3, 0), (2, 3, 0), (5, 3, -1)])
+def test_cell_tile_size_does_not_change_results(ctx, tc, scale, po):
+    """Scale 3 without a border: the query halo of a 1- or 2-cell item
+    (pixels -4..6 of its cell) reaches past its neighbour, so the items at
+    the raster edge are clipped unevenly and the widest item (11 pixels at
+    tc 1) is neither of the first two nor the last (10, 7, 7)."""
+    _say("tile cells", tc, "scale", scale, "pixel offset", po)
+    t = S.make_tile(22, 4000, 70, cell_size=2.0, scale=scale, kind="uniform")
+    p = dataclasses.replace(t.params, tile_cells=tc, pixel_offset=po)
     check_tile(ctx, p, t.X, t.Y, t.Z)
 
 
