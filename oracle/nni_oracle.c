@@ -80,6 +80,7 @@ int orc_frame_compute(const orc_desc *d, orc_frame *f)
         int64_t Rqb = d->algorithm == 0 ? Rq : isqrt64(Rq2);
         /* squared distances inside a region must fit in 32 bits */
         if (d->algorithm == 0 ? (2 * Rs * Rs >= (int64_t)NOD2) : (Rs2 >= (int64_t)NOD2)) continue;
+        if (2 * Rqb * Rqb >= (int64_t)NOD2) continue; /* query offsets square into 32 bits */
         int64_t ext = (int64_t)(f->W > f->H ? f->W : f->H) * ps_u + 2 * Rsb + 2 * Rqb;
         if (ext >= ((int64_t)1 << 30)) continue;
         f->upm = upm;

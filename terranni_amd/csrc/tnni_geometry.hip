@@ -77,6 +77,8 @@ tnni_status compute_frame(const tnni_tile_desc &d, tnni_frame_info &f, std::stri
         const int64_t Rqb = d.algorithm == TNNI_ALG_PLANES ? Rq : isqrt64(Rq2);
         if (d.algorithm == TNNI_ALG_PLANES ? (2 * Rs * Rs >= (int64_t)kNoD2) : (Rs2 >= (int64_t)kNoD2))
             continue;
+        // the query offsets (|dx|, |dy| <= Rqb) square into 32 bits as well
+        if (2 * Rqb * Rqb >= (int64_t)kNoD2) continue;
         const int64_t ext = std::max(W, H) * ps_u + 2 * Rsb + 2 * Rqb;
         if (ext >= ((int64_t)1 << 30)) continue;
         f.upm = upm;

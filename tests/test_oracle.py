@@ -350,6 +350,13 @@ def test_host_errors_without_gpu():
     with pytest.raises(T.TnniError) as e:
         T.frame(P(cell_size=0.123456))  # no exact fixed-point grid
     assert e.value.status == -2
+    # a query offset of 1 km (70710 units) does not square into 32 bits
+    with pytest.raises(T.TnniError) as e:
+        T.frame(P(query_radius=1000.0))
+    assert e.value.status == -2
+    with pytest.raises(ValueError):
+        O.frame(P(query_radius=1000.0))
+    assert T.frame(P(query_radius=300.0))["Rqb"] == O.frame(P(query_radius=300.0))["Rqb"] == 21213
     assert T.footprint(P(), 1000) > 12000
 
 
