@@ -117,8 +117,7 @@ uint64_t shape_class(const tnni_tile_desc &d, uint64_t n, const tnni_frame_info 
 }
 
 // Set the cell-tile size of t to tc and return the LDS its items need: the
-// per-wave stages plus the label tile (and search list) of the widest item (the first and
-// last items of a row also own the raster border, interior ones are alike).
+// per-wave stages plus the label tile (and search list) of the widest item.
 size_t tile_lds_bytes(TileDev &t, int tc)
 {
     t.tc = tc;
@@ -304,6 +303,11 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
     // A site of ring 0 or 1 around a pixel's bucket is within 2*bs of the
     // pixel on both axes (strictly less than 2*bs).
     t.ring1_covered = d.algorithm == TNNI_ALG_PLANES ? (2 * bs <= f.Rs) : (8 * bs * bs <= f.Rs2);
+    // More than kDenseSites sites per bucket at the tile's mean density: a
+    // pixel's own bucket often settles its search, so the searches go ring
+    // by ring and stop early; otherwise ring 1 is nearly always needed and
+    // the searches visit whole squares of buckets (fewer, longer ranges).
+    t.dense = (double)n * (double)bm * (double)bm > kDenseSites * (double)f.W * (double)f.H ? 1 : 0;
     // cell tiles: the most lane-efficient size that fits the LDS budget
     // of four workgroups per CU; the plain form avoids sizes that leave a
     // lone small tile with fewer items than twice the CUs

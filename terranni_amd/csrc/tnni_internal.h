@@ -23,6 +23,7 @@ constexpr int kScanItems = 4;          // items per thread per scan step
 constexpr int kMaxBands = 8192;        // bucket-row bands per tile (LDS histogram)
 constexpr int kBandBuckets = 8192;     // buckets per band (LDS counters)
 constexpr int kBandPts = 16384;        // points per band aimed at (mean density)
+constexpr double kDenseSites = 2.0;    // sites per bucket above which searches go ring by ring
 
 // Everything a kernel needs to know about one tile of a batch.  Lengths are
 // in the tile-local fixed-point unit (1/upm metre).
@@ -45,7 +46,7 @@ struct TileDev {
     int ecap;                   // cap of the empty-distance map
     uint32_t bm_magic;          // ceil(2^32 / bm), 0 for bm == 1
     int ring1_covered;          // every site of rings 0-1 covers the pixel
-    int pad1;
+    int dense;                  // buckets hold several sites: ring-by-ring searches
     double inv_bs;              // 1 / bs
     int64_t bk_base;            // first bucket of this tile in the batch
     int64_t e_base;             // first byte of its distance maps (4-aligned)

@@ -463,8 +463,41 @@ __device__ __forceinline__ void scan_range(uint32_t s, uint32_t e, const int4 *_
     if (q < qe) visit_site<ALG, TIME>(*q, px, py, R, R2, best, bval);
 }
 
-// The full square of buckets within R of (bx, by), from the global arrays:
-// one contiguous range per bucket row, the row offsets four rows at a time.
+// Ring r of the bucket square around (bx, by), from the global arrays.
+template <int ALG, bool TIME>
+__device__ __forceinline__ void ring_global(const TileDev &t, const uint32_t *__restrict__ offs,
+                                            const int4 *__restrict__ sorted, int bx, int by, int r,
+                                            int px, int py, int R, uint64_t &best, int &bval)
+{
+    const uint32_t *base = offs + t.bk_base;
+    for (int dy = -r; dy <= r; ++dy) {
+        const int row = by + dy;
+        if (row < 0 || row >= t.nby) continue;
+        const uint32_t *rs = base + (int64_t)row * t.nbx;
+        uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+        if (dy == -r || dy == r) {
+            const int c0 = max(bx - r, 0), c1 = min(bx + r, t.nbx - 1);
+            if (c0 <= c1) {
+                a0 = rs[c0];
+                a1 = rs[c1 + 1];
+            }
+        } else {
+            if (bx - r >= 0) {
+                a0 = rs[bx - r];
+                a1 = rs[bx - r + 1];
+            }
+            if (bx + r < t.nbx) {
+                b0 = rs[bx + r];
+                b1 = rs[bx + r + 1];
+            }
+        }
+        scan_range<ALG, TIME>(a0, a1, sorted, px, py, R, t.Rs2, best, bval);
+        scan_range<ALG, TIME>(b0, b1, sorted, px, py, R, t.Rs2, best, bval);
+    }
+}
+
+// The square of buckets within R of (bx, by), from the global arrays: one
+// contiguous range per bucket row, the row offsets four rows at a time.
 template <int ALG, bool TIME>
 __device__ __forceinline__ void square_global(const TileDev &t, const uint32_t *__restrict__ offs,
                                               const int4 *__restrict__ sorted, int bx, int by, int R,
@@ -473,7 +506,6 @@ __device__ __forceinline__ void square_global(const TileDev &t, const uint32_t *
     const uint32_t *base = offs + t.bk_base;
     const int c0 = max(bx - R, 0), c1 = min(bx + R, t.nbx - 1);
     const int r0 = max(by - R, 0), r1 = min(by + R, t.nby - 1);
-    // four rows' bucket offsets in flight at a time, then their ranges
     for (int row = r0; row <= r1; row += 4) {
         uint32_t a[4], e[4];
 #pragma unroll
@@ -516,14 +548,49 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The 3x3 square of buckets around (bx, by) (rings 0 and 1), from the stage.
+// Ring r (0 or 1) of the bucket square around (bx, by), from the stage.
 template <int ALG, bool TIME>
-__device__ __forceinline__ void square1_staged(const TileDev &t, const WaveStage &ws, int cx0, int cy0,
-                                               int wp, int bx, int by, int px, int py, int R,
-                                               uint64_t &best, int &bval)
+__device__ __forceinline__ void ring_staged(const TileDev &t, const WaveStage &ws, int cx0, int cy0,
+                                            int wp, int bx, int by, int r, int px, int py, int R,
+                                            uint64_t &best, int &bval)
 {
-    const int c0 = max(bx - 1, 0) - cx0, c1 = min(bx + 1, t.nbx - 1) + 1 - cx0;
-    for (int row = max(by - 1, 0); row <= min(by + 1, t.nby - 1); ++row) {
+    for (int dy = -r; dy <= r; ++dy) {
+        const int row = by + dy;
+        if (row < 0 || row >= t.nby) continue;
+        const int j = row - cy0;
+        const uint32_t *wr = ws.woff + j * wp;
+        const uint32_t rel = ws.rowpre[j] - wr[0];  // staged slot = offset + rel
+        uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+        if (dy == -r || dy == r) {
+            const int c0 = max(bx - r, 0), c1 = min(bx + r, t.nbx - 1);
+            if (c0 <= c1) {
+                a0 = wr[c0 - cx0] + rel;
+                a1 = wr[c1 + 1 - cx0] + rel;
+            }
+        } else {
+            if (bx - r >= 0) {
+                a0 = wr[bx - r - cx0] + rel;
+                a1 = wr[bx - r + 1 - cx0] + rel;
+            }
+            if (bx + r < t.nbx) {
+                b0 = wr[bx + r - cx0] + rel;
+                b1 = wr[bx + r + 1 - cx0] + rel;
+            }
+        }
+        scan_range<ALG, TIME>(a0, a1, ws.sites, px, py, R, t.Rs2, best, bval);
+        scan_range<ALG, TIME>(b0, b1, ws.sites, px, py, R, t.Rs2, best, bval);
+    }
+}
+
+// The square of buckets of radius r (0: the pixel's bucket, 1: the 3x3
+// square of rings 0 and 1) around (bx, by), from the stage.
+template <int ALG, bool TIME>
+__device__ __forceinline__ void square_staged(const TileDev &t, const WaveStage &ws, int cx0, int cy0,
+                                              int wp, int bx, int by, int r, int px, int py, int R,
+                                              uint64_t &best, int &bval)
+{
+    const int c0 = max(bx - r, 0) - cx0, c1 = min(bx + r, t.nbx - 1) + 1 - cx0;
+    for (int row = max(by - r, 0); row <= min(by + r, t.nby - 1); ++row) {
         const int j = row - cy0;
         const uint32_t *wr = ws.woff + j * wp;
         const uint32_t rel = ws.rowpre[j] - wr[0];  // staged slot = offset + rel
@@ -609,19 +676,24 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
     const int el = px - (t.gx0 + bx * bs) + 1, er = t.gx0 + (bx + 1) * bs - px;
     const int eb = py - (t.gy0 + by * bs) + 1, et = t.gy0 + (by + 1) * bs - py;
     const int e0 = min(min(el, er), min(eb, et));
-    if (r0 <= 1) {
+    const bool cover = t.ring1_covered;
+    if (t.dense) {
+        // ring by ring: ring 1 only where it may still hold a covering site
+        // nearer than the best (gap e0)
+        for (int r = r0; r <= 1; ++r) {
+            if (r > 0 && (e0 > t.Rsb || (uint64_t)e0 * (uint64_t)e0 > (best >> 32))) break;
+            if (!staged) ring_global<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, best, bval);
+            else if (cover) ring_staged<kAllCover, TIME>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
+            else ring_staged<ALG, TIME>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
+        }
+    } else if (r0 <= 1) {
         // rings 0 and 1 as one 3x3 square of buckets (three row ranges).
         // The ring loop would skip ring 1 only when e0 > Rsb (no site of it
         // covers the pixel) or e0^2 > d2 after ring 0 (none of it is nearer),
         // so visiting it anyway leaves the minimum as it is.
-        if (staged) {
-            if (t.ring1_covered)
-                square1_staged<kAllCover, TIME>(t, ws, cx0, cy0, wp, bx, by, px, py, R, best, bval);
-            else
-                square1_staged<ALG, TIME>(t, ws, cx0, cy0, wp, bx, by, px, py, R, best, bval);
-        } else {
-            square_global<ALG, TIME>(t, offs, sorted, bx, by, 1, px, py, R, best, bval);
-        }
+        if (!staged) square_global<ALG, TIME>(t, offs, sorted, bx, by, 1, px, py, R, best, bval);
+        else if (cover) square_staged<kAllCover, TIME>(t, ws, cx0, cy0, wp, bx, by, 1, px, py, R, best, bval);
+        else square_staged<ALG, TIME>(t, ws, cx0, cy0, wp, bx, by, 1, px, py, R, best, bval);
     }
     // every site not yet visited lies in ring max(r0, 2) or beyond, at least
     // gap away: if one may still cover the pixel and be nearer, the pixel
@@ -650,9 +722,25 @@ __device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *_
     const int el = px - (t.gx0 + bx * bs) + 1, er = t.gx0 + (bx + 1) * bs - px;
     const int eb = py - (t.gy0 + by * bs) + 1, et = t.gy0 + (by + 1) * bs - py;
     const int e0 = min(min(el, er), min(eb, et));
+    if (t.dense) {
+        // ring by ring from max(r0, 2), stopping as soon as the gap passes
+        // the best so far; a sparse stretch (r0 >= 2) opens with rings r0
+        // and r0 + 1 as one square (every ring inside r0 is empty)
+        for (int r = max(r0, 2);; ++r) {
+            const int gap = e0 + (r - 1) * bs;
+            if (gap > t.Rsb || (uint64_t)gap * (uint64_t)gap > (best >> 32)) break;
+            if (r == r0) {
+                square_global<ALG, TIME>(t, offs, sorted, bx, by, r + 1, px, py, R, best, bval);
+                ++r;
+            } else {
+                ring_global<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, best, bval);
+            }
+        }
+        return;
+    }
     // the last ring whose sites could still beat the best so far or lie in
-    // the site region: the ring loop of label_block would stop before the
-    // first ring past it
+    // the site region: the ring loop above would stop before the first ring
+    // past it
     const uint64_t bd = best >> 32;
     int rl = max(r0, 2) - 1;
     for (;;) {
@@ -662,8 +750,8 @@ __device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *_
     }
     // rings max(r0, 2) .. rl as the one square of radius rl (one range per
     // bucket row, the row offsets loaded four rows at a time): the rings
-    // inside it are empty (below r0) or already visited (0 and 1), and a
-    // site seen twice changes no minimum
+    // inside it are empty (below r0) or already visited (0 and 1, a few
+    // sites at this density), and a site seen twice changes no minimum
     if (rl >= max(r0, 2)) square_global<ALG, TIME>(t, offs, sorted, bx, by, rl, px, py, R, best, bval);
 }
 

@@ -67,6 +67,17 @@ def test_cell_tile_size_does_not_change_results(ctx, tc, scale, po):
     check_tile(ctx, p, t.X, t.Y, t.Z)
 
 
+@pytest.mark.parametrize("kind,bucket", [("clustered", 0), ("uniform", 0), ("uniform", 3)])
+def test_dense_tile_searches_ring_by_ring(ctx, kind, bucket):
+    """About 38 points a pixel: even 1-pixel buckets hold dozens of sites, so
+    the searches go ring by ring (the pixel's own bucket usually settles
+    them) instead of over whole squares of buckets; same labels either way."""
+    _say("dense", kind, "bucket", bucket)
+    t = S.make_tile(31, 200000, 64, cell_size=0.5, scale=1, kind=kind)
+    p = dataclasses.replace(t.params, bucket_px=bucket)
+    check_tile(ctx, p, t.X, t.Y, t.Z)
+
+
 def test_empty_tile(ctx):
     _say("empty tile")
     t = S.make_tile(1, 10, 16)
