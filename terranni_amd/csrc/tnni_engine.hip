@@ -479,7 +479,9 @@ uint64_t request_bytes(const TileDev &t, bool labels, bool d2)
                  nbx4 * t.nby * 2 + cells * 4;
     if (labels) b += pix * 4;
     if (d2) b += pix * 4;
-    return b + sizeof(TileDev) + ((uint64_t)t.n / kChunkPts + 1) * sizeof(Chunk);
+    const uint64_t chunks = (uint64_t)t.n / kChunkPts + 1;
+    // the band histograms: one entry per (band, chunk)
+    return b + sizeof(TileDev) + chunks * sizeof(Chunk) + ((uint64_t)t.nbands * chunks + 1) * 4;
 }
 
 void finish_request(tnni_ctx *ctx, const std::shared_ptr<Request> &r, tnni_status st)

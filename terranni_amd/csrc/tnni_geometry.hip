@@ -290,7 +290,9 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
     // one band-sort workgroup) and, at the tile's mean density, about
     // kBandPts points, so that dense tiles spread over many workgroups
     const int64_t per_row = std::max<int64_t>(1, (int64_t)n / nby);
-    const int64_t br_pts = std::max<int64_t>(1, kBandPts / per_row);
+    // rows enough to keep the tile within kMaxBands bands
+    const int64_t br_min = (nby + kMaxBands - 1) / kMaxBands;
+    const int64_t br_pts = std::max<int64_t>(br_min, kBandPts / per_row);
     t.br = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(nby, kBandBuckets / nbx), br_pts));
     t.nbands = (int)((nby + t.br - 1) / t.br);
     t.inv_br = 1.0 / (double)t.br;

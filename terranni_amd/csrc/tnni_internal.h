@@ -16,13 +16,15 @@ namespace tnni {
 
 constexpr int kBlock = 256;            // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
-constexpr int kChunkPts = 4096;        // points per chunk of the sort passes
+constexpr int kChunkPts = 16384;       // points per chunk of the sort passes
+constexpr int kSortBlock = 1024;       // threads of a sort-pass workgroup
+constexpr int kSortCap = 7680;         // records a band sorts in LDS (larger bands: two passes)
 constexpr uint32_t kNoD2 = 0xFFFFFFFFu;
 constexpr int kScanBlocks = 512;       // fixed grid of the device-wide scan
 constexpr int kScanItems = 4;          // items per thread per scan step
 constexpr int kMaxBands = 8192;        // bucket-row bands per tile (LDS histogram)
 constexpr int kBandBuckets = 8192;     // buckets per band (LDS counters)
-constexpr int kBandPts = 16384;        // points per band aimed at (mean density)
+constexpr int kBandPts = 4096;         // points per band aimed at (mean density)
 constexpr double kDenseSites = 2.0;    // sites per bucket above which searches go ring by ring
 
 // Everything a kernel needs to know about one tile of a batch.  Lengths are

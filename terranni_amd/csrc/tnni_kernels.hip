@@ -88,7 +88,7 @@ constexpr int kPtsPerStep = 4;  // points per thread with their loads in flight
 // Pass 1 of the site sort: per chunk of one tile, the number of its sites in
 // each band of the tile, written band-major (band, chunk) so that one scan
 // gives every chunk its first slot in every band.
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kSortBlock)
 k_band_count(BatchDev *b, const TileDev *tiles, const Chunk *chunks, uint32_t *hist)
 {
     __shared__ uint32_t h[kMaxBands];
@@ -101,13 +101,13 @@ k_band_count(BatchDev *b, const TileDev *tiles, const Chunk *chunks, uint32_t *h
         const Chunk ch = chunks[c];
         const TileDev &t = tiles[ch.tile];
         const int nb = t.nbands;
-        for (int i = threadIdx.x; i < nb; i += kBlock) h[i] = 0u;
+        for (int i = threadIdx.x; i < nb; i += kSortBlock) h[i] = 0u;
         __syncthreads();
-        for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kBlock * kPtsPerStep) {
+        for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kSortBlock * kPtsPerStep) {
             int32_t X[kPtsPerStep], Y[kPtsPerStep], Tm[kPtsPerStep];
 #pragma unroll
             for (int q = 0; q < kPtsPerStep; ++q) {
-                const int64_t i = i0 + (int64_t)q * kBlock;
+                const int64_t i = i0 + (int64_t)q * kSortBlock;
                 X[q] = i < ch.end ? t.X[i] : 0;
                 Y[q] = i < ch.end ? t.Y[i] : 0;
                 Tm[q] = (t.T && i < ch.end) ? t.T[i] : 0;
@@ -116,14 +116,14 @@ k_band_count(BatchDev *b, const TileDev *tiles, const Chunk *chunks, uint32_t *h
             for (int q = 0; q < kPtsPerStep; ++q) {
                 int xu, yu;
                 uint32_t dt2;
-                if (i0 + (int64_t)q * kBlock < ch.end && point_units(t, X[q], Y[q], xu, yu) &&
+                if (i0 + (int64_t)q * kSortBlock < ch.end && point_units(t, X[q], Y[q], xu, yu) &&
                     (!t.T || slice_site(t, Tm[q], dt2)))
                     atomicAdd(&h[point_band(t, yu)], 1u);
             }
         }
         __syncthreads();
         const int64_t col = c - t.chunk_base;
-        for (int i = threadIdx.x; i < nb; i += kBlock)
+        for (int i = threadIdx.x; i < nb; i += kSortBlock)
             hist[t.hist_base + (int64_t)i * t.nchunks + col] = h[i];
         __syncthreads();
     }
@@ -131,7 +131,7 @@ k_band_count(BatchDev *b, const TileDev *tiles, const Chunk *chunks, uint32_t *h
 
 // Pass 2: every chunk places its sites in band order (LDS cursors seeded
 // from the scanned band-major histogram).
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kSortBlock)
 k_band_scatter(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
                const uint32_t *__restrict__ hist, int4 *__restrict__ banded)
 {
@@ -142,14 +142,14 @@ k_band_scatter(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
         const TileDev &t = tiles[ch.tile];
         const int nb = t.nbands;
         const int64_t col = c - t.chunk_base;
-        for (int i = threadIdx.x; i < nb; i += kBlock)
+        for (int i = threadIdx.x; i < nb; i += kSortBlock)
             cur[i] = hist[t.hist_base + (int64_t)i * t.nchunks + col];
         __syncthreads();
-        for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kBlock * kPtsPerStep) {
+        for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kSortBlock * kPtsPerStep) {
             int32_t X[kPtsPerStep], Y[kPtsPerStep], W[kPtsPerStep];
 #pragma unroll
             for (int q = 0; q < kPtsPerStep; ++q) {
-                const int64_t i = i0 + (int64_t)q * kBlock;
+                const int64_t i = i0 + (int64_t)q * kSortBlock;
                 X[q] = i < ch.end ? t.X[i] : 0;
                 Y[q] = i < ch.end ? t.Y[i] : 0;
                 W[q] = i < ch.end ? (t.T ? t.T[i] : t.Z[i]) : 0;
@@ -159,7 +159,7 @@ k_band_scatter(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
             // slice its squared time distance
 #pragma unroll
             for (int q = 0; q < kPtsPerStep; ++q) {
-                const int64_t i = i0 + (int64_t)q * kBlock;
+                const int64_t i = i0 + (int64_t)q * kSortBlock;
                 int xu, yu;
                 uint32_t dt2 = 0;
                 if (i < ch.end && point_units(t, X[q], Y[q], xu, yu) &&
@@ -282,13 +282,21 @@ __device__ __forceinline__ int tile_of_band(const TileDev *tiles, int n_tiles, i
 // Pass 3: one workgroup per band sorts the band's sites into its buckets
 // (LDS counters), writes the bucket offsets, and the distance of every
 // bucket to the nearest bucket holding a site in its own row (hdist).
-__global__ void __launch_bounds__(kBlock)
+// A band of at most kSortCap sites is read once into registers, each
+// record taking its rank in its bucket from the counter; after the scan the
+// records are placed in an LDS copy of the band's output and written out in
+// order, whole lines.  A larger band is read twice and its records written
+// one by one to their slots.
+constexpr int kSortRecs = (kSortCap + kSortBlock - 1) / kSortBlock;  // records per thread
+
+__global__ void __launch_bounds__(kSortBlock)
 k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ hist,
             const int4 *__restrict__ banded, int4 *__restrict__ sorted, uint32_t *__restrict__ offs,
             uint8_t *__restrict__ hdist)
 {
     __shared__ uint32_t cnt[kBandBuckets];
-    __shared__ uint32_t s_w[kWaves];
+    __shared__ int4 stage[kSortCap];
+    __shared__ uint32_t s_w[kSortBlock / 64];
     const int64_t n_bands = b->n_bands;
     const int n_tiles = b->n_tiles;
     for (int64_t g = blockIdx.x; g < n_bands; g += gridDim.x) {
@@ -299,28 +307,50 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
         const int nbx = t.nbx, r0 = band * t.br, r1 = min(r0 + t.br, t.nby);
         const int nloc = (r1 - r0) * nbx;
         const int64_t obase = t.bk_base + (int64_t)r0 * nbx;
-        for (int i = threadIdx.x; i < nloc; i += kBlock) cnt[i] = 0u;
+        const uint32_t n = s1 - s0;
+        const bool in_lds = n <= (uint32_t)kSortCap;
+        for (int i = threadIdx.x; i < nloc; i += kSortBlock) cnt[i] = 0u;
         __syncthreads();
-        // kPtsPerStep records in flight per thread
-        for (uint32_t p0 = s0 + threadIdx.x; p0 < s1; p0 += kBlock * kPtsPerStep) {
-            int4 rec[kPtsPerStep];
+        int4 rec[kSortRecs];
+        uint32_t key[kSortRecs];  // bucket in the band | rank in the bucket << 16
+        if (in_lds) {
 #pragma unroll
-            for (int q = 0; q < kPtsPerStep; ++q) {
-                const uint32_t p = p0 + (uint32_t)q * kBlock;
-                rec[q] = p < s1 ? banded[p] : make_int4(0, t.gx0, t.gy0, 0);
+            for (int q = 0; q < kSortRecs; ++q) {
+                const uint32_t i = threadIdx.x + (uint32_t)q * kSortBlock;
+                rec[q] = i < n ? banded[s0 + i] : make_int4(0, t.gx0, t.gy0, 0);
             }
 #pragma unroll
-            for (int q = 0; q < kPtsPerStep; ++q) {
-                if (p0 + (uint32_t)q * kBlock >= s1) break;
-                const int bx = div_pos(rec[q].y - t.gx0, t.bs, t.inv_bs);
-                const int by = div_pos(rec[q].z - t.gy0, t.bs, t.inv_bs);
-                atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
+            for (int q = 0; q < kSortRecs; ++q) {
+                key[q] = 0u;
+                if (threadIdx.x + (uint32_t)q * kSortBlock < n) {
+                    const int bx = div_pos(rec[q].y - t.gx0, t.bs, t.inv_bs);
+                    const int by = div_pos(rec[q].z - t.gy0, t.bs, t.inv_bs);
+                    const uint32_t bk = (uint32_t)((by - r0) * nbx + bx);
+                    key[q] = bk | (atomicAdd(&cnt[bk], 1u) << 16);
+                }
+            }
+        } else {
+            // kPtsPerStep records in flight per thread
+            for (uint32_t p0 = s0 + threadIdx.x; p0 < s1; p0 += kSortBlock * kPtsPerStep) {
+                int4 r4[kPtsPerStep];
+#pragma unroll
+                for (int q = 0; q < kPtsPerStep; ++q) {
+                    const uint32_t p = p0 + (uint32_t)q * kSortBlock;
+                    r4[q] = p < s1 ? banded[p] : make_int4(0, t.gx0, t.gy0, 0);
+                }
+#pragma unroll
+                for (int q = 0; q < kPtsPerStep; ++q) {
+                    if (p0 + (uint32_t)q * kSortBlock >= s1) break;
+                    const int bx = div_pos(r4[q].y - t.gx0, t.bs, t.inv_bs);
+                    const int by = div_pos(r4[q].z - t.gy0, t.bs, t.inv_bs);
+                    atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
+                }
             }
         }
         __syncthreads();
         // row distances to a bucket holding a site, capped at ecap
         const int cap = t.ecap;
-        for (int i = threadIdx.x; i < nloc; i += kBlock) {
+        for (int i = threadIdx.x; i < nloc; i += kSortBlock) {
             const int rr = i / nbx, x = i - rr * nbx;
             int d = cap;
             for (int q = 0; q < cap; ++q)
@@ -332,35 +362,44 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
         }
         __syncthreads();
         // exclusive scan of the counters: each thread a contiguous segment
-        const int seg = (nloc + kBlock - 1) / kBlock;
+        const int seg = (nloc + kSortBlock - 1) / kSortBlock;
         const int sb = min(nloc, (int)threadIdx.x * seg), se = min(nloc, sb + seg);
         uint32_t sum = 0;
         for (int i = sb; i < se; ++i) sum += cnt[i];
         uint32_t total;
-        uint32_t run = block_excl_scan<kBlock>(sum, s_w, total);
+        uint32_t run = block_excl_scan<kSortBlock>(sum, s_w, total);
         for (int i = sb; i < se; ++i) {
             const uint32_t v = cnt[i];
             cnt[i] = run;
             run += v;
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < nloc; i += kBlock) offs[obase + i] = s0 + cnt[i];
+        for (int i = threadIdx.x; i < nloc; i += kSortBlock) offs[obase + i] = s0 + cnt[i];
         if (r1 == t.nby && threadIdx.x == 0) offs[t.bk_base + (int64_t)t.nby * nbx] = s1;
-        __syncthreads();
-        for (uint32_t p0 = s0 + threadIdx.x; p0 < s1; p0 += kBlock * kPtsPerStep) {
-            int4 rec[kPtsPerStep];
+        if (in_lds) {
 #pragma unroll
-            for (int q = 0; q < kPtsPerStep; ++q) {
-                const uint32_t p = p0 + (uint32_t)q * kBlock;
-                rec[q] = p < s1 ? banded[p] : make_int4(0, t.gx0, t.gy0, 0);
-            }
+            for (int q = 0; q < kSortRecs; ++q)
+                if (threadIdx.x + (uint32_t)q * kSortBlock < n)
+                    stage[cnt[key[q] & 0xFFFFu] + (key[q] >> 16)] = rec[q];
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += kSortBlock) sorted[s0 + i] = stage[i];
+        } else {
+            __syncthreads();
+            for (uint32_t p0 = s0 + threadIdx.x; p0 < s1; p0 += kSortBlock * kPtsPerStep) {
+                int4 r4[kPtsPerStep];
 #pragma unroll
-            for (int q = 0; q < kPtsPerStep; ++q) {
-                if (p0 + (uint32_t)q * kBlock >= s1) break;
-                const int bx = div_pos(rec[q].y - t.gx0, t.bs, t.inv_bs);
-                const int by = div_pos(rec[q].z - t.gy0, t.bs, t.inv_bs);
-                const uint32_t pos = atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
-                sorted[s0 + pos] = rec[q];
+                for (int q = 0; q < kPtsPerStep; ++q) {
+                    const uint32_t p = p0 + (uint32_t)q * kSortBlock;
+                    r4[q] = p < s1 ? banded[p] : make_int4(0, t.gx0, t.gy0, 0);
+                }
+#pragma unroll
+                for (int q = 0; q < kPtsPerStep; ++q) {
+                    if (p0 + (uint32_t)q * kSortBlock >= s1) break;
+                    const int bx = div_pos(r4[q].y - t.gx0, t.bs, t.inv_bs);
+                    const int by = div_pos(r4[q].z - t.gy0, t.bs, t.inv_bs);
+                    const uint32_t pos = atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
+                    sorted[s0 + pos] = r4[q];
+                }
             }
         }
         __syncthreads();
@@ -1008,16 +1047,16 @@ hipError_t enqueue_pipeline(const DevBuffers &d, const LaunchCfg &cfg, hipStream
                             hipEvent_t ev0, hipEvent_t ev1)
 {
     hipError_t e;
-    hipLaunchKernelGGL(k_band_count, dim3(cfg.sort_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
+    hipLaunchKernelGGL(k_band_count, dim3(cfg.sort_grid), dim3(kSortBlock), 0, s, d.batch, d.tiles,
                        d.chunks, d.hist);
     hipLaunchKernelGGL(k_scan_reduce, dim3(kScanBlocks), dim3(kBlock), 0, s, d.batch, d.hist,
                        d.partials);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanBlocks), 0, s, d.partials);
     hipLaunchKernelGGL(k_scan_down, dim3(kScanBlocks), dim3(kBlock), 0, s, d.batch, d.hist,
                        d.partials);
-    hipLaunchKernelGGL(k_band_scatter, dim3(cfg.sort_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
+    hipLaunchKernelGGL(k_band_scatter, dim3(cfg.sort_grid), dim3(kSortBlock), 0, s, d.batch, d.tiles,
                        d.chunks, d.hist, d.banded);
-    hipLaunchKernelGGL(k_band_sort, dim3(cfg.band_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
+    hipLaunchKernelGGL(k_band_sort, dim3(cfg.band_grid), dim3(kSortBlock), 0, s, d.batch, d.tiles,
                        d.hist, d.banded, d.sorted, d.offs, d.hdist);
     hipLaunchKernelGGL(k_edist_cols, dim3(cfg.row_grid), dim3(kBlock), 0, s, d.batch, d.tiles,
                        d.hdist, d.edist);
