@@ -131,7 +131,7 @@ def bench_batch(T, S, ctx, reps: int = 3, tune: bool = True):
     }, tiles, ctx.tune_table()
 
 
-def bench_served(T, tiles, rates, per_rate, resident: bool, seed=4, tune_table=None):
+def bench_served(T, tiles, rates, n_req, resident: bool, seed=4, tune_table=None):
     """Served form: Poisson arrivals at each rate of tiles drawn from the
     uneven-tile pool; one thread submits, another waits in order.  resident:
     the tiles' points and the output rasters live in HBM (tnni_submit_dev);
@@ -141,9 +141,10 @@ def bench_served(T, tiles, rates, per_rate, resident: bool, seed=4, tune_table=N
     import threading
     out = []
     rng = np.random.default_rng(seed)
-    ring = 192  # output rasters in flight (resident)
-    # admission by HBM footprint within a 200 GB budget
-    with T.Context(0, n_slots=3, max_batch_tiles=64, hbm_budget=200 << 30) as c:
+    ring = 512  # output rasters in flight (resident)
+    # admission by HBM footprint within a 200 GB budget; batches of up to 256
+    # tiles (the engine's default), three in flight
+    with T.Context(0, n_slots=3, max_batch_tiles=256, hbm_budget=200 << 30) as c:
         if tune_table:
             c.tune_load(tune_table)
         if resident:
@@ -162,6 +163,9 @@ def bench_served(T, tiles, rates, per_rate, resident: bool, seed=4, tune_table=N
             p.wait()
         for rate in rates:
             c.serve_stats_reset()
+            # saturation: enough requests that filling and draining the
+            # pipeline do not dominate the span
+            per_rate = n_req if rate > 0 else max(n_req, 1000)
             picks = rng.integers(0, len(tiles), per_rate)
             # rate 0: saturation, every request submitted as soon as it can be
             gaps = rng.exponential(1.0 / rate, per_rate) if rate > 0 else np.zeros(per_rate)
