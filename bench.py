@@ -409,6 +409,22 @@ def load_valu(kernel="k_tiles<0, 0, false>"):
     return None if k is None else k.get("SQ_INSTS_VALU")
 
 
+def load_lds(kernel="k_tiles<0, 0, false>"):
+    """LDS activity of a k_tiles launch on this workload, from the committed
+    LDS counter summary (profiles/lds_counters_single_4m.json): the share of
+    CU-cycles the LDS is busy (peak 128 B per CU per cycle) and the share of
+    those stalled by bank conflicts."""
+    path = os.path.join(ROOT, "profiles", "lds_counters_single_4m.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f).get("kernels", {}).get(kernel)
+    if k is None:
+        return None
+    return {"active_frac": k["lds_active_frac"], "bank_conflict_frac": k["lds_bank_conflict_frac"],
+            "peak_bytes_per_cu_clk": 128, "out_of_range_accesses": k["SQ_LDS_MEM_VIOLATIONS"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -542,6 +558,9 @@ def main():
         rate = valu / (k_ms * 1e-3)
         out["roofline"]["valu"] = {"instructions": valu, "achieved_per_s": rate,
                                    "peak_per_s": VALU_PEAK, "frac": rate / VALU_PEAK}
+    lds = load_lds()
+    if lds:
+        out["roofline"]["lds"] = lds
     for d in (dX, dY, dZ, dH):
         d.free()
     pool, table = None, None
