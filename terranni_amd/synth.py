@@ -112,6 +112,10 @@ def uniform_points(rng, n, lo, hi):
 def clustered_points(rng, n, ext, n_parents, water=0.30, dup=0.01, border=0.0):
     """Matern cluster process over [-border, ext + border)^2 with water."""
     mask, res = _water_mask(rng, ext, water)
+    if mask.all():
+        # an extent of a few metres can fall wholly in one water disc: there
+        # is no land to put parents on, so the water is dropped
+        mask[:] = False
     lo, hi = -border, ext + border
     # parents uniform over land
     px = np.empty(0)

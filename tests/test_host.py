@@ -1,8 +1,10 @@
-"""Host-side pieces without a GPU: the TPIE subset behind nni.h, and the
-nni.h API library's exports."""
+"""Host-side pieces without a GPU: the TPIE subset behind nni.h, the nni.h
+API library's exports, and the synthetic tile generator."""
 import ctypes
 import os
 import subprocess
+
+import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -54,3 +56,12 @@ def test_shape_class_separates_density_grid_and_form():
     assert T.shape_class(q, 4_000_000, True) != T.shape_class(p, 1_000_000, True)
     c = T.TileParams(nrows=1000, ncols=1000, cell_size=0.5, scale=1, algorithm=T.ALG_CONES)
     assert T.shape_class(c, 1_000_000, True) != base
+
+
+@pytest.mark.timeout(60)
+def test_clustered_tile_smaller_than_a_water_disc():
+    """A clustered tile of a few metres can lie wholly in one water disc of
+    the generator's 2 m mask; it still gets its points (land everywhere)."""
+    import terranni_amd.synth as S
+    t = S.make_tile(51, 200, 7, cell_size=0.25, scale=4, kind="clustered")
+    assert t.n == 200
