@@ -120,3 +120,41 @@ def test_sweep_covers_both_search_modes_and_time():
     assert any(c["n"] == 0 for c in cases) and any(c["n"] >= 20000 for c in cases)
     assert len({c["params"]["accum_mode"] for c in cases}) == 3
     assert len({c["params"]["algorithm"] for c in cases}) == 2
+
+
+@pytest.mark.parametrize("alg", [T.ALG_PLANES, T.ALG_CONES])
+def test_random_geometries_in_one_batch(ctx, alg):
+    """The sweep's 2-D tiles of one region shape and height mode, all in one
+    batched launch (so tiles of every scale, radius, bucket size and grid
+    shape share the sort passes and the work queue): every tile's labels,
+    d2 and heights equal the oracle's as when it runs alone.  Cell-tile
+    sizes are left to the engine (one explicit size too large for the LDS
+    would refuse the whole batch)."""
+    cases = [draw_case(k) for k in range(N_CASES)]
+    for mode in (T.ACC_FIXED64, T.ACC_FP32_2SUM, T.ACC_S0):
+        group = []
+        for c in cases:
+            if c["time_r"] is not None or c["params"]["algorithm"] != alg or c["params"]["accum_mode"] != mode:
+                continue
+            t = S.make_tile(c["seed"], max(c["n"], 1), c["grid"], cell_size=c["cs"], scale=c["scale"],
+                            kind=c["kind"], **c["params"])
+            p = dataclasses.replace(t.params, nrows=c["rows"], tile_cells=0)
+            group.append((p, t.X[:c["n"]], t.Y[:c["n"]], t.Z[:c["n"]]))
+        _say("batch", alg, mode, len(group), "tiles")
+        assert len(group) >= 3
+        dev = []
+        for p, X, Y, Z in group:
+            f = T.frame(p)
+            dev.append((ctx.upload(np.ascontiguousarray(X, np.int32)), ctx.upload(np.ascontiguousarray(Y, np.int32)),
+                        ctx.upload(np.ascontiguousarray(Z, np.int32)), ctx.alloc((p.nrows, p.ncols), np.float32),
+                        ctx.alloc((f["H"], f["W"]), np.int32), ctx.alloc((f["H"], f["W"]), np.uint32)))
+        ctx.batch_run_dev([g[0] for g in group], [d[0] for d in dev], [d[1] for d in dev],
+                          [d[2] for d in dev], [g[1].shape[0] for g in group], [d[3] for d in dev],
+                          [d[4] for d in dev], [d[5] for d in dev])
+        for (p, X, Y, Z), d in zip(group, dev):
+            ref = O.run(p, X, Y, Z, s0=mode == T.ACC_S0)
+            assert np.array_equal(d[4].to_host(), ref["labels"])
+            assert np.array_equal(d[5].to_host(), ref["d2"])
+            check_heights(mode, d[3].to_host(), ref)
+            for a in d:
+                a.free()
