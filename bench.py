@@ -283,7 +283,7 @@ def bench_stream(T, S, args):
         las = os.path.join(work, "las")
         os.makedirs(las)
         t0 = time.perf_counter()
-        files = S.streamed_terrain_las(las, seed=5, extent_m=ext, n_points=args.stream_points,
+        files = S.streamed_terrain_las(las, seed=5, extent_m=ext, n_points=args.stream_points, progress=log,
                                        origin=origin)
         n_recs = sum((os.path.getsize(f) - 227) // 20 for f in files)
         log(f"streamed terrain: {len(files)} LAS tiles, {n_recs} records, generated in "

@@ -343,11 +343,14 @@ def fbm_height(x, y, seed: int, hurst: float = 0.8, octaves: int = 12, base_m: f
 
 def streamed_terrain_las(out_dir: str, seed: int = 5, extent_m: float = 5000.0,
                          n_points: int = 20_000_000, tile_m: float = 1000.0,
-                         overlap: float = 0.05, origin=(500000.0, 4200000.0), chunk: int = 4_000_000):
+                         overlap: float = 0.05, origin=(500000.0, 4200000.0), chunk: int = 4_000_000,
+                         progress=None):
     """The streamed-terrain workload: n_points uniform over an extent_m
     square of fBm terrain (Hurst 0.8), cut into LAS tiles of tile_m with
     `overlap` of the tile side shared with each neighbour (points there are
-    written to both files, exact duplicates).  Returns the list of files."""
+    written to both files, exact duplicates).  Returns the list of files.
+    progress (optional): called with a line of text after each chunk of
+    points and each file written."""
     import os
     rng = np.random.default_rng(seed)
     nt = int(np.ceil(extent_m / tile_m))
@@ -369,6 +372,8 @@ def streamed_terrain_las(out_dir: str, seed: int = 5, extent_m: float = 5000.0,
                      (y >= r * tile_m - margin) & (y < (r + 1) * tile_m + margin))
                 if m.any():
                     per[r][c].append((X[m], Y[m], Z[m]))
+        if progress:
+            progress(f"terrain points {n_points - left} of {n_points}")
     files = []
     for r in range(nt):
         for c in range(nt):
@@ -381,4 +386,6 @@ def streamed_terrain_las(out_dir: str, seed: int = 5, extent_m: float = 5000.0,
             write_las(path, X, Y, Z, None, offset=(origin[0], origin[1], 0.0))
             files.append(path)
             per[r][c] = None
+            if progress:
+                progress(f"wrote {path} ({X.shape[0]} points)")
     return files
