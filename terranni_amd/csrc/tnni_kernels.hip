@@ -641,11 +641,12 @@ __device__ __forceinline__ void square_staged(const TileDev &t, const WaveStage 
 // pixel per lane; inactive lanes still take part in the staging).  Each
 // lane searches the bucket grid in rings around its pixel's bucket: rings
 // closer than the bucket's empty distance r0 hold no site; rings 0 and 1
-// are visited as one 3x3 square, from the wave's LDS stage when the window
-// fits it; the search ends when every unvisited site is provably farther
-// than the best so far (strictly, so an equal distance still reaches the
-// lower index) or outside the site region, and otherwise goes on later
-// (pending) from ring max(r0, 2).
+// come from the wave's LDS stage when the window fits it, as one 3x3
+// square on sparse tiles and ring by ring (stopping after ring 0 where it
+// settles the pixel) on dense ones; the search ends when every unvisited
+// site is provably farther than the best so far (strictly, so an equal
+// distance still reaches the lower index) or outside the site region, and
+// otherwise goes on later (pending) from ring max(r0, 2).
 template <int ALG, bool TIME>
 __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__restrict__ offs,
                                             const uint8_t *__restrict__ edist,
