@@ -134,6 +134,20 @@ public:
         return NO_ERROR;
     }
 
+    // len items at the current position (TPIE's write_array).
+    err write_array(const T *mm_space, TPIE_OS_SIZE_T len)
+    {
+        if (!f_ || (!mm_space && len)) return IO_ERROR;
+        if (len == 0) return NO_ERROR;
+        if (flush() != NO_ERROR) return IO_ERROR;
+        rbuf_.clear();
+        if (fseeko(f_, (off_t)pos_ * (off_t)sizeof(T), SEEK_SET) != 0) return IO_ERROR;
+        const size_t put = std::fwrite(mm_space, sizeof(T), (size_t)len, f_);
+        pos_ += (TPIE_OS_OFFSET)put;
+        if (pos_ > len_) len_ = pos_;
+        return put == (size_t)len ? NO_ERROR : IO_ERROR;
+    }
+
     err seek(TPIE_OS_OFFSET off)
     {
         if (off < 0 || off > length()) return OS_ERROR;

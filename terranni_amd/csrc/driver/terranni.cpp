@@ -43,6 +43,8 @@
 #include <string>
 #include <vector>
 
+#include <omp.h>
+
 #include "las_reader.h"
 #include "nni.h"
 #include "terranni.h"
@@ -273,54 +275,82 @@ struct Frame {
 
 // Bin every point of `files` into the streams of the packages whose extent
 // holds it, then sort each stream by time (stable, so equal times keep the
-// file order).  At most kMaxOpen streams are open at once: more packages take
-// more passes over the input.
+// file order; a stream whose times already arrived in order is left as it
+// is, which is what the stable sort would give).  At most kMaxOpen streams
+// are open at once: more packages take more passes over the input.
 template <class Rec, class Make>
 void bin_points(const std::vector<std::string> &files, const Layout &L, const std::string &dir,
                 const char *ext, Make make)
 {
     constexpr int kMaxOpen = 256;
     const int n_wp = L.cols * L.rows;
-    std::vector<int32_t> X, Y, Z;
-    std::vector<uint16_t> I;
     const size_t kBlock = 1 << 16;
-    X.resize(kBlock);
-    Y.resize(kBlock);
-    Z.resize(kBlock);
-    I.resize(kBlock);
+    // files are binned side by side, a group of them at a time, each into
+    // its own per-package buffers; the buffers then go to the streams in
+    // file order, so every stream holds its points in file order
+    const int n_par = std::max(1, omp_get_max_threads());
     for (int g0 = 0; g0 < n_wp; g0 += kMaxOpen) {
         const int g1 = std::min(n_wp, g0 + kMaxOpen);
+        const size_t ng = (size_t)(g1 - g0);
         std::vector<std::unique_ptr<tpie::ami::stream<Rec>>> st;
         for (int k = g0; k < g1; ++k) {
             st.emplace_back(new tpie::ami::stream<Rec>(wp_file(dir, k / L.cols, k % L.cols, ext)));
             if (!st.back()->is_valid()) die("cannot create " + wp_file(dir, k / L.cols, k % L.cols, ext));
             st.back()->truncate(0);
         }
-        for (const std::string &f : files) {
-            nni_io::las_file lf;
-            lf.open(f);
-            const nni_io::las_header &h = lf.header();
-            size_t got;
-            while ((got = lf.read_raw(kBlock, X.data(), Y.data(), Z.data(), I.data())) > 0) {
-                for (size_t i = 0; i < got; ++i) {
-                    const double x = (double)X[i] * h.scale[0] + h.offset[0];
-                    const double y = (double)Y[i] * h.scale[1] + h.offset[1];
-                    int c0, c1, r0, r1;
-                    if (!wp_span(x - L.minx, L.wpw(), L.border, L.cols, c0, c1)) continue;
-                    if (!wp_span(y - L.miny, L.wph(), L.border, L.rows, r0, r1)) continue;
-                    const Rec rec = make(h, X[i], Y[i], Z[i], I[i], x, y);
-                    for (int r = r0; r <= r1; ++r)
-                        for (int c = c0; c <= c1; ++c) {
-                            const int k = r * L.cols + c;
-                            if (k >= g0 && k < g1) st[(size_t)(k - g0)]->write_item(rec);
+        std::vector<int32_t> last_t(ng, INT32_MIN);
+        std::vector<char> in_order(ng, 1);
+        for (size_t f0 = 0; f0 < files.size(); f0 += (size_t)n_par) {
+            const size_t nf = std::min(files.size() - f0, (size_t)n_par);
+            std::vector<std::vector<std::vector<Rec>>> out(nf, std::vector<std::vector<Rec>>(ng));
+            std::vector<std::string> err(nf);
+#pragma omp parallel for schedule(dynamic, 1)
+            for (size_t fi = 0; fi < nf; ++fi) {
+                try {
+                    std::vector<int32_t> X(kBlock), Y(kBlock), Z(kBlock);
+                    std::vector<uint16_t> I(kBlock);
+                    nni_io::las_file lf;
+                    lf.open(files[f0 + fi]);
+                    const nni_io::las_header &h = lf.header();
+                    size_t got;
+                    while ((got = lf.read_raw(kBlock, X.data(), Y.data(), Z.data(), I.data())) > 0) {
+                        for (size_t i = 0; i < got; ++i) {
+                            const double x = (double)X[i] * h.scale[0] + h.offset[0];
+                            const double y = (double)Y[i] * h.scale[1] + h.offset[1];
+                            int c0, c1, r0, r1;
+                            if (!wp_span(x - L.minx, L.wpw(), L.border, L.cols, c0, c1)) continue;
+                            if (!wp_span(y - L.miny, L.wph(), L.border, L.rows, r0, r1)) continue;
+                            const Rec rec = make(h, X[i], Y[i], Z[i], I[i], x, y);
+                            for (int r = r0; r <= r1; ++r)
+                                for (int c = c0; c <= c1; ++c) {
+                                    const int k = r * L.cols + c;
+                                    if (k >= g0 && k < g1) out[fi][(size_t)(k - g0)].push_back(rec);
+                                }
                         }
+                    }
+                } catch (const std::exception &e) {
+                    err[fi] = e.what();
+                }
+            }
+            for (size_t fi = 0; fi < nf; ++fi) {
+                if (!err[fi].empty()) throw std::runtime_error(err[fi]);
+                for (size_t j = 0; j < ng; ++j) {
+                    const std::vector<Rec> &v = out[fi][j];
+                    if (v.empty()) continue;
+                    if (st[j]->write_array(v.data(), (TPIE_OS_SIZE_T)v.size()) != tpie::ami::NO_ERROR)
+                        die("writing a work package failed");
+                    for (const Rec &rec : v) {
+                        if ((int32_t)rec.t < last_t[j]) in_order[j] = 0;
+                        last_t[j] = (int32_t)rec.t;
+                    }
                 }
             }
         }
         Timecmp cmp;
-        for (auto &s : st) {
-            s->flush();
-            if (tpie::ami::sort(s.get(), &cmp) != tpie::ami::NO_ERROR) die("sorting a work package failed");
+        for (size_t j = 0; j < st.size(); ++j) {
+            st[j]->flush();
+            if (!in_order[j] && tpie::ami::sort(st[j].get(), &cmp) != tpie::ami::NO_ERROR)
+                die("sorting a work package failed");
         }
     }
 }

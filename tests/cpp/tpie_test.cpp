@@ -58,6 +58,27 @@ int main(int argc, char **argv)
         CHECK(s.read_array(buf.data(), &len) == tpie::ami::NO_ERROR && len == 4);
         CHECK(buf[0].key == 98 && buf[2].key == -1 && buf[3].key == 101);
         CHECK(s.read_item(&p) == tpie::ami::NO_ERROR && p->key == 102);
+        // bulk write after buffered single writes, at the end and over the middle
+        s.seek(50000);
+        s.write_item(Item{50000, 0});
+        std::vector<Item> more(3);
+        for (int k = 0; k < 3; ++k) more[k] = Item{50001 + k, 0};
+        CHECK(s.write_array(more.data(), 3) == tpie::ami::NO_ERROR);
+        CHECK(s.stream_len() == 50004 && s.tell() == 50004);
+        s.seek(200);
+        more[0].key = more[1].key = -2;
+        CHECK(s.write_array(more.data(), 2) == tpie::ami::NO_ERROR && s.tell() == 202);
+        s.seek(199);
+        len = 4;
+        CHECK(s.read_array(buf.data(), &len) == tpie::ami::NO_ERROR && len == 4);
+        CHECK(buf[0].key == 199 && buf[1].key == -2 && buf[2].key == -2 && buf[3].key == 202);
+        s.seek(50000);
+        CHECK(s.read_item(&p) == tpie::ami::NO_ERROR && p->key == 50000);
+        CHECK(s.read_item(&p) == tpie::ami::NO_ERROR && p->key == 50001);
+        s.seek(50003);
+        CHECK(s.read_item(&p) == tpie::ami::NO_ERROR && p->key == 50003);
+        CHECK(s.read_item(&p) == tpie::ami::END_OF_STREAM);
+        s.truncate(50000);
     }
     // named streams persist; APPEND_STREAM starts at the end
     {
