@@ -84,6 +84,10 @@ __device__ __forceinline__ int point_band(const TileDev &t, int yu)
 }
 
 constexpr int kPtsPerStep = 4;  // points per thread with their loads in flight
+// The count and scatter passes: half a thread's share of a chunk
+// (kChunkPts / kSortBlock points) with its loads in flight at once (all of
+// it would spill the FP64 de-scale of the count pass).
+constexpr int kChunkStep = kChunkPts / kSortBlock / 2;
 
 // Pass 1 of the site sort: per chunk of one tile, the number of its sites in
 // each band of the tile, written band-major (band, chunk) so that one scan
@@ -103,17 +107,17 @@ k_band_count(BatchDev *b, const TileDev *tiles, const Chunk *chunks, uint32_t *h
         const int nb = t.nbands;
         for (int i = threadIdx.x; i < nb; i += kSortBlock) h[i] = 0u;
         __syncthreads();
-        for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kSortBlock * kPtsPerStep) {
-            int32_t X[kPtsPerStep], Y[kPtsPerStep], Tm[kPtsPerStep];
+        for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kSortBlock * kChunkStep) {
+            int32_t X[kChunkStep], Y[kChunkStep], Tm[kChunkStep];
 #pragma unroll
-            for (int q = 0; q < kPtsPerStep; ++q) {
+            for (int q = 0; q < kChunkStep; ++q) {
                 const int64_t i = i0 + (int64_t)q * kSortBlock;
                 X[q] = i < ch.end ? t.X[i] : 0;
                 Y[q] = i < ch.end ? t.Y[i] : 0;
                 Tm[q] = (t.T && i < ch.end) ? t.T[i] : 0;
             }
 #pragma unroll
-            for (int q = 0; q < kPtsPerStep; ++q) {
+            for (int q = 0; q < kChunkStep; ++q) {
                 int xu, yu;
                 uint32_t dt2;
                 if (i0 + (int64_t)q * kSortBlock < ch.end && point_units(t, X[q], Y[q], xu, yu) &&
@@ -145,10 +149,10 @@ k_band_scatter(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
         for (int i = threadIdx.x; i < nb; i += kSortBlock)
             cur[i] = hist[t.hist_base + (int64_t)i * t.nchunks + col];
         __syncthreads();
-        for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kSortBlock * kPtsPerStep) {
-            int32_t X[kPtsPerStep], Y[kPtsPerStep], W[kPtsPerStep];
+        for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kSortBlock * kChunkStep) {
+            int32_t X[kChunkStep], Y[kChunkStep], W[kChunkStep];
 #pragma unroll
-            for (int q = 0; q < kPtsPerStep; ++q) {
+            for (int q = 0; q < kChunkStep; ++q) {
                 const int64_t i = i0 + (int64_t)q * kSortBlock;
                 X[q] = i < ch.end ? t.X[i] : 0;
                 Y[q] = i < ch.end ? t.Y[i] : 0;
@@ -158,7 +162,7 @@ k_band_scatter(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
             // of the search key); w the site's value, or for a space-time
             // slice its squared time distance
 #pragma unroll
-            for (int q = 0; q < kPtsPerStep; ++q) {
+            for (int q = 0; q < kChunkStep; ++q) {
                 const int64_t i = i0 + (int64_t)q * kSortBlock;
                 int xu, yu;
                 uint32_t dt2 = 0;
