@@ -382,6 +382,107 @@ def bench_stream(T, S, args):
         shutil.rmtree(work, ignore_errors=True)
 
 
+def bench_spacetime(T, S, args):
+    """--workload spacetime: one work package stepped through time as the
+    reference's time loop calls performNNI (nni.cpp:870-944): each call at
+    cur_time takes the points of the time-sorted stream within 2r + 1 of it
+    (every site of its 2 (int)r + 1 slices), device-resident, as a window of
+    one uploaded array.  Delta keeps each slice's site values and squared
+    distances in the slice cache and searches only the slice that is new;
+    Delta^2 searches every slice.  Both must give the same heights bit for
+    bit; the CPU oracle on one call is the CPU baseline and the check."""
+    from dataclasses import replace
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    r = args.time_radius
+    w = 2.0 * r + 1.0
+    first = float(args.st_first)
+    curs = [first + k for k in range(args.warmup + args.steps)]
+    # survey years so that every call's window is full
+    years = np.arange(int(np.floor(first - w)), int(np.ceil(curs[-1] + w)) + 1)
+    t0 = time.perf_counter()
+    tile = S.make_tile(9, args.st_points_per_year * len(years), args.st_grid, kind="clustered")
+    rng = np.random.default_rng(9)
+    times = rng.choice(years, tile.n).astype(np.int32)
+    o = np.argsort(times, kind="stable")
+    X, Y, Z, Tm = (np.ascontiguousarray(a[o]) for a in (tile.X, tile.Y, tile.Z, times))
+    p = replace(tile.params)
+    cells = p.nrows * p.ncols
+    log(f"space-time workload: {tile.n} points over {len(years)} years, {cells} cells, "
+        f"generated in {time.perf_counter() - t0:.1f}s")
+    ctx = T.Context(0)
+    dev = [ctx.upload(a) for a in (X, Y, Z, Tm)]
+    dH = ctx.alloc((p.nrows, p.ncols), np.float32)
+
+    def window(cur):
+        # integer times: |T - cur| <= w  <=>  ceil(cur - w) <= T <= floor(cur + w)
+        a = int(np.searchsorted(Tm, np.int32(np.ceil(cur - w)), side="left"))
+        b = int(np.searchsorted(Tm, np.int32(np.floor(cur + w)), side="right"))
+        views = [T.DevArray(d.ptr + 4 * a, 4 * (b - a), np.dtype(np.int32), (b - a,), ctx) for d in dev]
+        return a, b, views
+
+    windows = {cur: window(cur) for cur in curs}
+
+    def call(cur, cache):
+        a, b, (vX, vY, vZ, vT) = windows[cur]
+        ctx.tile_run_time_dev(p, vX, vY, vZ, vT, b - a, cur, r, dH, cache=cache)
+        return b - a
+
+    res = {}
+    heights = {}
+    for tag, use in (("delta", True), ("delta_squared", False)):
+        cache = ctx.time_cache() if use else None
+        for cur in curs[:args.warmup]:
+            call(cur, cache)
+        lat, pts, hs = [], 0, []
+        for cur in curs[args.warmup:]:
+            s0 = time.perf_counter()
+            pts += call(cur, cache)  # ends in a stream sync
+            lat.append(time.perf_counter() - s0)
+            hs.append(dH.to_host())
+        el = sum(lat)
+        res[tag] = {"cells_per_s": cells * args.steps / el, "ms_per_call": el / args.steps * 1e3,
+                    "window_points_per_call": pts / args.steps}
+        if cache is not None:
+            hits, misses = cache.stats()
+            res[tag].update({"slices_gathered": hits, "slices_searched": misses})
+            cache.close()
+        heights[tag] = hs
+        log(f"space-time {tag}: {res[tag]}")
+    same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+               for a, b in zip(heights["delta"], heights["delta_squared"]))
+    out = {
+        "metric": f"DEM cells/s (space-time tile {p.ncols}^2, time radius {r:g}, Delta slice cache)",
+        "value": res["delta"]["cells_per_s"], "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["delta"]["ms_per_call"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": f"synthetic (seeded Matern-cluster LiDAR, survey years {years[0]}-{years[-1]}, "
+                "LAS int32 at 0.01 m)",
+        "config": {"workload": f"space-time tile {p.ncols}x{p.nrows} cells at {p.cell_size} m, "
+                               f"{tile.n} points over {len(years)} years, time radius {r:g}, "
+                               f"cur_time {curs[args.warmup]:g}..{curs[-1]:g}",
+                   "points": tile.n, "cells": cells, "form": "plain, device-resident windows"},
+        "delta": res["delta"], "delta_squared": res["delta_squared"],
+        "delta_vs_delta_squared": res["delta"]["cells_per_s"] / res["delta_squared"]["cells_per_s"],
+        "delta_equals_delta_squared": same,
+    }
+    if not args.no_cpu:
+        cur = curs[-1]
+        a, b, _ = windows[cur]
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        s0 = time.perf_counter()
+        ref = O.run_time(p, X[a:b], Y[a:b], Z[a:b], Tm[a:b], cur, r, nthreads=threads)
+        o_s = time.perf_counter() - s0
+        out["cpu_baseline"] = {"value": cells / o_s, "unit": "cells/s", "cores": threads, "kind": "port",
+                               "sample": f"one call (cur_time {cur:g}, all slices) of the same tile"}
+        out["matches_oracle"] = bool(np.array_equal(heights["delta"][-1].view(np.uint32),
+                                                    ref["h_fixed"].view(np.uint32)))
+    for d in dev + [dH]:
+        d.free()
+    ctx.close()
+    print(json.dumps(out))
+
+
 def load_traffic(kernel="k_tiles<0, 0, false>"):
     """HBM bytes per k_tiles launch on this workload, from the committed
     FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_single_4m.json)."""
@@ -435,10 +536,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-cells", type=int, default=0, help="cell-tile side (0 = auto)")
     ap.add_argument("--bucket-px", type=int, default=0, help="bucket side in pixels (0 = auto)")
-    ap.add_argument("--workload", choices=("single", "batch", "stream"), default="single",
+    ap.add_argument("--workload", choices=("single", "batch", "stream", "spacetime"), default="single",
                     help="single: the 4096^2 tile (default line, plus the batched and served "
                          "comparisons); batch: only the batched 256-tile form; stream: the "
-                         "streamed-terrain driver pipeline (down-scaled)")
+                         "streamed-terrain driver pipeline (down-scaled); spacetime: one work "
+                         "package stepped through time, Delta slice cache against Delta^2")
+    ap.add_argument("--time-radius", type=float, default=2.0, help="spacetime: time radius")
+    ap.add_argument("--st-grid", type=int, default=1024, help="spacetime: tile side in cells")
+    ap.add_argument("--st-points-per-year", type=int, default=400_000)
+    ap.add_argument("--st-first", type=float, default=2010.0, help="spacetime: first cur_time")
     ap.add_argument("--stream-km", type=float, default=4.0, help="streamed terrain side (km)")
     ap.add_argument("--stream-points", type=int, default=12_000_000)
     ap.add_argument("--stream-wp-m", type=int, default=1000, help="work package side (m)")
@@ -467,6 +573,9 @@ def main():
         return
     if args.workload == "stream":
         bench_stream(T, S, args)
+        return
+    if args.workload == "spacetime":
+        bench_spacetime(T, S, args)
         return
     t0 = time.perf_counter()
     tile = S.workload_single_4m(seed=1, n=args.points, grid=args.grid,

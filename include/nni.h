@@ -5,8 +5,11 @@
  * Same namespace, types, layouts and functions, so the reference's callers
  * (terranni.cpp: nni::init at :694, performNNI at :419, clearRenderbuffers
  * at :422, cleanup at :721) compile and run unchanged.  The OpenGL render
- * buffers are gone: setRenderbuffers / setupRun / clearRenderbuffers are
- * kept as no-ops for source compatibility.
+ * buffers are gone: setRenderbuffers / setupRun are kept as no-ops for
+ * source compatibility.  performNNI with timeRadius > 0 and useDelta keeps
+ * each slice's Voronoi values in the engine's slice cache (the Delta render
+ * buffers, nni.cpp:281-379), which clearRenderbuffers empties; without
+ * useDelta (Delta^2) every slice is searched.  Both give the same heights.
  */
 #ifndef LIBNNI_H
 #define LIBNNI_H

@@ -208,8 +208,9 @@ tnni_status tnni_serve_stats_reset(tnni_ctx *ctx);
 
 /* ---- space-time (timeRadius > 0) ----------------------------------------
  * One work package at time cur_time with time radius r, as performNNI with
- * timeRadius > 0 (nni.cpp:870-944, replaces its slice loop :910-933 and the
- * Delta render-buffer cache :321-379).  T[i] is point i's time (the LAS
+ * timeRadius > 0 (nni.cpp:870-944, replaces its slice loop :910-933 with
+ * every slice searched, the Delta^2 algorithm; the Delta algorithm's
+ * render-buffer cache is tnni_time_cache below).  T[i] is point i's time (the LAS
  * intensity, las_reader.inl:59).  Slices t = cur_time + j (float), j =
  * -(int)r .. r: slice t's sites are the points with |t - T| <= r + 1e-6
  * (nni.cpp:770), each at squared distance |x - s|^2 + (|t - T| in metres,
@@ -223,6 +224,39 @@ tnni_status tnni_tile_run_time(tnni_ctx *ctx, const tnni_tile_desc *desc, const 
 tnni_status tnni_tile_run_time_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *dX,
                                    const int32_t *dY, const int32_t *dZ, const int32_t *dT,
                                    uint64_t n, float cur_time, float time_radius, float *d_out_h);
+
+/* Space-time slice cache: performNNI with useDelta (the Delta algorithm,
+ * nni.cpp:902-921) keeps the Voronoi diagram of each slice time it has drawn
+ * and draws only the slices it does not hold (switchRenderbuffers,
+ * nni.cpp:321-359, one buffer per slice of a call, the oldest reused,
+ * getMinIndex nni.cpp:293-300).
+ * Here a cache holds, per slice time, the site value and squared distance
+ * of every pixel of the tile (8 bytes per pixel per slice, up to
+ * 2 (int)r + 1 slices, least recently used reused); a slice found there is
+ * gathered without a search.  The result is bit for bit that of the
+ * uncached call.  The caller keeps each slice's sites the same, and in the
+ * same relative order, between the calls that use one cache (a time-sorted
+ * stream read in windows around cur_time does); a call with another tile
+ * descriptor or time radius empties the cache first.  A cache belongs to
+ * one context and is used under its lock; destroy caches before their
+ * context (a cache outliving it holds nothing and only accepts destroy). */
+typedef struct tnni_time_cache tnni_time_cache;
+tnni_status tnni_time_cache_create(tnni_ctx *ctx, tnni_time_cache **out);
+void tnni_time_cache_destroy(tnni_time_cache *cache);
+/* Empty the cache, keeping its device memory (nni::clearRenderbuffers,
+ * nni.cpp:362-379). */
+tnni_status tnni_time_cache_clear(tnni_time_cache *cache);
+/* Slices gathered from the cache and slices searched, since creation. */
+tnni_status tnni_time_cache_stats(const tnni_time_cache *cache, uint64_t *hits, uint64_t *misses);
+/* tnni_tile_run_time(_dev) through a cache (NULL: uncached). */
+tnni_status tnni_tile_run_time_cached(tnni_ctx *ctx, tnni_time_cache *cache, const tnni_tile_desc *desc,
+                                      const int32_t *X, const int32_t *Y, const int32_t *Z,
+                                      const int32_t *T, uint64_t n, float cur_time, float time_radius,
+                                      float *out_h);
+tnni_status tnni_tile_run_time_cached_dev(tnni_ctx *ctx, tnni_time_cache *cache,
+                                          const tnni_tile_desc *desc, const int32_t *dX,
+                                          const int32_t *dY, const int32_t *dZ, const int32_t *dT,
+                                          uint64_t n, float cur_time, float time_radius, float *d_out_h);
 
 /* ---- autotuning --------------------------------------------------------
  * New (the reference has fixed GL raster sizes, nni.cpp:601-668): the bucket

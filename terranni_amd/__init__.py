@@ -202,13 +202,23 @@ def lib():
                                      ctypes.c_float, ctypes.c_float, P]
     L.tnni_tile_run_time_dev.argtypes = [P, ctypes.POINTER(TileDesc), P, P, P, P, ctypes.c_uint64,
                                          ctypes.c_float, ctypes.c_float, P]
+    L.tnni_time_cache_create.argtypes = [P, ctypes.POINTER(P)]
+    L.tnni_time_cache_destroy.argtypes = [P]
+    L.tnni_time_cache_destroy.restype = None
+    L.tnni_time_cache_clear.argtypes = [P]
+    L.tnni_time_cache_stats.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+    L.tnni_tile_run_time_cached.argtypes = [P, P, ctypes.POINTER(TileDesc), P, P, P, P, ctypes.c_uint64,
+                                            ctypes.c_float, ctypes.c_float, P]
+    L.tnni_tile_run_time_cached_dev.argtypes = [P, P, ctypes.POINTER(TileDesc), P, P, P, P,
+                                                ctypes.c_uint64, ctypes.c_float, ctypes.c_float, P]
     for name in ("tnni_frame", "tnni_footprint", "tnni_ctx_create", "tnni_tile_run",
                  "tnni_tile_run_dev", "tnni_batch_run_dev", "tnni_dev_alloc", "tnni_dev_free",
                  "tnni_h2d", "tnni_d2h", "tnni_d2d", "tnni_profile_enable", "tnni_profile_read",
                  "tnni_submit", "tnni_submit_dev", "tnni_poll", "tnni_wait",
                  "tnni_serve_stats_read", "tnni_serve_stats_reset", "tnni_shape_class", "tnni_autotune",
                  "tnni_tune_read", "tnni_tune_load", "tnni_tune_clear", "tnni_tile_run_time",
-                 "tnni_tile_run_time_dev"):
+                 "tnni_tile_run_time_dev", "tnni_time_cache_create", "tnni_time_cache_clear",
+                 "tnni_time_cache_stats", "tnni_tile_run_time_cached", "tnni_tile_run_time_cached_dev"):
         getattr(L, name).restype = ctypes.c_int
     del i32p
     _lib = L
@@ -265,6 +275,36 @@ class DevArray:
         if self.ptr:
             self.ctx._free(self.ptr)
             self.ptr = 0
+
+
+class TimeCache:
+    """Per slice time, the site values and squared distances of a space-time
+    tile's pixels (tnni_time_cache; the Delta render-buffer cache of
+    nni.cpp:281-379).  Close it before its context."""
+
+    def __init__(self, h, ctx: "Context"):
+        self.h = h
+        self.ctx = ctx
+
+    def clear(self):
+        _check(lib().tnni_time_cache_clear(self.h))
+
+    def stats(self):
+        """(slices gathered from the cache, slices searched)."""
+        hits, misses = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().tnni_time_cache_stats(self.h, ctypes.byref(hits), ctypes.byref(misses)))
+        return hits.value, misses.value
+
+    def close(self):
+        if self.h:
+            lib().tnni_time_cache_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class Context:
@@ -359,24 +399,34 @@ class Context:
             arr(d_d2s) if d_d2s is not None else None))
 
     # ---- space-time ----
-    def tile_run_time(self, p: TileParams, X, Y, Z, T, cur_time: float, time_radius: float):
+    def tile_run_time(self, p: TileParams, X, Y, Z, T, cur_time: float, time_radius: float,
+                      cache: "TimeCache" = None):
         """Heights of a space-time tile (host buffers): slices cur_time + j,
-        j = -(int)r..r, sites within the time radius at their time distance."""
+        j = -(int)r..r, sites within the time radius at their time distance.
+        With a cache, slices it holds are gathered without a search."""
         X = np.ascontiguousarray(X, dtype=np.int32)
         Y = np.ascontiguousarray(Y, dtype=np.int32)
         Z = np.ascontiguousarray(Z, dtype=np.int32)
         T = np.ascontiguousarray(T, dtype=np.int32)
         h = np.empty((p.nrows, p.ncols), dtype=np.float32)
         d = p.to_desc()
-        self._chk(lib().tnni_tile_run_time(self.h, ctypes.byref(d), _ptr(X), _ptr(Y), _ptr(Z), _ptr(T),
-                                           X.shape[0], cur_time, time_radius, _ptr(h)))
+        self._chk(lib().tnni_tile_run_time_cached(self.h, cache.h if cache else None, ctypes.byref(d),
+                                                  _ptr(X), _ptr(Y), _ptr(Z), _ptr(T), X.shape[0],
+                                                  cur_time, time_radius, _ptr(h)))
         return h
 
     def tile_run_time_dev(self, p: TileParams, dX, dY, dZ, dT, n: int, cur_time: float,
-                          time_radius: float, d_h):
+                          time_radius: float, d_h, cache: "TimeCache" = None):
         d = p.to_desc()
-        self._chk(lib().tnni_tile_run_time_dev(self.h, ctypes.byref(d), dX.ptr, dY.ptr, dZ.ptr,
-                                               dT.ptr, n, cur_time, time_radius, d_h.ptr))
+        self._chk(lib().tnni_tile_run_time_cached_dev(self.h, cache.h if cache else None, ctypes.byref(d),
+                                                      dX.ptr, dY.ptr, dZ.ptr, dT.ptr, n, cur_time,
+                                                      time_radius, d_h.ptr))
+
+    def time_cache(self) -> "TimeCache":
+        """A space-time slice cache on this context (tnni_time_cache)."""
+        h = ctypes.c_void_p()
+        self._chk(lib().tnni_time_cache_create(self.h, ctypes.byref(h)))
+        return TimeCache(h, self)
 
     # ---- autotuning ----
     def autotune(self, params, dXs, dYs, dZs, ns, batched: bool = True, reps: int = 3):

@@ -21,6 +21,8 @@ namespace {
 
 nni::Settings g_settings;
 tnni_ctx *g_ctx = nullptr;
+// the slices of performNNI with useDelta (nni.cpp:281-379, 902-921)
+tnni_time_cache *g_cache = nullptr;
 int g_mode = TNNI_ACC_FIXED64;
 int g_status = TNNI_OK;
 std::string g_err;
@@ -72,13 +74,18 @@ int init(const Settings &s)
 
 void cleanup()
 {
+    tnni_time_cache_destroy(g_cache);
+    g_cache = nullptr;
     if (g_ctx) tnni_ctx_destroy(g_ctx);
     g_ctx = nullptr;
 }
 
 void setRenderbuffers(int, int, int) {}
 void setupRun(const WorkPackage &, int, int) {}
-void clearRenderbuffers() {}
+void clearRenderbuffers()
+{
+    if (g_cache) (void)tnni_time_cache_clear(g_cache);
+}
 
 void printTimes()
 {
@@ -138,8 +145,10 @@ void performNNI(const WorkPackage &wp, tpie::ami::stream<OutPoint> &out,
         // point of that window, where a later call with a later curTime
         // finds all of its points.  (The reference leaves it after the
         // slices it drew, relying on its render-buffer cache for the rest,
-        // nni.cpp:321-379,782-788; the engine recomputes a slice instead,
-        // and a slice's sites do not depend on curTime.)
+        // nni.cpp:321-379,782-788.  Here every slice's sites are read again:
+        // a slice's sites do not depend on curTime, and with useDelta the
+        // engine's slice cache gathers the slices it still holds without
+        // searching.)
         const double w = 2.0 * (double)timeRadius + 1.0;
         TPIE_OS_OFFSET start = -1;
         for (;;) {
@@ -203,9 +212,16 @@ void performNNI(const WorkPackage &wp, tpie::ami::stream<OutPoint> &out,
 
     g_t_tile.start();
     std::vector<float> h((size_t)wp.nrows * wp.ncols);
+    // useDelta keeps the slices for the next curTime; Delta^2 searches
+    // every slice again (nni.cpp:915-917).  The results are the same.
+    if (space_time && useDelta && !g_cache && (st = tnni_time_cache_create(g_ctx, &g_cache)) != TNNI_OK) {
+        g_t_tile.stop();
+        set_error(st, tnni_last_error(g_ctx));
+        return;
+    }
     if (space_time)
-        st = tnni_tile_run_time(g_ctx, &d, X.data(), Y.data(), Z.data(), T.data(), n, curTime,
-                                timeRadius, h.data());
+        st = tnni_tile_run_time_cached(g_ctx, useDelta ? g_cache : nullptr, &d, X.data(), Y.data(),
+                                       Z.data(), T.data(), n, curTime, timeRadius, h.data());
     else
         st = tnni_tile_run(g_ctx, &d, X.data(), Y.data(), Z.data(), n, h.data(), nullptr, nullptr);
     g_t_tile.stop();

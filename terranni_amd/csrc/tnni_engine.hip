@@ -7,12 +7,14 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -153,6 +155,30 @@ struct Slot {
 
 }  // namespace
 
+// Space-time slice cache (the Delta render-buffer cache, nni.cpp:281-379,
+// 902-921):
+// per slice time, the site value and squared distance of each of the tile's
+// W x H pixels.  A slice's labels depend on its time, the tile and the time
+// radius only, never on cur_time, so a later call gathers a cached slice
+// without searching again.  Values are kept, not point indices: the points
+// of a later call may sit at other indices (a window of a time-sorted
+// stream); ties between sites go to the earlier point, which the window
+// keeps in order.
+struct tnni_time_cache {
+    tnni_ctx *ctx = nullptr;  // null once the context is destroyed
+    bool keyed = false;       // desc and time_radius name the cached tile
+    tnni_tile_desc desc{};
+    float time_radius = 0.0f;
+    struct Slot {
+        bool full = false;
+        uint32_t time_bits = 0;  // the slice time as float bits (+0 for -0)
+        uint64_t used = 0;       // last use, for least-recently-used reuse
+        DevBuf val, d2;
+    };
+    std::vector<Slot> slots;
+    uint64_t clock = 0, hits = 0, misses = 0;
+};
+
 struct tnni_ctx {
     int device = 0;
     int n_cu = 0;
@@ -169,6 +195,7 @@ struct tnni_ctx {
     double prof_ms = 0.0;
 
     std::vector<void *> user_allocs;
+    std::vector<tnni_time_cache *> caches;  // space-time slice caches on this context
 
     // measured bucket / cell-tile sizes per shape class (tnni_autotune)
     TuneTable tuned;
@@ -819,6 +846,15 @@ void tnni_ctx_destroy(tnni_ctx *ctx)
     }
     ctx->pinned.clear();
     for (void *p : ctx->user_allocs) (void)hipFree(p);
+    for (tnni_time_cache *c : ctx->caches) {
+        for (auto &s : c->slots) {
+            release(ctx, s.val);
+            release(ctx, s.d2);
+        }
+        c->slots.clear();
+        c->keyed = false;
+        c->ctx = nullptr;
+    }
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->tev0) (void)hipEventDestroy(ctx->tev0);
@@ -951,8 +987,12 @@ namespace {
 // pipeline run over the tile's points that keeps the points within the time
 // radius of t as sites; the cells' sums stay in the arena between the runs
 // and the last run divides.  p.t.X/Y/Z/T and out_h are set by the caller.
+// With a cache (and desc, its key) a slice found there is only gathered
+// (k_gather_slice); one not found runs the pipeline with its labels and d2
+// written into a cache slot, whose labels then become site values.
 tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_time,
-                          float time_radius)
+                          float time_radius, tnni_time_cache *cache = nullptr,
+                          const tnni_tile_desc *desc = nullptr)
 {
     const TileDev &t = p.t;
     if (!(time_radius >= 0.0f) || time_radius > 1.0e4f)
@@ -969,6 +1009,27 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
     const int j0 = -(int)time_radius;
     int j1 = j0;
     while ((float)(j1 + 1) <= time_radius) ++j1;
+    const uint64_t pix = (uint64_t)t.W * t.H;
+    if (cache) {
+        if (!std::isfinite(cur_time)) return fail(ctx, TNNI_E_INVALID, "cur_time must be finite");
+        // another tile or radius: start over (the reference's driver clears
+        // the buffers per work package, terranni.cpp:422, nni.cpp:362-379)
+        if (!cache->keyed || cache->time_radius != time_radius ||
+            std::memcmp(&cache->desc, desc, sizeof *desc) != 0) {
+            for (auto &c : cache->slots) c.full = false;
+            cache->desc = *desc;
+            cache->time_radius = time_radius;
+            cache->keyed = true;
+        }
+        // one slot per slice of a call (numRB, nni.cpp:902)
+        const size_t want = (size_t)(j1 - j0 + 1);
+        while (cache->slots.size() > want) {
+            release(ctx, cache->slots.back().val);
+            release(ctx, cache->slots.back().d2);
+            cache->slots.pop_back();
+        }
+        cache->slots.resize(want);
+    }
     for (int j = j0; j <= j1; ++j) {
         std::vector<Prepared> v(1, p);
         TileDev &s = v[0].t;
@@ -979,7 +1040,40 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
         s.acc_load = j != j0;
         s.acc_final = j == j1;
         s.acc = (uint4 *)a.acc.p;
+        if (!cache) {
+            if ((st = run_prepared(ctx, a, v, false, Launch::Direct)) != TNNI_OK) return st;
+            continue;
+        }
+        uint32_t key;
+        const float kt = s.slice_t + 0.0f;
+        std::memcpy(&key, &kt, 4);
+        tnni_time_cache::Slot *hit = nullptr, *victim = nullptr;
+        for (auto &c : cache->slots) {
+            if (c.full && c.time_bits == key) {
+                hit = &c;
+                break;
+            }
+            if (!victim || (victim->full && (!c.full || c.used < victim->used))) victim = &c;
+        }
+        const uint64_t now = ++cache->clock;
+        if (hit) {
+            hit->used = now;
+            ++cache->hits;
+            HIP_TRY(ctx, launch_gather_slice(s, (const int32_t *)hit->val.p, (const uint32_t *)hit->d2.p,
+                                             a.stream));
+            continue;
+        }
+        victim->full = false;
+        if ((st = ensure(ctx, victim->val, pix * 4)) != TNNI_OK) return st;
+        if ((st = ensure(ctx, victim->d2, pix * 4)) != TNNI_OK) return st;
+        s.out_labels = (int32_t *)victim->val.p;
+        s.out_d2 = (uint32_t *)victim->d2.p;
         if ((st = run_prepared(ctx, a, v, false, Launch::Direct)) != TNNI_OK) return st;
+        HIP_TRY(ctx, launch_label_values(s, (int32_t *)victim->val.p, pix, a.stream));
+        victim->full = true;
+        victim->time_bits = key;
+        victim->used = now;
+        ++cache->misses;
     }
     return TNNI_OK;
 }
@@ -988,12 +1082,73 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
 
 extern "C" {
 
+tnni_status tnni_time_cache_create(tnni_ctx *ctx, tnni_time_cache **out)
+{
+    if (!ctx || !out) return fail(ctx, TNNI_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    tnni_time_cache *c = new (std::nothrow) tnni_time_cache;
+    if (!c) return fail(ctx, TNNI_E_NOMEM, "time cache");
+    c->ctx = ctx;
+    ctx->caches.push_back(c);
+    *out = c;
+    return TNNI_OK;
+}
+
+void tnni_time_cache_destroy(tnni_time_cache *cache)
+{
+    if (!cache) return;
+    if (tnni_ctx *ctx = cache->ctx) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        for (auto &s : cache->slots) {
+            release(ctx, s.val);
+            release(ctx, s.d2);
+        }
+        ctx->caches.erase(std::remove(ctx->caches.begin(), ctx->caches.end(), cache), ctx->caches.end());
+    }
+    delete cache;
+}
+
+tnni_status tnni_time_cache_clear(tnni_time_cache *cache)
+{
+    if (!cache) return fail(nullptr, TNNI_E_INVALID, "null argument");
+    if (!cache->ctx) return TNNI_OK;
+    std::lock_guard<std::mutex> lk(cache->ctx->mu);
+    for (auto &s : cache->slots) s.full = false;
+    cache->keyed = false;
+    return TNNI_OK;
+}
+
+tnni_status tnni_time_cache_stats(const tnni_time_cache *cache, uint64_t *hits, uint64_t *misses)
+{
+    if (!cache || !hits || !misses) return fail(nullptr, TNNI_E_INVALID, "null argument");
+    if (cache->ctx) {
+        std::lock_guard<std::mutex> lk(cache->ctx->mu);
+        *hits = cache->hits;
+        *misses = cache->misses;
+    } else {
+        *hits = cache->hits;
+        *misses = cache->misses;
+    }
+    return TNNI_OK;
+}
+
 tnni_status tnni_tile_run_time_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *dX,
                                    const int32_t *dY, const int32_t *dZ, const int32_t *dT,
                                    uint64_t n, float cur_time, float time_radius, float *d_out_h)
 {
+    return tnni_tile_run_time_cached_dev(ctx, nullptr, desc, dX, dY, dZ, dT, n, cur_time, time_radius,
+                                         d_out_h);
+}
+
+tnni_status tnni_tile_run_time_cached_dev(tnni_ctx *ctx, tnni_time_cache *cache,
+                                          const tnni_tile_desc *desc, const int32_t *dX,
+                                          const int32_t *dY, const int32_t *dZ, const int32_t *dT,
+                                          uint64_t n, float cur_time, float time_radius, float *d_out_h)
+{
     if (!ctx || !desc || !d_out_h || (n && (!dX || !dY || !dZ || !dT)))
         return fail(ctx, TNNI_E_INVALID, "null argument");
+    if (cache && cache->ctx != ctx) return fail(ctx, TNNI_E_INVALID, "the cache belongs to another context");
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     Prepared p;
@@ -1006,7 +1161,7 @@ tnni_status tnni_tile_run_time_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, co
     // with no points T is never read: any non-null pointer marks the time mode
     p.t.T = n ? dT : (const int32_t *)d_out_h;
     p.t.out_h = d_out_h;
-    if ((st = run_time_tile(ctx, ctx->main, p, cur_time, time_radius)) != TNNI_OK) return st;
+    if ((st = run_time_tile(ctx, ctx->main, p, cur_time, time_radius, cache, desc)) != TNNI_OK) return st;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->main.stream));
     return TNNI_OK;
 }
@@ -1015,8 +1170,17 @@ tnni_status tnni_tile_run_time(tnni_ctx *ctx, const tnni_tile_desc *desc, const 
                                const int32_t *Y, const int32_t *Z, const int32_t *T, uint64_t n,
                                float cur_time, float time_radius, float *out_h)
 {
+    return tnni_tile_run_time_cached(ctx, nullptr, desc, X, Y, Z, T, n, cur_time, time_radius, out_h);
+}
+
+tnni_status tnni_tile_run_time_cached(tnni_ctx *ctx, tnni_time_cache *cache, const tnni_tile_desc *desc,
+                                      const int32_t *X, const int32_t *Y, const int32_t *Z,
+                                      const int32_t *T, uint64_t n, float cur_time, float time_radius,
+                                      float *out_h)
+{
     if (!ctx || !desc || !out_h || (n && (!X || !Y || !Z || !T)))
         return fail(ctx, TNNI_E_INVALID, "null argument");
+    if (cache && cache->ctx != ctx) return fail(ctx, TNNI_E_INVALID, "the cache belongs to another context");
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     Prepared p;
@@ -1041,7 +1205,7 @@ tnni_status tnni_tile_run_time(tnni_ctx *ctx, const tnni_tile_desc *desc, const 
     p.t.Z = (const int32_t *)a.sz.p;
     p.t.T = n ? (const int32_t *)a.st.p : (const int32_t *)a.sh.p;  // non-null: time mode
     p.t.out_h = (float *)a.sh.p;
-    if ((st = run_time_tile(ctx, a, p, cur_time, time_radius)) != TNNI_OK) return st;
+    if ((st = run_time_tile(ctx, a, p, cur_time, time_radius, cache, desc)) != TNNI_OK) return st;
     HIP_TRY(ctx, hipMemcpyAsync(out_h, a.sh.p, cells * 4, hipMemcpyDeviceToHost, a.stream));
     HIP_TRY(ctx, hipStreamSynchronize(a.stream));
     return TNNI_OK;

@@ -175,6 +175,12 @@ hipError_t enqueue_pipeline(const DevBuffers &b, const LaunchCfg &cfg,
                             hipStream_t stream, hipEvent_t ev_main0,
                             hipEvent_t ev_main1);
 
+// Space-time slice cache: a slice's label raster turned into site values
+// in place, and the query gather of a cached slice (values and squared
+// distances of its W x H pixels) into t.acc / t.out_h.
+hipError_t launch_label_values(const TileDev &t, int32_t *lab, uint64_t pix, hipStream_t s);
+hipError_t launch_gather_slice(const TileDev &t, const int32_t *val, const uint32_t *d2, hipStream_t s);
+
 // Largest workgroups per CU of the tile kernel at the given LDS size.
 int tile_kernel_occupancy(size_t lds_bytes);
 // LDS of the per-wave site stages of the tile kernel (in front of the

@@ -1046,7 +1046,84 @@ hipError_t launch_tiles(const DevBuffers &d, const LaunchCfg &cfg, hipStream_t s
     return hipGetLastError();
 }
 
+// ---- space-time slice cache ---------------------------------------------
+
+// A slice's label raster (written by k_tiles) turned into the labels' site
+// values in place (0 where no site covers the pixel), so that the slice
+// can be gathered again in later calls whose point arrays index
+// differently.
+__global__ void __launch_bounds__(kBlock)
+k_label_values(const TileDev t, int32_t *__restrict__ lab, uint64_t pix)
+{
+    for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < pix; p += (uint64_t)gridDim.x * kBlock) {
+        const int32_t l = lab[p];
+        lab[p] = l < 0 ? 0 : site_value(t, t.Z[l]);
+    }
+}
+
+// The query gather of one cached slice (values and squared distances of
+// its pixels in HBM): the cell loop of process_block's gather, in the same
+// order of pixels, so the sums are the same bit for bit.
+template <int ALG, int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_gather_slice(const TileDev t, const int32_t *__restrict__ val, const uint32_t *__restrict__ d2r)
+{
+    const int qa = query_lo(t), qb = query_hi(t);
+    const int64_t cells = (int64_t)t.nrows * t.ncols;
+    for (int64_t cell = (int64_t)blockIdx.x * kBlock + threadIdx.x; cell < cells;
+         cell += (int64_t)gridDim.x * kBlock) {
+        const int j = (int)(cell / t.ncols), i = (int)(cell - (int64_t)j * t.ncols);
+        const int ma = max(qa, -i * t.scale), mb = min(qb, t.W - 1 - i * t.scale);
+        const int na = max(qa, -j * t.scale), nb = min(qb, t.H - 1 - j * t.scale);
+        Acc<MODE> acc;
+        if (t.acc_load) acc.load(t.acc[cell]);
+        for (int n = na; n <= nb; ++n) {
+            const uint32_t uy = (uint32_t)(t.y0 + n * t.ps_u);
+            const int64_t row = (int64_t)(j * t.scale + n) * t.W + i * t.scale;
+            for (int m = ma; m <= mb; ++m) {
+                const uint32_t ux = (uint32_t)(t.x0 + m * t.ps_u);
+                const uint32_t dq = ux * ux + uy * uy;
+                const uint32_t d2 = d2r[row + m];
+                bool steal = d2 != kNoD2 && dq + t.jj2 <= d2;
+                if (ALG == TNNI_ALG_CONES) steal = steal && dq <= t.Rq2;
+                if (steal) acc.add(val[row + m]);
+            }
+        }
+        if (t.acc_final) t.out_h[cell] = acc.result(t);
+        else t.acc[cell] = acc.save();
+    }
+}
+
+template <int ALG, int MODE>
+hipError_t launch_gather_slice_t(const TileDev &t, const int32_t *val, const uint32_t *d2, hipStream_t s)
+{
+    const int64_t cells = (int64_t)t.nrows * t.ncols;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((cells + kBlock - 1) / kBlock, 8192));
+    hipLaunchKernelGGL((k_gather_slice<ALG, MODE>), dim3(grid), dim3(kBlock), 0, s, t, val, d2);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_label_values(const TileDev &t, int32_t *lab, uint64_t pix, hipStream_t s)
+{
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((pix + kBlock - 1) / kBlock, 8192));
+    hipLaunchKernelGGL(k_label_values, dim3(grid), dim3(kBlock), 0, s, t, lab, pix);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_slice(const TileDev &t, const int32_t *val, const uint32_t *d2, hipStream_t s)
+{
+    switch (t.algorithm * 3 + t.mode) {
+    case 0: return launch_gather_slice_t<0, 0>(t, val, d2, s);
+    case 1: return launch_gather_slice_t<0, 1>(t, val, d2, s);
+    case 2: return launch_gather_slice_t<0, 2>(t, val, d2, s);
+    case 3: return launch_gather_slice_t<1, 0>(t, val, d2, s);
+    case 4: return launch_gather_slice_t<1, 1>(t, val, d2, s);
+    case 5: return launch_gather_slice_t<1, 2>(t, val, d2, s);
+    default: return hipErrorInvalidValue;
+    }
+}
 
 hipError_t enqueue_pipeline(const DevBuffers &d, const LaunchCfg &cfg, hipStream_t s,
                             hipEvent_t ev0, hipEvent_t ev1)

@@ -44,6 +44,7 @@ int main(int argc, char **argv)
     wp.ncols = (int)num("wp_ncols", s.ncols);
     const int slices = (int)num("slices", 1);
     const float time_radius = (float)num("time_radius", 0);  // space-time when > 0
+    const bool delta = num("delta", 1) != 0;                  // useDelta (0: Delta^2)
     if (kv.count("accum")) nni::setAccumulationMode((int)num("accum", 0));
 
     tpie::ami::stream<nni::Point4D> in;
@@ -57,7 +58,7 @@ int main(int argc, char **argv)
     if (nni::init(s) == 0) return 4;
     tpie::ami::stream<nni::OutPoint> out;
     for (int t = 0; t < slices; ++t) {
-        nni::performNNI(wp, out, in, time_radius, (float)t, true);
+        nni::performNNI(wp, out, in, time_radius, (float)t, delta);
         if (nni::lastStatus() != 0) return 5;
     }
     nni::printTimes();

@@ -75,3 +75,25 @@ def test_perform_nni_matches_oracle(tmp_path, grid, n, cs, scale, slices, gox, t
     assert out.shape == exp.shape
     assert np.array_equal(out["i"], exp["i"]) and np.array_equal(out["j"], exp["j"])
     assert np.array_equal(out["h"].view(np.uint32), exp["h"].view(np.uint32))
+
+
+def test_delta_and_delta_squared_agree(tmp_path):
+    """performNNI with useDelta (slices kept between calls) and without
+    (Delta^2: every slice searched again) write the same cells, over six
+    calls at time radius 2 on one stream."""
+    grid, n = 40, 5000
+    tile = S.make_tile(77, n, grid, cell_size=1.0, kind="clustered", origin=(500.0, 800.0))
+    p = tile.params
+    rng = np.random.default_rng(77)
+    pts = np.zeros(tile.n, P4)
+    pts["x"] = tile.X * 0.01 + p.las_offset[0]
+    pts["y"] = tile.Y * 0.01 + p.las_offset[1]
+    pts["z"] = tile.Z * 0.01
+    pts["t"] = np.sort(rng.integers(0, 8, tile.n)).astype(np.int32)
+    args = dict(cell_size=1.0, nrows=grid, ncols=grid, origin_x=p.origin_x, origin_y=p.origin_y,
+                slices=6, time_radius=2.0)
+    a = _run_api(tmp_path, pts, dict(args, delta=1))
+    b = _run_api(tmp_path, pts, dict(args, delta=0))
+    assert a.shape == (6 * grid * grid,)
+    assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    assert (a["h"] != T.NODATA).mean() > 0.5
