@@ -231,8 +231,8 @@ tnni_status tnni_tile_run_time_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, co
  * nni.cpp:321-359, one buffer per slice of a call, the oldest reused,
  * getMinIndex nni.cpp:293-300).
  * Here a cache holds, per slice time, the site value and squared distance
- * of every pixel of the tile (8 bytes per pixel per slice, up to
- * 2 (int)r + 1 slices, least recently used reused); a slice found there is
+ * of every pixel of the tile (8 bytes per pixel per slice, up to its slot
+ * count, least recently used reused); a slice found there is
  * gathered without a search.  The result is bit for bit that of the
  * uncached call.  The caller keeps each slice's sites the same, and in the
  * same relative order, between the calls that use one cache (a time-sorted
@@ -241,7 +241,10 @@ tnni_status tnni_tile_run_time_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, co
  * one context and is used under its lock; destroy caches before their
  * context (a cache outliving it holds nothing and only accepts destroy). */
 typedef struct tnni_time_cache tnni_time_cache;
-tnni_status tnni_time_cache_create(tnni_ctx *ctx, tnni_time_cache **out);
+/* slots <= 0: one slot per slice of a call (2 (int)r + 1, the reference's
+ * numRB); more slots keep slice times of several phases (curTime stepped by
+ * 0.5 alternates two).  At most 4096. */
+tnni_status tnni_time_cache_create(tnni_ctx *ctx, int slots, tnni_time_cache **out);
 void tnni_time_cache_destroy(tnni_time_cache *cache);
 /* Empty the cache, keeping its device memory (nni::clearRenderbuffers,
  * nni.cpp:362-379). */

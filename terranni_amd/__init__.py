@@ -202,7 +202,7 @@ def lib():
                                      ctypes.c_float, ctypes.c_float, P]
     L.tnni_tile_run_time_dev.argtypes = [P, ctypes.POINTER(TileDesc), P, P, P, P, ctypes.c_uint64,
                                          ctypes.c_float, ctypes.c_float, P]
-    L.tnni_time_cache_create.argtypes = [P, ctypes.POINTER(P)]
+    L.tnni_time_cache_create.argtypes = [P, ctypes.c_int, ctypes.POINTER(P)]
     L.tnni_time_cache_destroy.argtypes = [P]
     L.tnni_time_cache_destroy.restype = None
     L.tnni_time_cache_clear.argtypes = [P]
@@ -422,10 +422,11 @@ class Context:
                                                       dX.ptr, dY.ptr, dZ.ptr, dT.ptr, n, cur_time,
                                                       time_radius, d_h.ptr))
 
-    def time_cache(self) -> "TimeCache":
-        """A space-time slice cache on this context (tnni_time_cache)."""
+    def time_cache(self, slots: int = 0) -> "TimeCache":
+        """A space-time slice cache on this context (tnni_time_cache); slots
+        0: one per slice of a call."""
         h = ctypes.c_void_p()
-        self._chk(lib().tnni_time_cache_create(self.h, ctypes.byref(h)))
+        self._chk(lib().tnni_time_cache_create(self.h, slots, ctypes.byref(h)))
         return TimeCache(h, self)
 
     # ---- autotuning ----

@@ -533,10 +533,115 @@ Meta read_meta(const std::string &dir)
 
 // One package of a row in flight: its points of the slice and its raster.
 struct Flight {
-    std::vector<int32_t> X, Y, Z, T;
+    std::vector<int32_t> X, Y, Z;
     std::vector<float> h;
     uint64_t ticket = 0;
 };
+
+// The grid rows of one row of packages, top down, from the packages' cell
+// rasters (h[col], wp_nr x wp_nc, bottom row first), appended to w.
+void write_band(RasterOut &w, const Layout &L, int row, const std::vector<const float *> &h,
+                std::vector<float> &band)
+{
+    const int j_hi = std::min(L.nrrows, (row + 1) * L.wp_nr) - 1, j_lo = row * L.wp_nr;
+    const int nb = j_hi - j_lo + 1;
+    band.assign((size_t)std::max(nb, 0) * L.nrcols, -9999.0f);
+    for (int jj = 0; jj < nb; ++jj) {
+        const int j = j_hi - jj;       // global row
+        const int lj = j - j_lo;       // row inside the packages
+        float *dst = band.data() + (size_t)jj * L.nrcols;
+        for (int col = 0; col < L.cols; ++col) {
+            const int i0 = col * L.wp_nc, n = std::min(L.wp_nc, L.nrcols - i0);
+            if (n > 0) std::memcpy(dst + i0, h[(size_t)col] + (size_t)lj * L.wp_nc, (size_t)n * 4);
+        }
+    }
+    if (nb > 0) w.write_rows(band.data(), nb);
+}
+
+std::vector<RawPoint> read_package(const Options &o, int row, int col)
+{
+    tpie::ami::stream<RawPoint> st(wp_file(o.wp_path, row, col, ".raw"), tpie::ami::READ_STREAM);
+    if (!st.is_valid()) die("missing work package " + wp_file(o.wp_path, row, col, ".raw"));
+    std::vector<RawPoint> all((size_t)st.stream_len());
+    TPIE_OS_SIZE_T got = all.size();
+    if (!all.empty() && st.read_array(all.data(), &got) != tpie::ami::NO_ERROR)
+        die("cannot read " + wp_file(o.wp_path, row, col, ".raw"));
+    return all;
+}
+
+// Space-time (timeRadius > 0) on the batched engine's packages: rows of
+// packages outer and slice times inner, so that each package's slice cache
+// (the Delta algorithm, nni.cpp:902-921) still holds the slices that one
+// call shares with the next; --delta-squared searches every slice.  Each
+// call takes the points within 2r + 1 of its time (every site of its
+// slices), one run of the time-sorted package.  One output raster per slice
+// time is open throughout; each gets the band of a package row in turn.
+void run_batched_time(tnni_ctx *ctx, const Options &o, const Layout &L, const tnni_tile_desc &base,
+                      const std::vector<float> &slices, double &nni_s, double &out_s)
+{
+    const float tr = (float)o.time_radius;
+    const size_t cells = (size_t)L.wp_nr * L.wp_nc;
+    std::vector<std::unique_ptr<RasterOut>> outs;
+    for (float t : slices)
+        outs.emplace_back(new RasterOut(slice_name(o.output, t), o.format, L.nrrows, L.nrcols, L.minx, L.miny, L.cs));
+    // slots: the slices of a call for each phase of the slice times
+    // (curTime stepped by 0.5 alternates two), at most 4 phases
+    std::vector<float> phases;
+    for (float t : slices) {
+        const float f = t - std::floor(t);
+        if (std::find(phases.begin(), phases.end(), f) == phases.end()) phases.push_back(f);
+    }
+    const int per_call = 2 * (int)tr + 1;
+    const int slots = per_call * (int)std::min<size_t>(phases.size(), 4);
+    std::vector<tnni_time_cache *> caches((size_t)L.cols, nullptr);
+    if (!o.delta_squared)
+        for (auto &c : caches)
+            if (tnni_time_cache_create(ctx, slots, &c) != TNNI_OK) die(std::string("cache: ") + tnni_last_error(ctx));
+    std::vector<std::vector<RawPoint>> pkg((size_t)L.cols);
+    std::vector<std::vector<float>> h((size_t)L.cols, std::vector<float>(cells));
+    std::vector<const float *> hp((size_t)L.cols);
+    for (int col = 0; col < L.cols; ++col) hp[(size_t)col] = h[(size_t)col].data();
+    std::vector<int32_t> X, Y, Z, T;
+    std::vector<float> band;
+    const double w = 2.0 * (double)tr + 1.0;
+    for (int row = L.rows - 1; row >= 0; --row) {
+        const double ts = wall();
+        for (int col = 0; col < L.cols; ++col) pkg[(size_t)col] = read_package(o, row, col);
+        for (size_t si = 0; si < slices.size(); ++si) {
+            const float t = slices[si];
+            for (int col = 0; col < L.cols; ++col) {
+                const std::vector<RawPoint> &all = pkg[(size_t)col];
+                size_t b = 0;
+                while (b < all.size() && (double)all[b].t - (double)t < -w) ++b;
+                size_t e = b;
+                while (e < all.size() && (double)all[e].t - (double)t <= w) ++e;
+                const size_t k = e - b;
+                X.resize(k);
+                Y.resize(k);
+                Z.resize(k);
+                T.resize(k);
+                for (size_t q = 0; q < k; ++q) {
+                    const RawPoint &p = all[b + q];
+                    X[q] = p.X;
+                    Y[q] = p.Y;
+                    Z[q] = p.Z;
+                    T[q] = p.t;
+                }
+                tnni_tile_desc d = base;
+                d.origin_x = L.origin_x(col);
+                d.origin_y = L.origin_y(row);
+                if (tnni_tile_run_time_cached(ctx, caches[(size_t)col], &d, X.data(), Y.data(), Z.data(), T.data(),
+                                              k, t, tr, h[(size_t)col].data()) != TNNI_OK)
+                    die(std::string("tile: ") + tnni_last_error(ctx));
+            }
+            const double tw = wall();
+            write_band(*outs[si], L, row, hp, band);
+            out_s += wall() - tw;
+        }
+        nni_s += wall() - ts;
+    }
+    for (tnni_time_cache *c : caches) tnni_time_cache_destroy(c);
+}
 
 void run_batched(tnni_ctx *ctx, const Options &o, const Layout &L, const nni::Settings &s,
                  const Meta &m, const std::vector<float> &slices, double &t_nni, double &t_out)
@@ -561,65 +666,47 @@ void run_batched(tnni_ctx *ctx, const Options &o, const Layout &L, const nni::Se
 
     double nni_s = 0, out_s = 0;
     const size_t cells = (size_t)L.wp_nr * L.wp_nc;
+    if (o.time_radius > 0) {
+        run_batched_time(ctx, o, L, base, slices, nni_s, out_s);
+        t_nni = nni_s - out_s;
+        t_out = out_s;
+        return;
+    }
     for (float t : slices) {
         RasterOut w(slice_name(o.output, t), o.format, L.nrrows, L.nrcols, L.minx, L.miny, L.cs);
-        const float tr = (float)o.time_radius;
+        const float tr = 0.0f;
         // submit every package of one row of packages
         auto submit_row = [&](int row) {
             std::vector<std::unique_ptr<Flight>> fl;
             for (int col = 0; col < L.cols; ++col) {
                 std::unique_ptr<Flight> f(new Flight());
-                tpie::ami::stream<RawPoint> st(wp_file(o.wp_path, row, col, ".raw"), tpie::ami::READ_STREAM);
-                if (!st.is_valid()) die("missing work package " + wp_file(o.wp_path, row, col, ".raw"));
                 // the whole package in one read; it is sorted by time, so a
                 // slice's points are one run of it
-                std::vector<RawPoint> all((size_t)st.stream_len());
-                TPIE_OS_SIZE_T got = all.size();
-                if (!all.empty() && st.read_array(all.data(), &got) != tpie::ami::NO_ERROR)
-                    die("cannot read " + wp_file(o.wp_path, row, col, ".raw"));
-                size_t b = 0, e = 0;
-                if (tr <= 0) {
-                    // the slice's points (nni.cpp:770 filter)
-                    while (b < all.size() && !((double)std::fabs(t - (float)all[b].t) <= (double)tr + 0.000001) &&
-                           !((float)all[b].t > t + tr))
-                        ++b;
-                    e = b;
-                    while (e < all.size() && (double)std::fabs(t - (float)all[e].t) <= (double)tr + 0.000001) ++e;
-                } else {
-                    // space-time: every point within 2r + 1 of t; the engine
-                    // keeps each slice's own
-                    const double w = 2.0 * (double)tr + 1.0;
-                    while (b < all.size() && (double)all[b].t - (double)t < -w) ++b;
-                    e = b;
-                    while (e < all.size() && (double)all[e].t - (double)t <= w) ++e;
-                }
+                const std::vector<RawPoint> all = read_package(o, row, col);
+                // the slice's points (nni.cpp:770 filter)
+                size_t b = 0;
+                while (b < all.size() && !((double)std::fabs(t - (float)all[b].t) <= (double)tr + 0.000001) &&
+                       !((float)all[b].t > t + tr))
+                    ++b;
+                size_t e = b;
+                while (e < all.size() && (double)std::fabs(t - (float)all[e].t) <= (double)tr + 0.000001) ++e;
                 const size_t k = e - b;
                 f->X.resize(k);
                 f->Y.resize(k);
                 f->Z.resize(k);
-                if (tr > 0) f->T.resize(k);
                 for (size_t q = 0; q < k; ++q) {
                     const RawPoint &p = all[b + q];
                     f->X[q] = p.X;
                     f->Y[q] = p.Y;
                     f->Z[q] = p.Z;
-                    if (tr > 0) f->T[q] = p.t;
                 }
                 f->h.resize(cells);
                 tnni_tile_desc d = base;
                 d.origin_x = L.origin_x(col);
                 d.origin_y = L.origin_y(row);
-                tnni_status stt;
-                if (tr <= 0) {
-                    stt = tnni_submit(ctx, &d, f->X.data(), f->Y.data(), f->Z.data(), f->X.size(),
-                                      f->h.data(), nullptr, nullptr, &f->ticket);
-                } else {
-                    // the slices of one package run in order on the GPU
-                    stt = tnni_tile_run_time(ctx, &d, f->X.data(), f->Y.data(), f->Z.data(), f->T.data(),
-                                             f->X.size(), t, tr, f->h.data());
-                    f->ticket = 0;
-                }
-                if (stt != TNNI_OK) die(std::string("tile: ") + tnni_last_error(ctx));
+                if (tnni_submit(ctx, &d, f->X.data(), f->Y.data(), f->Z.data(), f->X.size(), f->h.data(), nullptr,
+                                nullptr, &f->ticket) != TNNI_OK)
+                    die(std::string("tile: ") + tnni_last_error(ctx));
                 fl.push_back(std::move(f));
             }
             return fl;
@@ -631,23 +718,11 @@ void run_batched(tnni_ctx *ctx, const Options &o, const Layout &L, const nni::Se
             std::vector<std::unique_ptr<Flight>> next;
             if (row > 0) next = submit_row(row - 1);
             for (auto &f : pending)
-                if (f->ticket && tnni_wait(ctx, f->ticket, nullptr) != TNNI_OK)
-                    die(std::string("tile: ") + tnni_last_error(ctx));
+                if (tnni_wait(ctx, f->ticket, nullptr) != TNNI_OK) die(std::string("tile: ") + tnni_last_error(ctx));
             const double tw = wall();
-            // rows of this band inside the grid, top down
-            const int j_hi = std::min(L.nrrows, (row + 1) * L.wp_nr) - 1, j_lo = row * L.wp_nr;
-            const int nb = j_hi - j_lo + 1;
-            band.assign((size_t)std::max(nb, 0) * L.nrcols, -9999.0f);
-            for (int jj = 0; jj < nb; ++jj) {
-                const int j = j_hi - jj;       // global row
-                const int lj = j - j_lo;       // row inside the packages
-                float *dst = band.data() + (size_t)jj * L.nrcols;
-                for (int col = 0; col < L.cols; ++col) {
-                    const int i0 = col * L.wp_nc, n = std::min(L.wp_nc, L.nrcols - i0);
-                    if (n > 0) std::memcpy(dst + i0, pending[(size_t)col]->h.data() + (size_t)lj * L.wp_nc, (size_t)n * 4);
-                }
-            }
-            if (nb > 0) w.write_rows(band.data(), nb);
+            std::vector<const float *> hp;
+            for (auto &f : pending) hp.push_back(f->h.data());
+            write_band(w, L, row, hp, band);
             out_s += wall() - tw;
             pending = std::move(next);
         }

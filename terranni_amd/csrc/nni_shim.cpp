@@ -214,7 +214,7 @@ void performNNI(const WorkPackage &wp, tpie::ami::stream<OutPoint> &out,
     std::vector<float> h((size_t)wp.nrows * wp.ncols);
     // useDelta keeps the slices for the next curTime; Delta^2 searches
     // every slice again (nni.cpp:915-917).  The results are the same.
-    if (space_time && useDelta && !g_cache && (st = tnni_time_cache_create(g_ctx, &g_cache)) != TNNI_OK) {
+    if (space_time && useDelta && !g_cache && (st = tnni_time_cache_create(g_ctx, 0, &g_cache)) != TNNI_OK) {
         g_t_tile.stop();
         set_error(st, tnni_last_error(g_ctx));
         return;

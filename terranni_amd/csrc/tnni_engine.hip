@@ -169,6 +169,7 @@ struct tnni_time_cache {
     bool keyed = false;       // desc and time_radius name the cached tile
     tnni_tile_desc desc{};
     float time_radius = 0.0f;
+    int max_slots = 0;  // 0: one per slice of a call
     struct Slot {
         bool full = false;
         uint32_t time_bits = 0;  // the slice time as float bits (+0 for -0)
@@ -1021,8 +1022,8 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
             cache->time_radius = time_radius;
             cache->keyed = true;
         }
-        // one slot per slice of a call (numRB, nni.cpp:902)
-        const size_t want = (size_t)(j1 - j0 + 1);
+        // by default one slot per slice of a call (numRB, nni.cpp:902)
+        const size_t want = cache->max_slots > 0 ? (size_t)cache->max_slots : (size_t)(j1 - j0 + 1);
         while (cache->slots.size() > want) {
             release(ctx, cache->slots.back().val);
             release(ctx, cache->slots.back().d2);
@@ -1082,13 +1083,15 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
 
 extern "C" {
 
-tnni_status tnni_time_cache_create(tnni_ctx *ctx, tnni_time_cache **out)
+tnni_status tnni_time_cache_create(tnni_ctx *ctx, int slots, tnni_time_cache **out)
 {
     if (!ctx || !out) return fail(ctx, TNNI_E_INVALID, "null argument");
+    if (slots > 4096) return fail(ctx, TNNI_E_INVALID, "at most 4096 slots");
     std::lock_guard<std::mutex> lk(ctx->mu);
     tnni_time_cache *c = new (std::nothrow) tnni_time_cache;
     if (!c) return fail(ctx, TNNI_E_NOMEM, "time cache");
     c->ctx = ctx;
+    c->max_slots = std::max(slots, 0);
     ctx->caches.push_back(c);
     *out = c;
     return TNNI_OK;

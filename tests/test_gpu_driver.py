@@ -136,3 +136,35 @@ def test_driver_space_time_both_engines(tmp_path):
     assert np.array_equal(valid, hp != T.NODATA)
     rel = np.abs(hp[valid] - ref["h64"][valid]) / np.maximum(np.abs(ref["h64"][valid]), REL_FLOOR)
     assert rel.max() <= 1e-6
+
+
+def test_driver_space_time_series_delta_and_delta_squared(tmp_path):
+    """A time series at radius 1 stepped by 0.5 (slices 0, 0.5, ..., 4): the
+    batched engine with its per-package slice caches (Delta) writes the same
+    grids bit for bit as with --delta-squared (every slice searched), and
+    the grids at 1 and 1.5 equal the oracle's whole-grid tile."""
+    (tmp_path / "las").mkdir()
+    strips = write_scene(str(tmp_path / "las"))
+    X = np.concatenate([s[0] for s in strips])
+    Y = np.concatenate([s[1] for s in strips])
+    Z = np.concatenate([s[2] for s in strips])
+    times = np.concatenate([s[3] for s in strips]).astype(np.int32)
+    o = np.argsort(times, kind="stable")
+    X, Y, Z, times = X[o], Y[o], Z[o], times[o]
+    args = ["--time-start=0", "--time-length=5", "--time-step=0.5", "--time-radius=1", "--format=flt"]
+    names = ["%#g" % (0.5 * k) for k in range(9)]  # ostream showpoint: 0.500000, 1.00000
+    _say("driver batched time series, Delta")
+    _, out = run_driver(tmp_path, "batched", args)
+    delta = {n: read_flt(out, n) for n in names}
+    _say("driver batched time series, Delta^2")
+    _, out = run_driver(tmp_path, "batched", args + ["--delta-squared", "--wp-computed"])
+    for n in names:
+        assert np.array_equal(read_flt(out, n).view(np.uint32), delta[n].view(np.uint32)), n
+    g = GRID
+    p = T.TileParams(nrows=g["rows"], ncols=g["cols"], cell_size=g["cs"], scale=g["scale"],
+                     site_radius=g["sr"], origin_x=OFFSET[0], origin_y=OFFSET[1],
+                     las_scale=S.LAS_SCALE, las_offset=OFFSET)
+    for t in (1.0, 1.5):
+        ref = O.run_time(p, X, Y, Z, times, t, 1.0)
+        assert np.array_equal(delta["%#g" % t].view(np.uint32), ref["h_fixed"].view(np.uint32)), t
+    assert (delta["4.00000"] == T.NODATA).mean() < 1.0

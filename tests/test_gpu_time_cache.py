@@ -142,3 +142,23 @@ def test_cache_from_another_context_is_refused(ctx):
             ctx.tile_run_time(p, X, Y, Z, Tm, 2001.0, 1.0, cache=c)
         assert e.value.status == -1
         c.close()
+
+
+def test_cache_with_slots_for_half_steps(ctx):
+    """cur_time stepped by 0.5 alternates two phases of slice times; a cache
+    of 2 x 5 slots keeps both, so every call after the first two searches
+    one slice."""
+    p, X, Y, Z, Tm = _stream(51, 8000, 40, T.ALG_PLANES, T.ACC_FP32_2SUM)
+    with ctx.time_cache(slots=10) as c, ctx.time_cache() as d:
+        for k in range(8):
+            cur = 2001.0 + 0.5 * k
+            w = _window(Tm, cur, 2.0)
+            h = ctx.tile_run_time(p, X[w], Y[w], Z[w], Tm[w], cur, 2.0, cache=c)
+            g = ctx.tile_run_time(p, X[w], Y[w], Z[w], Tm[w], cur, 2.0, cache=d)
+            u = ctx.tile_run_time(p, X[w], Y[w], Z[w], Tm[w], cur, 2.0)
+            assert _same(h, u) and _same(g, u), cur
+        assert c.stats() == (6 * 4, 10 + 6)
+        # 5 slots: each phase evicts the other's slices
+        assert d.stats() == (0, 40)
+    with pytest.raises(T.TnniError):
+        ctx.time_cache(slots=5000)
