@@ -49,6 +49,41 @@ def cpu_baseline(tile, budget_s: float = 12.0):
     return tile.cells / per, threads, per, reps, ref
 
 
+def cpu_single_thread(tile):
+    """The same oracle on one thread, one run: cells/s."""
+    import pyoracle as O
+    t0 = time.perf_counter()
+    O.run(tile.params, tile.X, tile.Y, tile.Z, nthreads=1)
+    return tile.cells / (time.perf_counter() - t0)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def copy_bandwidth(ctx, mib: int = 1024, reps: int = 10):
+    """STREAM-like device copy (hipMemcpy device to device, one GPU): read +
+    write bytes over the best of reps timed copies, GB/s."""
+    nb = mib << 20
+    a, c = ctx.alloc((nb,), np.uint8), ctx.alloc((nb,), np.uint8)
+    ctx.copy(c, a)
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ctx.copy(c, a)  # ends in a stream sync
+        best = min(best, time.perf_counter() - t0)
+    a.free()
+    c.free()
+    return 2.0 * nb / best / 1e9
+
+
 def bench_batch(T, S, ctx, reps: int = 3, tune: bool = True):
     """The 256 uneven tiles (log-normal point counts, grids 256^2..2048^2,
     half uniform, half clustered): the plain form (one tile per launch
@@ -623,6 +658,7 @@ def main():
     for _ in range(3):
         ctx.tile_run(tile.params, tile.X, tile.Y, tile.Z)
     pcie_cells_s = 3 * tile.cells / (time.perf_counter() - t_h)
+    copy_gbs = copy_bandwidth(ctx)
     if args.copy_probe_mb:
         nb = args.copy_probe_mb << 20
         a, c = ctx.alloc((nb,), np.uint8), ctx.alloc((nb,), np.uint8)
@@ -659,6 +695,7 @@ def main():
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
+                     "peak_measured_copy": copy_gbs,
                      "kernel": "k_tiles", "kernel_ms": k_ms, "alg_bytes": alg_bytes},
     }
     valu = load_valu()
@@ -690,7 +727,9 @@ def main():
         c_cells, threads, per, reps, ref = cpu_baseline(tile)
         out["cpu_baseline"] = {"value": c_cells, "unit": "cells/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"the same tile, full, {reps} run(s), {per:.2f} s each"}
+                               "sample": f"the same tile, full, {reps} run(s), {per:.2f} s each",
+                               "single_thread_value": cpu_single_thread(tile),
+                               "cpu_model": cpu_model()}
         valid = ref["h64"] != T.NODATA
         # relative to the FP64 mean, floored at 1 micrometre: an exact integer
         # sum of 0 (cancellation) leaves FP64 rounding noise in the oracle mean
@@ -700,7 +739,8 @@ def main():
                            "bits_differ_from_oracle": int(np.count_nonzero(
                                h.view(np.uint32) != ref["h_fixed"].view(np.uint32))),
                            "nodata_match": bool(np.array_equal(valid, h != T.NODATA)),
-                           "stolen_pairs": ref["S"], "candidate_pairs": ref["K"]}
+                           "stolen_pairs": ref["S"], "candidate_pairs": ref["K"],
+                           "stolen_pairs_per_s": ref["S"] / (elapsed / args.steps)}
     ctx.close()
     print(json.dumps(out))
 
