@@ -422,10 +422,11 @@ def bench_spacetime(T, S, args):
     reference's time loop calls performNNI (nni.cpp:870-944): each call at
     cur_time takes the points of the time-sorted stream within 2r + 1 of it
     (every site of its 2 (int)r + 1 slices), device-resident, as a window of
-    one uploaded array.  Delta keeps each slice's site values and squared
-    distances in the slice cache and searches only the slice that is new;
-    Delta^2 searches every slice.  Both must give the same heights bit for
-    bit; the CPU oracle on one call is the CPU baseline and the check."""
+    one uploaded array.  Delta keeps per point time the 2-D Voronoi layer
+    in the space-time cache, combines each slice from the layers within its
+    radius and searches only the layer that is new; Delta^2 searches every
+    slice.  Both must give the same heights bit for bit; the CPU oracle on
+    one call is the CPU baseline and the check."""
     from dataclasses import replace
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
@@ -480,14 +481,14 @@ def bench_spacetime(T, S, args):
                     "window_points_per_call": pts / args.steps}
         if cache is not None:
             hits, misses = cache.stats()
-            res[tag].update({"slices_gathered": hits, "slices_searched": misses})
+            res[tag].update({"layers_reused": hits, "layers_searched": misses})
             cache.close()
         heights[tag] = hs
         log(f"space-time {tag}: {res[tag]}")
     same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
                for a, b in zip(heights["delta"], heights["delta_squared"]))
     out = {
-        "metric": f"DEM cells/s (space-time tile {p.ncols}^2, time radius {r:g}, Delta slice cache)",
+        "metric": f"DEM cells/s (space-time tile {p.ncols}^2, time radius {r:g}, Delta cache)",
         "value": res["delta"]["cells_per_s"], "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": res["delta"]["ms_per_call"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int32",
@@ -575,7 +576,7 @@ def main():
                     help="single: the 4096^2 tile (default line, plus the batched and served "
                          "comparisons); batch: only the batched 256-tile form; stream: the "
                          "streamed-terrain driver pipeline (down-scaled); spacetime: one work "
-                         "package stepped through time, Delta slice cache against Delta^2")
+                         "package stepped through time, Delta cache against Delta^2")
     ap.add_argument("--time-radius", type=float, default=2.0, help="spacetime: time radius")
     ap.add_argument("--st-grid", type=int, default=1024, help="spacetime: tile side in cells")
     ap.add_argument("--st-points-per-year", type=int, default=400_000)

@@ -7,9 +7,10 @@
  * at :422, cleanup at :721) compile and run unchanged.  The OpenGL render
  * buffers are gone: setRenderbuffers / setupRun are kept as no-ops for
  * source compatibility.  performNNI with timeRadius > 0 and useDelta keeps
- * each slice's Voronoi values in the engine's slice cache (the Delta render
- * buffers, nni.cpp:281-379), which clearRenderbuffers empties; without
- * useDelta (Delta^2) every slice is searched.  Both give the same heights.
+ * per point time the 2-D Voronoi layer in the engine's space-time cache (in
+ * place of the Delta render buffers, nni.cpp:281-379) and builds each slice
+ * from it; clearRenderbuffers empties it; without useDelta (Delta^2) every
+ * slice is searched.  Both give the same heights.
  */
 #ifndef LIBNNI_H
 #define LIBNNI_H

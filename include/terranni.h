@@ -225,31 +225,36 @@ tnni_status tnni_tile_run_time_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, co
                                    const int32_t *dY, const int32_t *dZ, const int32_t *dT,
                                    uint64_t n, float cur_time, float time_radius, float *d_out_h);
 
-/* Space-time slice cache: performNNI with useDelta (the Delta algorithm,
+/* Space-time cache: performNNI with useDelta (the Delta algorithm,
  * nni.cpp:902-921) keeps the Voronoi diagram of each slice time it has drawn
  * and draws only the slices it does not hold (switchRenderbuffers,
  * nni.cpp:321-359, one buffer per slice of a call, the oldest reused,
  * getMinIndex nni.cpp:293-300).
- * Here a cache holds, per slice time, the site value and squared distance
- * of every pixel of the tile (8 bytes per pixel per slice, up to its slot
- * count, least recently used reused); a slice found there is
- * gathered without a search.  The result is bit for bit that of the
- * uncached call.  The caller keeps each slice's sites the same, and in the
- * same relative order, between the calls that use one cache (a time-sorted
- * stream read in windows around cur_time does); a call with another tile
- * descriptor or time radius empties the cache first.  A cache belongs to
+ * Here a cache holds per-time layers instead: for each point time T, the
+ * 2-D Voronoi of the points of time T (site value and squared distance of
+ * every pixel, 8 bytes per pixel per layer).  A slice's nearest site is the
+ * least (d2 + dt2) over the layers within its radius, so a slice is a
+ * combine of layers plus the gather, and a layer serves every slice and
+ * every later cur_time that reaches its time; only the layers a call lacks
+ * are searched (one per call when cur_time steps by 1, or by 0.5).  The
+ * result is bit for bit that of the uncached call.  The points must be
+ * sorted by time (a call whose points are not runs without the cache,
+ * searching every slice); the caller keeps the points of each time the
+ * same, and in the same relative order, between the calls that use one
+ * cache (a time-sorted stream read in windows around cur_time does); a call
+ * with another tile descriptor empties the cache first.  A cache belongs to
  * one context and is used under its lock; destroy caches before their
  * context (a cache outliving it holds nothing and only accepts destroy). */
 typedef struct tnni_time_cache tnni_time_cache;
-/* slots <= 0: one slot per slice of a call (2 (int)r + 1, the reference's
- * numRB); more slots keep slice times of several phases (curTime stepped by
- * 0.5 alternates two).  At most 4096. */
+/* slots <= 0: as many layers as a call reaches (9 at r = 2: the times
+ * within 2r of cur_time); more slots keep layers longer (a call reaching
+ * more layers than the slots still gets them).  At most 4096. */
 tnni_status tnni_time_cache_create(tnni_ctx *ctx, int slots, tnni_time_cache **out);
 void tnni_time_cache_destroy(tnni_time_cache *cache);
 /* Empty the cache, keeping its device memory (nni::clearRenderbuffers,
  * nni.cpp:362-379). */
 tnni_status tnni_time_cache_clear(tnni_time_cache *cache);
-/* Slices gathered from the cache and slices searched, since creation. */
+/* Layers taken from the cache and layers searched, since creation. */
 tnni_status tnni_time_cache_stats(const tnni_time_cache *cache, uint64_t *hits, uint64_t *misses);
 /* tnni_tile_run_time(_dev) through a cache (NULL: uncached). */
 tnni_status tnni_tile_run_time_cached(tnni_ctx *ctx, tnni_time_cache *cache, const tnni_tile_desc *desc,

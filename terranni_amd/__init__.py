@@ -278,9 +278,10 @@ class DevArray:
 
 
 class TimeCache:
-    """Per slice time, the site values and squared distances of a space-time
-    tile's pixels (tnni_time_cache; the Delta render-buffer cache of
-    nni.cpp:281-379).  Close it before its context."""
+    """Per point time, the 2-D Voronoi layer (site values and squared
+    distances of the pixels) of a space-time tile, from which its slices are
+    combined (tnni_time_cache; what the Delta render-buffer cache of
+    nni.cpp:281-379 saves).  Close it before its context."""
 
     def __init__(self, h, ctx: "Context"):
         self.h = h
@@ -290,7 +291,7 @@ class TimeCache:
         _check(lib().tnni_time_cache_clear(self.h))
 
     def stats(self):
-        """(slices gathered from the cache, slices searched)."""
+        """(layers taken from the cache, layers searched)."""
         hits, misses = ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().tnni_time_cache_stats(self.h, ctypes.byref(hits), ctypes.byref(misses)))
         return hits.value, misses.value
@@ -403,7 +404,8 @@ class Context:
                       cache: "TimeCache" = None):
         """Heights of a space-time tile (host buffers): slices cur_time + j,
         j = -(int)r..r, sites within the time radius at their time distance.
-        With a cache, slices it holds are gathered without a search."""
+        With a cache (and points sorted by time), the slices are combined
+        from per-time layers, searching only the layers it lacks."""
         X = np.ascontiguousarray(X, dtype=np.int32)
         Y = np.ascontiguousarray(Y, dtype=np.int32)
         Z = np.ascontiguousarray(Z, dtype=np.int32)
@@ -423,8 +425,8 @@ class Context:
                                                       time_radius, d_h.ptr))
 
     def time_cache(self, slots: int = 0) -> "TimeCache":
-        """A space-time slice cache on this context (tnni_time_cache); slots
-        0: one per slice of a call."""
+        """A space-time cache on this context (tnni_time_cache); slots 0: the
+        layers of one call."""
         h = ctypes.c_void_p()
         self._chk(lib().tnni_time_cache_create(self.h, slots, ctypes.byref(h)))
         return TimeCache(h, self)

@@ -570,9 +570,10 @@ std::vector<RawPoint> read_package(const Options &o, int row, int col)
 }
 
 // Space-time (timeRadius > 0) on the batched engine's packages: rows of
-// packages outer and slice times inner, so that each package's slice cache
-// (the Delta algorithm, nni.cpp:902-921) still holds the slices that one
-// call shares with the next; --delta-squared searches every slice.  Each
+// packages outer and slice times inner, so that each package's space-time
+// cache (the Delta algorithm, nni.cpp:902-921) still holds the per-time
+// Voronoi layers that one call shares with the next; --delta-squared
+// searches every slice.  Each
 // call takes the points within 2r + 1 of its time (every site of its
 // slices), one run of the time-sorted package.  One output raster per slice
 // time is open throughout; each gets the band of a package row in turn.
@@ -584,19 +585,10 @@ void run_batched_time(tnni_ctx *ctx, const Options &o, const Layout &L, const tn
     std::vector<std::unique_ptr<RasterOut>> outs;
     for (float t : slices)
         outs.emplace_back(new RasterOut(slice_name(o.output, t), o.format, L.nrrows, L.nrcols, L.minx, L.miny, L.cs));
-    // slots: the slices of a call for each phase of the slice times
-    // (curTime stepped by 0.5 alternates two), at most 4 phases
-    std::vector<float> phases;
-    for (float t : slices) {
-        const float f = t - std::floor(t);
-        if (std::find(phases.begin(), phases.end(), f) == phases.end()) phases.push_back(f);
-    }
-    const int per_call = 2 * (int)tr + 1;
-    const int slots = per_call * (int)std::min<size_t>(phases.size(), 4);
     std::vector<tnni_time_cache *> caches((size_t)L.cols, nullptr);
     if (!o.delta_squared)
         for (auto &c : caches)
-            if (tnni_time_cache_create(ctx, slots, &c) != TNNI_OK) die(std::string("cache: ") + tnni_last_error(ctx));
+            if (tnni_time_cache_create(ctx, 0, &c) != TNNI_OK) die(std::string("cache: ") + tnni_last_error(ctx));
     std::vector<std::vector<RawPoint>> pkg((size_t)L.cols);
     std::vector<std::vector<float>> h((size_t)L.cols, std::vector<float>(cells));
     std::vector<const float *> hp((size_t)L.cols);

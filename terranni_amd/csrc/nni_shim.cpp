@@ -147,8 +147,8 @@ void performNNI(const WorkPackage &wp, tpie::ami::stream<OutPoint> &out,
         // slices it drew, relying on its render-buffer cache for the rest,
         // nni.cpp:321-379,782-788.  Here every slice's sites are read again:
         // a slice's sites do not depend on curTime, and with useDelta the
-        // engine's slice cache gathers the slices it still holds without
-        // searching.)
+        // engine's space-time cache rebuilds the slices from the per-time
+        // layers it still holds without searching them.)
         const double w = 2.0 * (double)timeRadius + 1.0;
         TPIE_OS_OFFSET start = -1;
         for (;;) {

@@ -62,6 +62,7 @@ struct Arena {
     DevBuf tiles, chunks, batch, hist, partials, banded, sorted, offs, hdist, edist;
     DevBuf sx, sy, sz, sh, sl, sd;  // device staging of host-pointer calls
     DevBuf st, acc;                 // space-time: point times, per-cell sums
+    DevBuf cv, cd, flag;            // space-time: a slice combined from layers, sortedness flag
     HostBuf h_desc;                 // pinned descriptors
     HostBuf h_in, h_out;            // pinned staging of the served form
     hipGraphExec_t gexec = nullptr;
@@ -155,25 +156,28 @@ struct Slot {
 
 }  // namespace
 
-// Space-time slice cache (the Delta render-buffer cache, nni.cpp:281-379,
-// 902-921):
-// per slice time, the site value and squared distance of each of the tile's
-// W x H pixels.  A slice's labels depend on its time, the tile and the time
-// radius only, never on cur_time, so a later call gathers a cached slice
-// without searching again.  Values are kept, not point indices: the points
-// of a later call may sit at other indices (a window of a time-sorted
-// stream); ties between sites go to the earlier point, which the window
-// keeps in order.
+// Space-time cache (what the Delta render-buffer cache saves, nni.cpp:
+// 281-379, 902-921): per point time ("year"), the 2-D Voronoi of the sites
+// of that time, as the site value and squared plane distance of each of the
+// tile's W x H pixels.  A slice's nearest site is the least (d2 + dt2) over
+// the layers of the times within its radius, dt2 being constant per layer
+// and slice, so a slice is a combine of layers plus the gather, and a layer
+// depends on the tile and its time only: every slice of every later call
+// that reaches that time reuses it.  Values are kept, not point indices: a
+// later call's points may sit at other indices (a window of a time-sorted
+// stream).  Ties within a time go to the earlier point, which the window
+// keeps in order; ties across times go to the earlier time, which for
+// points sorted by time is the lower index (calls whose points are not
+// sorted by time run without the cache).
 struct tnni_time_cache {
     tnni_ctx *ctx = nullptr;  // null once the context is destroyed
-    bool keyed = false;       // desc and time_radius name the cached tile
+    bool keyed = false;       // desc names the cached tile
     tnni_tile_desc desc{};
-    float time_radius = 0.0f;
-    int max_slots = 0;  // 0: one per slice of a call
+    int max_slots = 0;  // 0: the layers of one call
     struct Slot {
         bool full = false;
-        uint32_t time_bits = 0;  // the slice time as float bits (+0 for -0)
-        uint64_t used = 0;       // last use, for least-recently-used reuse
+        uint32_t time_bits = 0;  // the layer's time as float bits
+        uint64_t used = 0;       // last call that used it, for least-recently-used reuse
         DevBuf val, d2;
     };
     std::vector<Slot> slots;
@@ -196,7 +200,7 @@ struct tnni_ctx {
     double prof_ms = 0.0;
 
     std::vector<void *> user_allocs;
-    std::vector<tnni_time_cache *> caches;  // space-time slice caches on this context
+    std::vector<tnni_time_cache *> caches;  // space-time caches on this context
 
     // measured bucket / cell-tile sizes per shape class (tnni_autotune)
     TuneTable tuned;
@@ -306,7 +310,7 @@ void arena_free(tnni_ctx *ctx, Arena &a)
     if (a.stream) (void)hipStreamSynchronize(a.stream);
     DevBuf *bufs[] = {&a.tiles, &a.chunks, &a.batch, &a.hist, &a.partials, &a.banded, &a.sorted,
                       &a.offs, &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd,
-                      &a.st, &a.acc};
+                      &a.st, &a.acc, &a.cv, &a.cd, &a.flag};
     for (DevBuf *b : bufs) release(ctx, *b);
     HostBuf *hb[] = {&a.h_desc, &a.h_in, &a.h_out};
     for (HostBuf *b : hb) {
@@ -524,7 +528,7 @@ void arena_trim(tnni_ctx *ctx, Arena &a)
 {
     DevBuf *bufs[] = {&a.tiles, &a.chunks, &a.batch, &a.hist, &a.partials, &a.banded, &a.sorted,
                       &a.offs, &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd,
-                      &a.st, &a.acc};
+                      &a.st, &a.acc, &a.cv, &a.cd, &a.flag};
     for (DevBuf *b : bufs) release(ctx, *b);
     if (a.gexec) (void)hipGraphExecDestroy(a.gexec);
     a.gexec = nullptr;
@@ -988,9 +992,11 @@ namespace {
 // pipeline run over the tile's points that keeps the points within the time
 // radius of t as sites; the cells' sums stay in the arena between the runs
 // and the last run divides.  p.t.X/Y/Z/T and out_h are set by the caller.
-// With a cache (and desc, its key) a slice found there is only gathered
-// (k_gather_slice); one not found runs the pipeline with its labels and d2
-// written into a cache slot, whose labels then become site values.
+// With a cache (and desc, its key) and points sorted by time, the slices
+// are built from per-time layers instead: the layers the cache lacks are
+// searched (a radius-0 slice at that time, its labels turned into site
+// values), then each slice combines its layers (k_combine_layers) and is
+// gathered (k_gather_slice).
 tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_time,
                           float time_radius, tnni_time_cache *cache = nullptr,
                           const tnni_tile_desc *desc = nullptr)
@@ -1011,29 +1017,153 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
     int j1 = j0;
     while ((float)(j1 + 1) <= time_radius) ++j1;
     const uint64_t pix = (uint64_t)t.W * t.H;
+    // the slices' site filter and time distance, as the kernels take them
+    // (renderInputPoints' filter nni.cpp:770, drawPlane's term nni.cpp:197)
+    auto in_slice = [time_radius](float st_, float ft) {
+        return (double)std::fabs(st_ - ft) <= (double)time_radius + 0.000001;
+    };
+    auto dt2_of = [&t](float st_, float ft) {
+        const uint32_t uu = (uint32_t)std::rint((double)std::fabs(st_ - ft) * (double)t.upm);
+        return uu * uu;
+    };
+    std::vector<float> years;  // the point times (as floats, ascending) the slices reach
+    std::vector<uint32_t> runs;
     if (cache) {
         if (!std::isfinite(cur_time)) return fail(ctx, TNNI_E_INVALID, "cur_time must be finite");
-        // another tile or radius: start over (the reference's driver clears
-        // the buffers per work package, terranni.cpp:422, nni.cpp:362-379)
-        if (!cache->keyed || cache->time_radius != time_radius ||
-            std::memcmp(&cache->desc, desc, sizeof *desc) != 0) {
-            for (auto &c : cache->slots) c.full = false;
-            cache->desc = *desc;
-            cache->time_radius = time_radius;
-            cache->keyed = true;
+        for (int j = j0; j <= j1; ++j) {
+            const float sj = cur_time + (float)j;
+            const double lo = std::floor((double)sj - (double)time_radius) - 1.0;
+            const double hi = std::ceil((double)sj + (double)time_radius) + 1.0;
+            for (double y = lo; y <= hi; y += 1.0) {
+                if (y < (double)INT32_MIN || y > (double)INT32_MAX) continue;
+                const float ft = (float)(int32_t)y;
+                if (in_slice(sj, ft) && std::find(years.begin(), years.end(), ft) == years.end())
+                    years.push_back(ft);
+            }
         }
-        // by default one slot per slice of a call (numRB, nni.cpp:902)
-        const size_t want = cache->max_slots > 0 ? (size_t)cache->max_slots : (size_t)(j1 - j0 + 1);
-        while (cache->slots.size() > want) {
-            release(ctx, cache->slots.back().val);
-            release(ctx, cache->slots.back().d2);
-            cache->slots.pop_back();
-        }
-        cache->slots.resize(want);
+        std::sort(years.begin(), years.end());
+        if ((int)years.size() > kMaxLayers) return fail(ctx, TNNI_E_INVALID, "a call reaches more than 64 point times");
+        if (t.n > 0xFFFFFFFFll) return fail(ctx, TNNI_E_INVALID, "more than 2^32 points");
+        // whether the points are sorted by time, and each time's run of them
+        LayerTimes ys;
+        ys.n = (int)years.size();
+        for (int k = 0; k < ys.n; ++k) ys.t[k] = years[(size_t)k];
+        runs.assign(1 + 2 * kMaxLayers, (uint32_t)t.n);
+        runs[0] = 0u;
+        const size_t rb = runs.size() * 4;
+        if ((st = ensure(ctx, a.flag, rb)) != TNNI_OK) return st;
+        HIP_TRY(ctx, hipMemcpyAsync(a.flag.p, runs.data(), rb, hipMemcpyHostToDevice, a.stream));
+        HIP_TRY(ctx, launch_time_runs(t.T, (uint64_t)t.n, ys, (uint32_t *)a.flag.p, a.stream));
+        HIP_TRY(ctx, hipMemcpyAsync(runs.data(), a.flag.p, rb, hipMemcpyDeviceToHost, a.stream));
+        HIP_TRY(ctx, hipStreamSynchronize(a.stream));
+        // points out of time order: ties across times would not follow the
+        // index, so this call searches its slices without the cache
+        if (runs[0]) cache = nullptr;
     }
+    if (!cache) {
+        // every slice searched: one pipeline run each, its gather summing
+        // into the cells' running sums
+        for (int j = j0; j <= j1; ++j) {
+            std::vector<Prepared> v(1, p);
+            TileDev &s = v[0].t;
+            s.slice_t = cur_time + (float)j;
+            s.time_radius = time_radius;
+            const int64_t ju = (int64_t)j * t.upm;
+            s.jj2 = (uint32_t)(ju * ju);
+            s.acc_load = j != j0;
+            s.acc_final = j == j1;
+            s.acc = (uint4 *)a.acc.p;
+            if ((st = run_prepared(ctx, a, v, false, Launch::Direct)) != TNNI_OK) return st;
+        }
+        return TNNI_OK;
+    }
+    // another tile: start over (the reference's driver clears the buffers
+    // per work package, terranni.cpp:422, nni.cpp:362-379)
+    if (!cache->keyed || std::memcmp(&cache->desc, desc, sizeof *desc) != 0) {
+        for (auto &c : cache->slots) c.full = false;
+        cache->desc = *desc;
+        cache->keyed = true;
+    }
+    const size_t want = std::max((size_t)cache->max_slots, years.size());
+    if (cache->slots.size() < want) cache->slots.resize(want);
+    // the layers: those held first (so that none of them is reused for
+    // another time in this call), then the missing ones, searched together
+    // into the least recently used slots
+    const uint64_t now = ++cache->clock;
+    std::vector<tnni_time_cache::Slot *> lay(years.size(), nullptr);
+    for (size_t k = 0; k < years.size(); ++k) {
+        uint32_t key;
+        std::memcpy(&key, &years[k], 4);
+        for (auto &c : cache->slots)
+            if (c.full && c.time_bits == key) {
+                lay[k] = &c;
+                c.used = now;
+                ++cache->hits;
+                break;
+            }
+    }
+    if ((st = ensure(ctx, a.cv, pix * 4)) != TNNI_OK) return st;
+    if ((st = ensure(ctx, a.cd, pix * 4)) != TNNI_OK) return st;
+    std::vector<Prepared> v;
+    std::vector<tnni_time_cache::Slot *> vslot;
+    // cell blocks of the layer searches: a call that searches one or two
+    // layers would leave most workgroups idle with the plain form's blocks,
+    // so below 16 items per CU a layer's blocks shrink (to no less than 8
+    // cells a side, and only to a size its LDS budget takes; the block size
+    // never changes a result)
+    size_t n_missing = 0;
+    for (size_t k = 0; k < years.size(); ++k) n_missing += lay[k] ? 0 : 1;
+    const double item_target = 16.0 * ctx->n_cu / (double)std::max<size_t>(n_missing, 1);
+    const int tc_small = std::max(8, (int)std::floor(std::sqrt((double)t.nrows * t.ncols / item_target)));
+    for (size_t k = 0; k < years.size(); ++k) {
+        if (lay[k]) continue;
+        // a slot this call uses (held, or already taken for another missing
+        // time; those fill only after the search) is never reused
+        tnni_time_cache::Slot *victim = nullptr;
+        for (auto &c : cache->slots) {
+            if (c.used == now) continue;
+            if (!victim || (victim->full && (!c.full || c.used < victim->used))) victim = &c;
+        }
+        if (!victim) return fail(ctx, TNNI_E_INVALID, "space-time cache: no free slot");
+        victim->full = false;
+        if ((st = ensure(ctx, victim->val, pix * 4)) != TNNI_OK) return st;
+        if ((st = ensure(ctx, victim->d2, pix * 4)) != TNNI_OK) return st;
+        // the 2-D Voronoi of the points of this time: the plain tile of
+        // their run (its heights go to scratch that the slices overwrite)
+        const uint32_t lo = runs[1 + k], hi = std::max(runs[1 + k], runs[1 + kMaxLayers + k]);
+        Prepared q;
+        std::string err;
+        if ((st = prepare_tile(*desc, hi - lo, ctx->n_cu, false, &ctx->tuned, q.t, q.info, err)) != TNNI_OK)
+            return fail(ctx, st, err);
+        if (desc->tile_cells <= 0 && (double)q.t.ntx * q.t.nty < item_target && tc_small < q.t.tc) {
+            tnni_tile_desc ld = *desc;
+            ld.tile_cells = tc_small;
+            Prepared qs;
+            if (prepare_tile(ld, hi - lo, ctx->n_cu, false, &ctx->tuned, qs.t, qs.info, err) == TNNI_OK) q = qs;
+        }
+        q.t.X = t.X + lo;
+        q.t.Y = t.Y + lo;
+        q.t.Z = t.Z + lo;
+        q.t.out_h = (float *)a.cv.p;
+        q.t.out_labels = (int32_t *)victim->val.p;
+        q.t.out_d2 = (uint32_t *)victim->d2.p;
+        v.push_back(q);
+        vslot.push_back(victim);
+        std::memcpy(&victim->time_bits, &years[k], 4);
+        victim->used = now;
+        lay[k] = victim;
+        ++cache->misses;
+    }
+    if (!v.empty()) {
+        if ((st = run_prepared(ctx, a, v, false, Launch::Direct)) != TNNI_OK) return st;
+        for (size_t i = 0; i < v.size(); ++i) {
+            HIP_TRY(ctx, launch_label_values(v[i].t, (int32_t *)vslot[i]->val.p, pix, a.stream));
+            vslot[i]->full = true;
+        }
+    }
+    // each slice: its layers combined, then gathered
     for (int j = j0; j <= j1; ++j) {
-        std::vector<Prepared> v(1, p);
-        TileDev &s = v[0].t;
+        TileDev s = p.t;
         s.slice_t = cur_time + (float)j;
         s.time_radius = time_radius;
         const int64_t ju = (int64_t)j * t.upm;
@@ -1041,40 +1171,17 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
         s.acc_load = j != j0;
         s.acc_final = j == j1;
         s.acc = (uint4 *)a.acc.p;
-        if (!cache) {
-            if ((st = run_prepared(ctx, a, v, false, Launch::Direct)) != TNNI_OK) return st;
-            continue;
+        LayerSet ls;
+        ls.n = 0;
+        for (size_t k = 0; k < years.size(); ++k) {
+            if (!in_slice(s.slice_t, years[k])) continue;
+            ls.dt2[ls.n] = dt2_of(s.slice_t, years[k]);
+            ls.val[ls.n] = (const int32_t *)lay[k]->val.p;
+            ls.d2[ls.n] = (const uint32_t *)lay[k]->d2.p;
+            ++ls.n;
         }
-        uint32_t key;
-        const float kt = s.slice_t + 0.0f;
-        std::memcpy(&key, &kt, 4);
-        tnni_time_cache::Slot *hit = nullptr, *victim = nullptr;
-        for (auto &c : cache->slots) {
-            if (c.full && c.time_bits == key) {
-                hit = &c;
-                break;
-            }
-            if (!victim || (victim->full && (!c.full || c.used < victim->used))) victim = &c;
-        }
-        const uint64_t now = ++cache->clock;
-        if (hit) {
-            hit->used = now;
-            ++cache->hits;
-            HIP_TRY(ctx, launch_gather_slice(s, (const int32_t *)hit->val.p, (const uint32_t *)hit->d2.p,
-                                             a.stream));
-            continue;
-        }
-        victim->full = false;
-        if ((st = ensure(ctx, victim->val, pix * 4)) != TNNI_OK) return st;
-        if ((st = ensure(ctx, victim->d2, pix * 4)) != TNNI_OK) return st;
-        s.out_labels = (int32_t *)victim->val.p;
-        s.out_d2 = (uint32_t *)victim->d2.p;
-        if ((st = run_prepared(ctx, a, v, false, Launch::Direct)) != TNNI_OK) return st;
-        HIP_TRY(ctx, launch_label_values(s, (int32_t *)victim->val.p, pix, a.stream));
-        victim->full = true;
-        victim->time_bits = key;
-        victim->used = now;
-        ++cache->misses;
+        HIP_TRY(ctx, launch_combine_layers(ls, pix, (int32_t *)a.cv.p, (uint32_t *)a.cd.p, a.stream));
+        HIP_TRY(ctx, launch_gather_slice(s, (const int32_t *)a.cv.p, (const uint32_t *)a.cd.p, a.stream));
     }
     return TNNI_OK;
 }

@@ -175,11 +175,28 @@ hipError_t enqueue_pipeline(const DevBuffers &b, const LaunchCfg &cfg,
                             hipStream_t stream, hipEvent_t ev_main0,
                             hipEvent_t ev_main1);
 
-// Space-time slice cache: a slice's label raster turned into site values
-// in place, and the query gather of a cached slice (values and squared
-// distances of its W x H pixels) into t.acc / t.out_h.
+// Space-time year layers: a label raster turned into site values in place;
+// a slice's values and squared distances combined from the layers of its
+// times (ascending, dt2 each); the query gather of a slice (values and
+// squared distances of its W x H pixels) into t.acc / t.out_h; whether the
+// points' times are out of order, and the run of each layer's time.
+constexpr int kMaxLayers = 64;
+struct LayerSet {
+    int n;
+    uint32_t dt2[kMaxLayers];
+    const int32_t *val[kMaxLayers];
+    const uint32_t *d2[kMaxLayers];
+};
 hipError_t launch_label_values(const TileDev &t, int32_t *lab, uint64_t pix, hipStream_t s);
+hipError_t launch_combine_layers(const LayerSet &ls, uint64_t pix, int32_t *val, uint32_t *d2, hipStream_t s);
 hipError_t launch_gather_slice(const TileDev &t, const int32_t *val, const uint32_t *d2, hipStream_t s);
+struct LayerTimes {
+    int n;
+    float t[kMaxLayers];
+};
+// out: 1 + 2 * kMaxLayers words, filled by the caller with 0, then
+// kMaxLayers times n, then kMaxLayers times n.
+hipError_t launch_time_runs(const int32_t *T, uint64_t n, const LayerTimes &ys, uint32_t *out, hipStream_t s);
 
 // Largest workgroups per CU of the tile kernel at the given LDS size.
 int tile_kernel_occupancy(size_t lds_bytes);

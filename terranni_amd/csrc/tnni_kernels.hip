@@ -1046,12 +1046,11 @@ hipError_t launch_tiles(const DevBuffers &d, const LaunchCfg &cfg, hipStream_t s
     return hipGetLastError();
 }
 
-// ---- space-time slice cache ---------------------------------------------
+// ---- space-time cache of per-time layers ---------------------------------
 
-// A slice's label raster (written by k_tiles) turned into the labels' site
-// values in place (0 where no site covers the pixel), so that the slice
-// can be gathered again in later calls whose point arrays index
-// differently.
+// A layer's label raster (written by k_tiles) turned into the labels' site
+// values in place (0 where no site covers the pixel), so that the layer
+// serves later calls whose point arrays index differently.
 __global__ void __launch_bounds__(kBlock)
 k_label_values(const TileDev t, int32_t *__restrict__ lab, uint64_t pix)
 {
@@ -1094,6 +1093,54 @@ k_gather_slice(const TileDev t, const int32_t *__restrict__ val, const uint32_t 
     }
 }
 
+// A space-time slice's site values and squared distances from per-year
+// layers: each layer holds the 2-D Voronoi of the sites of one time (the
+// nearest covering site's value and squared plane distance, kNoD2 where no
+// site covers the pixel), and within the slice every site of that time is
+// at the same time distance dt2.  The slice's nearest site is the least
+// (d2 + dt2) over its layers; an equal sum goes to the earlier time, which
+// for points sorted by time is the lower index.
+__global__ void __launch_bounds__(kBlock)
+k_combine_layers(const LayerSet ls, uint64_t pix, int32_t *__restrict__ val, uint32_t *__restrict__ d2)
+{
+    for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < pix; p += (uint64_t)gridDim.x * kBlock) {
+        uint64_t best = ~0ull;
+        int32_t v = 0;
+        for (int k = 0; k < ls.n; ++k) {
+            const uint32_t d = ls.d2[k][p];
+            if (d == kNoD2) continue;
+            const uint64_t key = ((uint64_t)(d + ls.dt2[k]) << 32) | (uint32_t)k;
+            if (key < best) {
+                best = key;
+                v = ls.val[k][p];
+            }
+        }
+        d2[p] = best == ~0ull ? kNoD2 : (uint32_t)(best >> 32);
+        val[p] = v;
+    }
+}
+
+// One pass over the points' times: out[0] = 1 when some T[i] > T[i + 1];
+// for points sorted by time, the run of the points of time ys.t[k] (as
+// float) is [out[1 + k], out[1 + kMaxLayers + k]).  The caller fills out
+// with 0 and n first; every boundary (a change of time between neighbours)
+// sets the runs of the times it passes.
+__global__ void __launch_bounds__(kBlock)
+k_time_runs(const int32_t *__restrict__ T, uint64_t n, const LayerTimes ys, uint32_t *out)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        const int32_t b = T[i];
+        if (i + 1 < n && b > T[i + 1]) out[0] = 1u;
+        if (i > 0 && T[i - 1] == b) continue;
+        const float fa = i > 0 ? (float)T[i - 1] : -__builtin_huge_valf(), fb = (float)b;
+        for (int k = 0; k < ys.n; ++k) {
+            const float y = ys.t[k];
+            if (fa < y && y <= fb) out[1 + k] = (uint32_t)i;               // first of time >= y
+            if (fa <= y && y < fb) out[1 + kMaxLayers + k] = (uint32_t)i;  // first of time > y
+        }
+    }
+}
+
 template <int ALG, int MODE>
 hipError_t launch_gather_slice_t(const TileDev &t, const int32_t *val, const uint32_t *d2, hipStream_t s)
 {
@@ -1109,6 +1156,23 @@ hipError_t launch_label_values(const TileDev &t, int32_t *lab, uint64_t pix, hip
 {
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((pix + kBlock - 1) / kBlock, 8192));
     hipLaunchKernelGGL(k_label_values, dim3(grid), dim3(kBlock), 0, s, t, lab, pix);
+    return hipGetLastError();
+}
+
+hipError_t launch_combine_layers(const LayerSet &ls, uint64_t pix, int32_t *val, uint32_t *d2, hipStream_t s)
+{
+    if (ls.n < 0 || ls.n > kMaxLayers) return hipErrorInvalidValue;
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((pix + kBlock - 1) / kBlock, 8192));
+    hipLaunchKernelGGL(k_combine_layers, dim3(grid), dim3(kBlock), 0, s, ls, pix, val, d2);
+    return hipGetLastError();
+}
+
+hipError_t launch_time_runs(const int32_t *T, uint64_t n, const LayerTimes &ys, uint32_t *out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    if (ys.n < 0 || ys.n > kMaxLayers || n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((n + kBlock - 1) / kBlock, 4096));
+    hipLaunchKernelGGL(k_time_runs, dim3(grid), dim3(kBlock), 0, s, T, n, ys, out);
     return hipGetLastError();
 }
 

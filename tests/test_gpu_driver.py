@@ -140,7 +140,7 @@ def test_driver_space_time_both_engines(tmp_path):
 
 def test_driver_space_time_series_delta_and_delta_squared(tmp_path):
     """A time series at radius 1 stepped by 0.5 (slices 0, 0.5, ..., 4): the
-    batched engine with its per-package slice caches (Delta) writes the same
+    batched engine with its per-package space-time caches (Delta) writes the same
     grids bit for bit as with --delta-squared (every slice searched), and
     the grids at 1 and 1.5 equal the oracle's whole-grid tile."""
     (tmp_path / "las").mkdir()

@@ -1,10 +1,12 @@
-"""The space-time slice cache on the GPU (performNNI's Delta algorithm,
+"""The space-time cache on the GPU (what performNNI's Delta algorithm saves,
 nni.cpp:281-379 and 902-921): a caller steps cur_time through a
 time-sorted stream, each call with the points of a window around cur_time
-(as the nni.h layer reads them); slices already held are gathered without
-a search.  Every cached call must equal the uncached call bit for bit, and
-the uncached call the oracle; the hit and miss counts follow from the
-slice times."""
+(as the nni.h layer reads them); the cache keeps per point time the 2-D
+Voronoi layer of that time's points, each slice is combined from the
+layers within its radius, and only missing layers are searched.  Every
+cached call must equal the uncached call (which searches every slice) bit
+for bit, and the uncached call the oracle; the layer hit and miss counts
+follow from the times the calls reach."""
 import dataclasses
 
 import numpy as np
@@ -21,10 +23,12 @@ def _say(*a):
     print(*a, flush=True)
 
 
-def _stream(seed, n, grid, alg, mode, t0=1996, t1=2010, dup=200, **kw):
+def _stream(seed, n, grid, alg, mode, t0=1996, t1=2010, dup=200, xdup=0, **kw):
     """A tile whose points are sorted by time (stable), with `dup` points
-    repeated at the same place and time but another height, so that equal
-    distances are decided by the earlier point."""
+    repeated at the same place and time but another height, and `xdup`
+    repeated at the same place two time units later, so that equal
+    distances are decided by the earlier point (within a time and, for a
+    slice halfway between, across times)."""
     t = S.make_tile(seed, n, grid, kind="clustered", algorithm=alg, accum_mode=mode,
                     point_min=-60, multiplier=1000, **kw)
     rng = np.random.default_rng(seed)
@@ -35,6 +39,11 @@ def _stream(seed, n, grid, alg, mode, t0=1996, t1=2010, dup=200, **kw):
         X, Y = np.concatenate([X, X[k]]), np.concatenate([Y, Y[k]])
         Z = np.concatenate([Z, Z[k] + rng.integers(1, 500, dup).astype(np.int32)])
         times = np.concatenate([times, times[k]])
+    if xdup:
+        k = rng.choice(t.n, xdup, replace=False)
+        X, Y = np.concatenate([X, X[k]]), np.concatenate([Y, Y[k]])
+        Z = np.concatenate([Z, Z[k] - rng.integers(1, 500, xdup).astype(np.int32)])
+        times = np.concatenate([times, times[k] + 2])
     o = np.argsort(times, kind="stable")
     return t.params, X[o], Y[o], Z[o], times[o]
 
@@ -69,14 +78,15 @@ def test_cached_slices_equal_uncached(ctx, alg, mode):
                     assert _same(h, ref["h_fixed"])
                 elif mode == T.ACC_S0:
                     assert _same(h, ref["h_s0"])
-        # 5 slices in the first call, then one new slice per call
-        assert c.stats() == (6 * 4, 5 + 6)
+        # layers of the times within 4 of cur_time: 9 in the first call,
+        # then one new layer per call
+        assert c.stats() == (6 * 8, 9 + 6)
 
 
-def test_cache_keys_tile_radius_and_slice_time(ctx):
-    """Another radius, another tile or a clear empties the cache; slice times
-    between integers are their own keys; going back in time reuses the
-    slices still held."""
+def test_cache_keys_tile_and_point_time(ctx):
+    """Layers are keyed by point time: slices between integers and other
+    radii reuse them; another tile or a clear empties the cache; a slice
+    with no point time within its radius has no sites."""
     p, X, Y, Z, Tm = _stream(21, 10000, 48, T.ALG_PLANES, T.ACC_FIXED64)
     with ctx.time_cache() as c:
         def run(q, cur, r):
@@ -84,30 +94,26 @@ def test_cache_keys_tile_radius_and_slice_time(ctx):
             h = ctx.tile_run_time(q, X[w], Y[w], Z[w], Tm[w], cur, r, cache=c)
             assert _same(h, ctx.tile_run_time(q, X[w], Y[w], Z[w], Tm[w], cur, r)), (cur, r)
 
-        run(p, 2001.0, 1.0)
-        assert c.stats() == (0, 3)
-        run(p, 2002.0, 1.0)             # 2001, 2002 held
-        assert c.stats() == (2, 4)
-        run(p, 2001.5, 1.0)             # 2000.5 .. 2002.5: none held
-        assert c.stats() == (2, 7)
-        run(p, 2001.5, 1.0)
-        assert c.stats() == (5, 7)
-        run(p, 2001.5, 1.5)             # another radius
-        assert c.stats() == (5, 10)
+        run(p, 2001.0, 1.0)             # times 1999 .. 2003
+        assert c.stats() == (0, 5)
+        run(p, 2002.0, 1.0)             # 2000 .. 2004: 2004 is new
+        assert c.stats() == (4, 6)
+        run(p, 2001.5, 1.0)             # slices 2000.5 .. 2002.5: times 2000 .. 2003
+        assert c.stats() == (8, 6)
+        run(p, 2001.5, 1.5)             # another radius: 1999 .. 2004
+        assert c.stats() == (13, 7)
         q = dataclasses.replace(p, origin_x=p.origin_x + 0.5)
         run(q, 2001.5, 1.5)             # another tile
-        assert c.stats() == (5, 13)
+        assert c.stats() == (13, 13)
         run(q, 2001.5, 1.5)
-        assert c.stats() == (8, 13)
+        assert c.stats() == (19, 13)
         c.clear()
         run(q, 2001.5, 1.5)
-        assert c.stats() == (8, 16)
-        # r = 0: one slice per call, one slot
-        run(q, 2003.0, 0.0)
-        run(q, 2003.0, 0.0)
-        run(q, 2004.0, 0.0)
-        run(q, 2003.0, 0.0)
-        assert c.stats() == (9, 19)
+        assert c.stats() == (19, 19)
+        run(q, 2003.0, 0.0)             # r = 0: the one time, held
+        assert c.stats() == (20, 19)
+        run(q, 2003.5, 0.0)             # no point time at the slice: no sites
+        assert c.stats() == (20, 19)
 
 
 def test_cache_device_buffers_and_empty_windows(ctx):
@@ -129,9 +135,9 @@ def test_cache_device_buffers_and_empty_windows(ctx):
                 assert (u == T.NODATA).all()
             for a in (dX, dY, dZ, dT, dH):
                 a.free()
-        # 1999..2003, 2000..2004 (4 held), 2007..2011, 2008..2012 (4 held),
-        # 2001..2005 (the slots hold 2008..2012)
-        assert c.stats() == (8, 17)
+        # times 1997..2005, 1998..2006 (8 held), 2005..2013 (2 held),
+        # 2006..2014 (8 held), 1999..2007 (2 held; 9 slots)
+        assert c.stats() == (20, 25)
 
 
 def test_cache_from_another_context_is_refused(ctx):
@@ -144,10 +150,10 @@ def test_cache_from_another_context_is_refused(ctx):
         c.close()
 
 
-def test_cache_with_slots_for_half_steps(ctx):
-    """cur_time stepped by 0.5 alternates two phases of slice times; a cache
-    of 2 x 5 slots keeps both, so every call after the first two searches
-    one slice."""
+def test_cache_half_steps_reuse_layers(ctx):
+    """cur_time stepped by 0.5: the slices of integer and half-integer
+    cur_time reach the same point times, so every call after the first
+    searches at most one layer, whatever the slot count."""
     p, X, Y, Z, Tm = _stream(51, 8000, 40, T.ALG_PLANES, T.ACC_FP32_2SUM)
     with ctx.time_cache(slots=10) as c, ctx.time_cache() as d:
         for k in range(8):
@@ -157,8 +163,32 @@ def test_cache_with_slots_for_half_steps(ctx):
             g = ctx.tile_run_time(p, X[w], Y[w], Z[w], Tm[w], cur, 2.0, cache=d)
             u = ctx.tile_run_time(p, X[w], Y[w], Z[w], Tm[w], cur, 2.0)
             assert _same(h, u) and _same(g, u), cur
-        assert c.stats() == (6 * 4, 10 + 6)
-        # 5 slots: each phase evicts the other's slices
-        assert d.stats() == (0, 40)
+        # times 1997..2005, 1998..2005, 1998..2006, 1999..2006, ...
+        assert c.stats() == (56, 12)
+        assert d.stats() == (56, 12)
     with pytest.raises(T.TnniError):
         ctx.time_cache(slots=5000)
+
+
+def test_cache_ties_across_times_and_unsorted_points(ctx):
+    """Points repeated two time units later at the same place tie with the
+    original in the slice between them (equal plane and time distances):
+    the earlier point wins, as in the slice search.  Points not sorted by
+    time run without the cache (its counts stay) and still give the
+    uncached result."""
+    for alg in (T.ALG_PLANES, T.ALG_CONES):
+        p, X, Y, Z, Tm = _stream(61 + alg, 6000, 40, alg, T.ACC_FIXED64, t0=2000, t1=2006,
+                                 dup=50, xdup=400)
+        with ctx.time_cache() as c:
+            for cur in (2002.0, 2003.0, 2004.0):
+                w = _window(Tm, cur, 1.0)
+                h = ctx.tile_run_time(p, X[w], Y[w], Z[w], Tm[w], cur, 1.0, cache=c)
+                assert _same(h, ctx.tile_run_time(p, X[w], Y[w], Z[w], Tm[w], cur, 1.0)), cur
+                ref = O.run_time(p, X[w], Y[w], Z[w], Tm[w], cur, 1.0)
+                assert _same(h, ref["h_fixed"]), cur
+            before = c.stats()
+            rng = np.random.default_rng(5)
+            o = rng.permutation(len(X))
+            h = ctx.tile_run_time(p, X[o], Y[o], Z[o], Tm[o], 2003.0, 1.0, cache=c)
+            assert _same(h, ctx.tile_run_time(p, X[o], Y[o], Z[o], Tm[o], 2003.0, 1.0))
+            assert c.stats() == before
