@@ -208,9 +208,12 @@ tnni_status tnni_serve_stats_reset(tnni_ctx *ctx);
 
 /* ---- space-time (timeRadius > 0) ----------------------------------------
  * One work package at time cur_time with time radius r, as performNNI with
- * timeRadius > 0 (nni.cpp:870-944, replaces its slice loop :910-933 with
- * every slice searched, the Delta^2 algorithm; the Delta algorithm's
- * render-buffer cache is tnni_time_cache below).  T[i] is point i's time (the LAS
+ * timeRadius > 0 (nni.cpp:870-944, replaces its slice loop :910-933).
+ * Nothing is kept between calls (the Delta^2 algorithm; the Delta
+ * algorithm's render-buffer cache is tnni_time_cache below).  Points sorted
+ * by time (at most 64 point times within reach) are searched once per time
+ * and each slice combined from those per-time layers; other calls search
+ * every slice.  Both give the same result.  T[i] is point i's time (the LAS
  * intensity, las_reader.inl:59).  Slices t = cur_time + j (float), j =
  * -(int)r .. r: slice t's sites are the points with |t - T| <= r + 1e-6
  * (nni.cpp:770), each at squared distance |x - s|^2 + (|t - T| in metres,
@@ -238,8 +241,9 @@ tnni_status tnni_tile_run_time_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, co
  * every later cur_time that reaches its time; only the layers a call lacks
  * are searched (one per call when cur_time steps by 1, or by 0.5).  The
  * result is bit for bit that of the uncached call.  The points must be
- * sorted by time (a call whose points are not runs without the cache,
- * searching every slice); the caller keeps the points of each time the
+ * sorted by time (a call whose points are not, or that reaches more than 64
+ * point times, runs without the cache, searching every slice); the caller
+ * keeps the points of each time the
  * same, and in the same relative order, between the calls that use one
  * cache (a time-sorted stream read in windows around cur_time does); a call
  * with another tile descriptor empties the cache first.  A cache belongs to

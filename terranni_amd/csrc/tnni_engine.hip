@@ -201,6 +201,7 @@ struct tnni_ctx {
 
     std::vector<void *> user_allocs;
     std::vector<tnni_time_cache *> caches;  // space-time caches on this context
+    tnni_time_cache *scratch_cache = nullptr;  // the layers of uncached space-time calls
 
     // measured bucket / cell-tile sizes per shape class (tnni_autotune)
     TuneTable tuned;
@@ -860,6 +861,7 @@ void tnni_ctx_destroy(tnni_ctx *ctx)
         c->keyed = false;
         c->ctx = nullptr;
     }
+    delete ctx->scratch_cache;
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->tev0) (void)hipEventDestroy(ctx->tev0);
@@ -992,11 +994,13 @@ namespace {
 // pipeline run over the tile's points that keeps the points within the time
 // radius of t as sites; the cells' sums stay in the arena between the runs
 // and the last run divides.  p.t.X/Y/Z/T and out_h are set by the caller.
-// With a cache (and desc, its key) and points sorted by time, the slices
-// are built from per-time layers instead: the layers the cache lacks are
-// searched (a radius-0 slice at that time, its labels turned into site
-// values), then each slice combines its layers (k_combine_layers) and is
-// gathered (k_gather_slice).
+// With points sorted by time (and at most kMaxLayers times within reach) the
+// slices are built from per-time layers instead: the layers the cache (or,
+// without one, the context's scratch cache emptied first) lacks are
+// searched together as plain tiles of their runs of points, their labels
+// turned into site values, then each slice combines its layers
+// (k_combine_layers) and is gathered (k_gather_slice).  desc is the
+// cache's key.
 tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_time,
                           float time_radius, tnni_time_cache *cache = nullptr,
                           const tnni_tile_desc *desc = nullptr)
@@ -1028,8 +1032,12 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
     };
     std::vector<float> years;  // the point times (as floats, ascending) the slices reach
     std::vector<uint32_t> runs;
-    if (cache) {
-        if (!std::isfinite(cur_time)) return fail(ctx, TNNI_E_INVALID, "cur_time must be finite");
+    if (cache && !std::isfinite(cur_time)) return fail(ctx, TNNI_E_INVALID, "cur_time must be finite");
+    // per-time layers when the call's points are sorted by time and reach
+    // at most kMaxLayers times; the layers are kept between calls only with
+    // a cache (without one, the context's scratch cache starts empty)
+    bool layered = std::isfinite(cur_time) && t.n <= 0xFFFFFFFFll;
+    if (layered) {
         for (int j = j0; j <= j1; ++j) {
             const float sj = cur_time + (float)j;
             const double lo = std::floor((double)sj - (double)time_radius) - 1.0;
@@ -1042,8 +1050,9 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
             }
         }
         std::sort(years.begin(), years.end());
-        if ((int)years.size() > kMaxLayers) return fail(ctx, TNNI_E_INVALID, "a call reaches more than 64 point times");
-        if (t.n > 0xFFFFFFFFll) return fail(ctx, TNNI_E_INVALID, "more than 2^32 points");
+        layered = (int)years.size() <= kMaxLayers;
+    }
+    if (layered) {
         // whether the points are sorted by time, and each time's run of them
         LayerTimes ys;
         ys.n = (int)years.size();
@@ -1057,10 +1066,10 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
         HIP_TRY(ctx, hipMemcpyAsync(runs.data(), a.flag.p, rb, hipMemcpyDeviceToHost, a.stream));
         HIP_TRY(ctx, hipStreamSynchronize(a.stream));
         // points out of time order: ties across times would not follow the
-        // index, so this call searches its slices without the cache
-        if (runs[0]) cache = nullptr;
+        // index, so this call searches every slice
+        layered = runs[0] == 0u;
     }
-    if (!cache) {
+    if (!layered) {
         // every slice searched: one pipeline run each, its gather summing
         // into the cells' running sums
         for (int j = j0; j <= j1; ++j) {
@@ -1076,6 +1085,17 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
             if ((st = run_prepared(ctx, a, v, false, Launch::Direct)) != TNNI_OK) return st;
         }
         return TNNI_OK;
+    }
+    if (!cache) {
+        // Delta^2: nothing is kept between calls
+        if (!ctx->scratch_cache) {
+            ctx->scratch_cache = new (std::nothrow) tnni_time_cache;
+            if (!ctx->scratch_cache) return fail(ctx, TNNI_E_NOMEM, "time cache");
+            ctx->scratch_cache->ctx = ctx;
+            ctx->caches.push_back(ctx->scratch_cache);
+        }
+        cache = ctx->scratch_cache;
+        cache->keyed = false;
     }
     // another tile: start over (the reference's driver clears the buffers
     // per work package, terranni.cpp:422, nni.cpp:362-379)

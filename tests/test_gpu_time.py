@@ -32,14 +32,22 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("order", ["unsorted", "sorted"])
 @pytest.mark.parametrize("seed,n,grid,kind,scale,alg,mode,r,cur", CASES)
-def test_space_time_tile_matches_oracle(ctx, seed, n, grid, kind, scale, alg, mode, r, cur):
-    _say(f"space-time seed={seed} n={n} grid={grid} alg={alg} mode={mode} r={r}")
+def test_space_time_tile_matches_oracle(ctx, seed, n, grid, kind, scale, alg, mode, r, cur, order):
+    """unsorted: points in random time order, every slice searched; sorted:
+    points sorted by time, searched once per time as 2-D layers and every
+    slice combined from them."""
+    _say(f"space-time seed={seed} n={n} grid={grid} alg={alg} mode={mode} r={r} {order}")
     t = S.make_tile(seed, n, grid, scale=scale, kind=kind, algorithm=alg, accum_mode=mode,
                     point_min=-60, multiplier=1000)
     times = np.random.default_rng(seed).integers(1998, 2007, t.n).astype(np.int32)
-    h = ctx.tile_run_time(t.params, t.X, t.Y, t.Z, times, cur, r)
-    ref = O.run_time(t.params, t.X, t.Y, t.Z, times, cur, r, s0=mode == T.ACC_S0)
+    X, Y, Z = t.X, t.Y, t.Z
+    if order == "sorted":
+        o = np.argsort(times, kind="stable")
+        X, Y, Z, times = X[o], Y[o], Z[o], times[o]
+    h = ctx.tile_run_time(t.params, X, Y, Z, times, cur, r)
+    ref = O.run_time(t.params, X, Y, Z, times, cur, r, s0=mode == T.ACC_S0)
     if mode == T.ACC_FIXED64:
         assert np.array_equal(h.view(np.uint32), ref["h_fixed"].view(np.uint32))
     elif mode == T.ACC_S0:
@@ -80,3 +88,24 @@ def test_space_time_empty_and_bucket_sweep(ctx):
         assert np.array_equal(h.view(np.uint32), ref["h_fixed"].view(np.uint32)), (bm, tc)
     e = ctx.tile_run_time(t.params, t.X[:0], t.Y[:0], t.Z[:0], times[:0], 2002.0, 2.0)
     assert (e == T.NODATA).all()
+
+
+def test_space_time_sorted_ties_across_times(ctx):
+    """Sorted points repeated two time units later at the same place: in the
+    slice halfway between, both are at the same distance and the earlier
+    point (the earlier time) wins, in the per-time layers as in the oracle;
+    the random order of the same points (every slice searched) matches the
+    oracle on its own order too."""
+    t = S.make_tile(51, 6000, 40, kind="clustered")
+    rng = np.random.default_rng(51)
+    times = rng.integers(2000, 2004, t.n).astype(np.int32)
+    k = rng.choice(t.n, 600, replace=False)
+    X, Y = np.concatenate([t.X, t.X[k]]), np.concatenate([t.Y, t.Y[k]])
+    Z = np.concatenate([t.Z, t.Z[k] + 777])
+    Tm = np.concatenate([times, times[k] + 2])
+    for order in ("sorted", "unsorted"):
+        o = np.argsort(Tm, kind="stable") if order == "sorted" else rng.permutation(len(X))
+        for cur, r in ((2002.0, 1.0), (2003.0, 2.0), (2002.5, 1.5)):
+            h = ctx.tile_run_time(t.params, X[o], Y[o], Z[o], Tm[o], cur, r)
+            ref = O.run_time(t.params, X[o], Y[o], Z[o], Tm[o], cur, r)
+            assert np.array_equal(h.view(np.uint32), ref["h_fixed"].view(np.uint32)), (order, cur, r)
