@@ -307,6 +307,14 @@ class TimeCache:
     def __exit__(self, *a):
         self.close()
 
+    def __del__(self):
+        # safe in either order with the context: a cache that outlives its
+        # context holds nothing and only accepts destroy
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 class Context:
     """One engine context on one GPU (tnni_ctx)."""
