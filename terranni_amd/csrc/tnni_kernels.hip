@@ -721,9 +721,12 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
     const int eb = py - (t.gy0 + by * bs) + 1, et = t.gy0 + (by + 1) * bs - py;
     const int e0 = min(min(el, er), min(eb, et));
     const bool cover = t.ring1_covered;
-    if (t.dense) {
+    if (t.dense || !staged) {
         // ring by ring: ring 1 only where it may still hold a covering site
-        // nearer than the best (gap e0)
+        // nearer than the best (gap e0).  Also where the block's window
+        // overflows the stage (a cluster core in a sparser tile): there the
+        // 3x3 square would read hundreds of sites per pixel from global
+        // memory, and ring 0 nearly always settles the pixel
         for (int r = r0; r <= 1; ++r) {
             if (r > 0 && (e0 > t.Rsb || (uint64_t)e0 * (uint64_t)e0 > (best >> 32))) break;
             if (!staged) ring_global<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, best, bval);
