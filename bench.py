@@ -659,7 +659,9 @@ def main():
     for _ in range(3):
         ctx.tile_run(tile.params, tile.X, tile.Y, tile.Z)
     pcie_cells_s = 3 * tile.cells / (time.perf_counter() - t_h)
-    copy_gbs = copy_bandwidth(ctx)
+    # (not in a counter calibration run, whose one probe copy must be the
+    # only large copy)
+    copy_gbs = copy_bandwidth(ctx) if not args.copy_probe_mb else None
     if args.copy_probe_mb:
         nb = args.copy_probe_mb << 20
         a, c = ctx.alloc((nb,), np.uint8), ctx.alloc((nb,), np.uint8)
