@@ -212,8 +212,9 @@ tnni_status tnni_serve_stats_reset(tnni_ctx *ctx);
  * Nothing is kept between calls (the Delta^2 algorithm; the Delta
  * algorithm's render-buffer cache is tnni_time_cache below).  Points sorted
  * by time (at most 64 point times within reach) are searched once per time
- * and each slice combined from those per-time layers; other calls search
- * every slice.  Both give the same result.  T[i] is point i's time (the LAS
+ * and each slice combined from those per-time layers (their buffers, 8
+ * bytes per pixel per time reached, stay in the context for the next call);
+ * other calls search every slice.  Both give the same result.  T[i] is point i's time (the LAS
  * intensity, las_reader.inl:59).  Slices t = cur_time + j (float), j =
  * -(int)r .. r: slice t's sites are the points with |t - T| <= r + 1e-6
  * (nni.cpp:770), each at squared distance |x - s|^2 + (|t - T| in metres,
