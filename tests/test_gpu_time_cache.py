@@ -192,3 +192,20 @@ def test_cache_ties_across_times_and_unsorted_points(ctx):
             h = ctx.tile_run_time(p, X[o], Y[o], Z[o], Tm[o], 2003.0, 1.0, cache=c)
             assert _same(h, ctx.tile_run_time(p, X[o], Y[o], Z[o], Tm[o], 2003.0, 1.0))
             assert c.stats() == before
+
+
+def test_many_point_times_fall_back_to_slice_search(ctx):
+    """A call reaching more than 64 point times (r = 20 over 60 years: the 81
+    times within 40 of cur_time) searches every slice, with or without a
+    cache (whose counts stay); r = 10 (41 times) uses the layers.  Both
+    equal the oracle."""
+    p, X, Y, Z, Tm = _stream(71, 3000, 24, T.ALG_PLANES, T.ACC_FIXED64, t0=1970, t1=2030, dup=0)
+    with ctx.time_cache() as c:
+        for r, layered in ((20.0, False), (10.0, True)):
+            before = c.stats()
+            h = ctx.tile_run_time(p, X, Y, Z, Tm, 2000.0, r, cache=c)
+            u = ctx.tile_run_time(p, X, Y, Z, Tm, 2000.0, r)
+            ref = O.run_time(p, X, Y, Z, Tm, 2000.0, r)
+            assert _same(h, u) and _same(h, ref["h_fixed"]), r
+            hits, misses = c.stats()
+            assert (misses > before[1]) == layered, (r, before, (hits, misses))
