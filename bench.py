@@ -84,7 +84,7 @@ def copy_bandwidth(ctx, mib: int = 1024, reps: int = 10):
     return 2.0 * nb / best / 1e9
 
 
-def bench_batch(T, S, ctx, reps: int = 3, tune: bool = True):
+def bench_batch(T, S, ctx, reps: int = 3, tune: bool = True, cpu: bool = True):
     """The 256 uneven tiles (log-normal point counts, grids 256^2..2048^2,
     half uniform, half clustered): the plain form (one tile per launch
     sequence, synchronised) against the batched form (one bucket sort + one
@@ -140,6 +140,23 @@ def bench_batch(T, S, ctx, reps: int = 3, tune: bool = True):
     for d in dev:
         for a in d:
             a.free()
+    # the CPU oracle over every tile, timed (the CPU baseline of this
+    # workload in full), and every batched tile against its exact heights
+    cpu_base = None
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        o_s, bits = 0.0, 0
+        for t, h in zip(tiles, batch_out):
+            a = time.perf_counter()
+            ref = O.run(t.params, t.X, t.Y, t.Z, nthreads=threads)
+            o_s += time.perf_counter() - a
+            bits += int(np.count_nonzero(h.view(np.uint32) != ref["h_fixed"].view(np.uint32)))
+        cpu_base = {"value": cells / o_s, "unit": "cells/s", "cores": threads, "kind": "port",
+                    "sample": f"all {len(tiles)} tiles, {o_s:.1f} s",
+                    "batched_bits_differ_from_oracle": bits}
+        log(f"CPU oracle over the {len(tiles)} tiles: {o_s:.1f}s, {bits} height bits differ from the batched form")
 
     def pct(v):
         return {"p50": float(np.percentile(v, 50) * 1e3), "p99": float(np.percentile(v, 99) * 1e3)}
@@ -163,6 +180,7 @@ def bench_batch(T, S, ctx, reps: int = 3, tune: bool = True):
                                       for e in tuned_b]} if tune else None,
         "batched_equals_plain": bool(same(plain_out, batch_out) and same(plain_out, plain_t_out)
                                      and same(plain_out, batch_t_out)),
+        "cpu_baseline": cpu_base,
     }, tiles, ctx.tune_table()
 
 
@@ -714,7 +732,7 @@ def main():
         d.free()
     pool, table = None, None
     if not args.no_batch:
-        out["batched_vs_plain"], pool, table = bench_batch(T, S, ctx, tune=not args.no_tune)
+        out["batched_vs_plain"], pool, table = bench_batch(T, S, ctx, tune=not args.no_tune, cpu=not args.no_cpu)
     if not args.no_served:
         if pool is None:
             pool = S.workload_batch()
