@@ -184,6 +184,39 @@ def bench_batch(T, S, ctx, reps: int = 3, tune: bool = True, cpu: bool = True):
     }, tiles, ctx.tune_table()
 
 
+def bench_uniform_100k(T, S, ctx, steps: int = 20, cpu: bool = True):
+    """The reference's own CPU-runnable case: 100k uniform points in 1000 m,
+    a 512^2 DEM at 2 m, scale 5 (2584^2 pixels), plain form on device
+    buffers; the CPU oracle on the same tile and the height bits that
+    differ from it."""
+    t = S.workload_uniform_100k()
+    dX, dY, dZ = ctx.upload(t.X), ctx.upload(t.Y), ctx.upload(t.Z)
+    dH = ctx.alloc((t.params.nrows, t.params.ncols), np.float32)
+    for _ in range(3):
+        ctx.tile_run_dev(t.params, dX, dY, dZ, t.n, dH)
+    s0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.tile_run_dev(t.params, dX, dY, dZ, t.n, dH)
+    g_s = (time.perf_counter() - s0) / steps
+    h = dH.to_host()
+    for a in (dX, dY, dZ, dH):
+        a.free()
+    out = {"workload": "100k uniform points, 512x512 DEM at 2 m, scale 5", "points": t.n,
+           "cells": t.cells, "gpu_ms": g_s * 1e3, "gpu_cells_per_s": t.cells / g_s}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        s0 = time.perf_counter()
+        ref = O.run(t.params, t.X, t.Y, t.Z, nthreads=threads)
+        o_s = time.perf_counter() - s0
+        out["cpu_baseline"] = {"value": t.cells / o_s, "unit": "cells/s", "cores": threads, "kind": "port",
+                               "sample": f"the same tile, {o_s:.2f} s"}
+        out["bits_differ_from_oracle"] = int(np.count_nonzero(h.view(np.uint32) != ref["h_fixed"].view(np.uint32)))
+    log(f"uniform 100k: {out}")
+    return out
+
+
 def bench_served(T, tiles, rates, n_req, resident: bool, seed=4, tune_table=None):
     """Served form: Poisson arrivals at each rate of tiles drawn from the
     uneven-tile pool; one thread submits, another waits in order.  resident:
@@ -731,6 +764,7 @@ def main():
     for d in (dX, dY, dZ, dH):
         d.free()
     pool, table = None, None
+    out["uniform_100k"] = bench_uniform_100k(T, S, ctx, cpu=not args.no_cpu)
     if not args.no_batch:
         out["batched_vs_plain"], pool, table = bench_batch(T, S, ctx, tune=not args.no_tune, cpu=not args.no_cpu)
     if not args.no_served:
