@@ -4,7 +4,8 @@ Each case draws, from its own seed: cell size, scale, site and query radius,
 region shape (planes / cones), height mode, bucket and cell-tile sizes,
 pixel offset, grid shape (rows may differ from columns), point count and
 distribution (uniform, clustered, a few, none), and for a quarter of the
-cases a space-time slice.  Labels and their squared distances must be
+cases a space-time tile, run with its points in random time order and
+sorted by time.  Labels and their squared distances must be
 bit-equal to the oracle's, heights bit-equal to the exact formula (int64
 fixed point, S0) or within 1e-5 relative of FP64 (FP32 two-sum), the
 NODATA sets equal.  A frame the fixed-point grid cannot hold must be
@@ -97,10 +98,14 @@ def test_random_geometry_matches_oracle(ctx, k):
         assert np.array_equal(d2, ref["d2"])
         check_heights(mode, h, ref)
     else:
+        # the points in random time order (every slice searched), then
+        # sorted by time (per-time layers, slices combined from them)
         times = np.random.default_rng(c["seed"]).integers(1998, 2005, c["n"]).astype(np.int32)
-        h, p = run_or_auto(lambda q: ctx.tile_run_time(q, X, Y, Z, times, 2001.0, c["time_r"]), p)
-        ref = O.run_time(p, X, Y, Z, times, 2001.0, c["time_r"], s0=mode == T.ACC_S0)
-        check_heights(mode, h, ref)
+        o = np.argsort(times, kind="stable")
+        for Xs, Ys, Zs, Ts in ((X, Y, Z, times), (X[o], Y[o], Z[o], times[o])):
+            h, p = run_or_auto(lambda q: ctx.tile_run_time(q, Xs, Ys, Zs, Ts, 2001.0, c["time_r"]), p)
+            ref = O.run_time(p, Xs, Ys, Zs, Ts, 2001.0, c["time_r"], s0=mode == T.ACC_S0)
+            check_heights(mode, h, ref)
 
 
 def test_sweep_covers_both_search_modes_and_time():
