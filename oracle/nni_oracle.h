@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
  * calls this code.  It is used by tests/, by __graft_entry__.smoke() and by
- * bench.py's cpu_baseline leg, and there only as the checker (or as the timed
+ * bench.py's cpu_baseline leg, and there only to check results (or as the timed
  * CPU baseline), never as the thing measured or shipped.
  *
  * It restates, on the CPU and in exact integer arithmetic, what TerraNNI's

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/nni_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.  The
+cpu_baseline leg of bench.py, to check results or as the timed CPU baseline.  The
 product library never imports it.
 """
 from __future__ import annotations
