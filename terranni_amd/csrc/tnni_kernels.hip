@@ -563,6 +563,49 @@ __device__ __forceinline__ void square_global(const TileDev &t, const uint32_t *
     }
 }
 
+// Visit every G-th site of [s, e) from s + sub: the G lanes of a group
+// (sub = 0 .. G-1) share one range, neighbouring lanes reading neighbouring
+// records.
+template <int ALG, bool TIME>
+__device__ __forceinline__ void scan_range_part(uint32_t s, uint32_t e, uint32_t sub, uint32_t G,
+                                                const int4 *__restrict__ sites, int px, int py, int R,
+                                                uint32_t R2, uint64_t &best, int &bval)
+{
+    uint32_t i = s + sub;
+    for (; i + G < e; i += 2 * G) {
+        const int4 a = sites[i];
+        const int4 b = sites[i + G];
+        visit_site<ALG, TIME>(a, px, py, R, R2, best, bval);
+        visit_site<ALG, TIME>(b, px, py, R, R2, best, bval);
+    }
+    if (i < e) visit_site<ALG, TIME>(sites[i], px, py, R, R2, best, bval);
+}
+
+// square_global's sites shared by the G lanes of a group: lane sub visits
+// every G-th site of each bucket row's range.
+template <int ALG, bool TIME>
+__device__ __forceinline__ void square_global_part(const TileDev &t, const uint32_t *__restrict__ offs,
+                                                   const int4 *__restrict__ sorted, int bx, int by, int R,
+                                                   int px, int py, int Rr, uint32_t sub, uint32_t G,
+                                                   uint64_t &best, int &bval)
+{
+    const uint32_t *base = offs + t.bk_base;
+    const int c0 = max(bx - R, 0), c1 = min(bx + R, t.nbx - 1);
+    const int r0 = max(by - R, 0), r1 = min(by + R, t.nby - 1);
+    for (int row = r0; row <= r1; row += 4) {
+        uint32_t a[4], e[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t *rs = base + (int64_t)min(row + q, r1) * t.nbx;
+            a[q] = row + q <= r1 ? rs[c0] : 0u;
+            e[q] = row + q <= r1 ? rs[c1 + 1] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            scan_range_part<ALG, TIME>(a[q], e[q], sub, G, sorted, px, py, Rr, t.Rs2, best, bval);
+    }
+}
+
 // k / bm for a pixel index 0 <= k < 2^20: multiply by ceil(2^32 / bm)
 // (exact since k * (ceil(2^32/bm) * bm - 2^32) < 2^32); bm == 1 has no
 // 32-bit multiplier and is taken as is.
@@ -759,7 +802,7 @@ template <int ALG, bool TIME>
 __device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *__restrict__ offs,
                                              const uint8_t *__restrict__ edist,
                                              const int4 *__restrict__ sorted, int k, int l,
-                                             uint64_t &best, int &bval)
+                                             uint32_t sub, uint32_t G, uint64_t &best, int &bval)
 {
     const int bs = t.bs;
     const int bx = t.bK + div_bm(t, k), by = t.bK + div_bm(t, l);
@@ -799,7 +842,8 @@ __device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *_
     // bucket row, the row offsets loaded four rows at a time): the rings
     // inside it are empty (below r0) or already visited (0 and 1, a few
     // sites at this density), and a site seen twice changes no minimum
-    if (rl >= max(r0, 2)) square_global<ALG, TIME>(t, offs, sorted, bx, by, rl, px, py, R, best, bval);
+    if (rl >= max(r0, 2))
+        square_global_part<ALG, TIME>(t, offs, sorted, bx, by, rl, px, py, R, sub, G, best, bval);
 }
 
 template <int MODE>
@@ -885,7 +929,7 @@ struct Acc {
 
 template <int ALG, int MODE, bool TIME>
 __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, uint32_t *smem,
-                                              uint32_t *s_next, uint32_t *s_npend,
+                                              uint32_t *s_next, uint32_t *s_npend, uint32_t share,
                                               const uint32_t *__restrict__ offs,
                                               const uint8_t *__restrict__ edist,
                                               const int4 *__restrict__ sorted)
@@ -944,15 +988,36 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
     }
     __syncthreads();
 
-    // 1b. the pixels whose search goes past ring 1, one per thread
+    // 1b. the pixels whose search goes past ring 1.  Where there are few
+    //     of them (a sparse stretch beside a cluster: each one's square
+    //     may hold thousands of sites), G lanes of a wave (a power of two)
+    //     share each pixel's square and the group takes the least key of
+    //     its lanes: the least (d2, index) is the same whichever lane met
+    //     it.  Groups cover up to share times the workgroup's lanes in
+    //     pixels.  Ring-by-ring searches (dense tiles) keep one pixel a
+    //     thread
     const uint32_t npend = *s_npend;
-    for (uint32_t q = threadIdx.x; q < npend; q += kBlock) {
+    uint32_t G = 1;
+    if (!t.dense)
+        while (G < 64 && npend * (2 * G) <= share * (uint32_t)kBlock) G *= 2;
+    const uint32_t sub = threadIdx.x & (G - 1), ngrp = kBlock / G;
+    for (uint32_t q = threadIdx.x / G; q < npend; q += ngrp) {
         const int p = s_pend[q];
         const int k = g.rk0 + p % RW, l = g.rl0 + p / RW;
         uint64_t best = ((uint64_t)s_d2[p] << 32) | (uint32_t)s_val[p];
-        int bval = (!TIME && best != ~0ull) ? site_value(t, t.Z[(uint32_t)best]) : 0;
-        finish_pixel<ALG, TIME>(t, offs, edist, sorted, k, l, best, bval);
-        if (TIME) bval = best == ~0ull ? 0 : site_value(t, t.Z[(uint32_t)best]);
+        int bval = (!TIME && G == 1 && best != ~0ull) ? site_value(t, t.Z[(uint32_t)best]) : 0;
+        finish_pixel<ALG, TIME>(t, offs, edist, sorted, k, l, sub, G, best, bval);
+        if (G > 1) {
+            // every lane of the group is here (same q, same trip count)
+            for (uint32_t m = G >> 1; m; m >>= 1) {
+                const uint32_t ohi = (uint32_t)__shfl_xor((int)(uint32_t)(best >> 32), (int)m, 64);
+                const uint32_t olo = (uint32_t)__shfl_xor((int)(uint32_t)best, (int)m, 64);
+                const uint64_t o = ((uint64_t)ohi << 32) | olo;
+                best = o < best ? o : best;
+            }
+            if (sub != 0) continue;
+        }
+        if (TIME || G > 1) bval = best == ~0ull ? 0 : site_value(t, t.Z[(uint32_t)best]);
         const uint32_t d2 = (uint32_t)(best >> 32);
         s_d2[p] = d2;
         s_val[p] = bval;
@@ -1015,6 +1080,11 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
     __shared__ uint32_t s_npend;  // its pixels still searching past ring 1
     const int64_t n_items = b->n_items;
     const int n_tiles = b->n_tiles;
+    // Fewer than 8 items per resident workgroup: the launch's time is its
+    // slowest items, so a long search is shared more widely (several
+    // pixels per lane group in turn) to shorten the heaviest ones; with
+    // more items the machine is full and one pass of groups does less work
+    const uint32_t share = n_items < 8 * (int64_t)gridDim.x ? 4u : 1u;
     for (;;) {
         if (threadIdx.x == 0) {
             s_item = atomicAdd(&b->work_counter, 1u);
@@ -1036,7 +1106,7 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
         const TileDev t = tiles[lo];
         const int64_t local = item - t.item_base;
         const int tj = (int)(local / t.ntx), ti = (int)(local % t.ntx);
-        process_block<ALG, MODE, TIME>(t, ti, tj, smem, &s_next, &s_npend, offs, edist, sorted);
+        process_block<ALG, MODE, TIME>(t, ti, tj, smem, &s_next, &s_npend, share, offs, edist, sorted);
         __syncthreads();
     }
 }

@@ -116,6 +116,37 @@ def test_exact_duplicates_take_lowest_index(ctx):
     check_tile(ctx, t.params, X, Y, Z)
 
 
+@pytest.mark.parametrize("seed,grid,cs,scale,lattice,bucket", [
+    (11, 512, 0.5, 1, True, 0),
+    (12, 512, 0.5, 1, False, 0),
+    (13, 256, 1.0, 2, True, 3),
+    (14, 384, 0.5, 1, False, 2),
+])
+def test_cluster_beside_sparse_stretch(ctx, seed, grid, cs, scale, lattice, bucket):
+    """A few pixels between sparse sites whose search squares reach a dense
+    cluster: the long searches that lane groups share.  A lattice cluster
+    in shuffled order puts equal distances on different lanes of a group,
+    so the lower index must still win."""
+    _say("cluster beside sparse stretch", seed, grid, cs, scale, lattice, bucket)
+    rng = np.random.default_rng(seed)
+    ext = grid * cs
+    border = 5 * cs + 5.0
+    c = rng.uniform(0.3 * ext, 0.7 * ext, 2)
+    if lattice:
+        g = np.arange(-70, 70) * 0.03
+        cx, cy = (a.ravel() for a in np.meshgrid(c[0] + g, c[1] + g))
+    else:
+        cx, cy = c[0] + rng.normal(0, 1.0, 20000), c[1] + rng.normal(0, 1.0, 20000)
+    sx, sy = rng.uniform(-border, ext + border, (2, 60))
+    x, y = np.concatenate([cx, sx]), np.concatenate([cy, sy])
+    o = rng.permutation(x.shape[0])
+    x, y = x[o], y[o]
+    z = rng.normal(100.0, 5.0, x.shape[0])
+    X, Y, Z = S._to_las(x, y, z, (0.0, 0.0))
+    p = S._params(grid, grid, cs, scale, (0.0, 0.0), bucket_px=bucket)
+    check_tile(ctx, p, X, Y, Z)
+
+
 def test_single_4m_tile_labels_and_heights(ctx):
     """The 4096 x 4096 tile with 4M clustered points, labels emitted."""
     _say("4M-point 4096^2 tile")
