@@ -639,6 +639,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (built-in bucket and cell-tile sizes)")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-ref-case", action="store_true",
+                    help="skip the 100k-point 512^2 case (profiler runs of the 4M tile)")
     ap.add_argument("--served-rates", default="50,200,500,1000,2000,0",
                     help="offered tiles/s (0: saturated, back-to-back submits)")
     ap.add_argument("--served-requests", type=int, default=300,
@@ -764,7 +766,8 @@ def main():
     for d in (dX, dY, dZ, dH):
         d.free()
     pool, table = None, None
-    out["uniform_100k"] = bench_uniform_100k(T, S, ctx, cpu=not args.no_cpu)
+    if not args.no_ref_case:
+        out["uniform_100k"] = bench_uniform_100k(T, S, ctx, cpu=not args.no_cpu)
     if not args.no_batch:
         out["batched_vs_plain"], pool, table = bench_batch(T, S, ctx, tune=not args.no_tune, cpu=not args.no_cpu)
     if not args.no_served:
