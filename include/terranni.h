@@ -113,7 +113,9 @@ tnni_status tnni_ctx_create(const tnni_ctx_opts *opts, tnni_ctx **out);
 /* Replaces nni::cleanup / cuda_nni::cleanupCUDA (nni.cpp:563-566,
  * cuda_nni.cu:624-632).  Waits for outstanding requests. */
 void tnni_ctx_destroy(tnni_ctx *ctx);
-/* Last error message of this thread (or of ctx when ctx != NULL). */
+/* Last error message of this thread (or of ctx when ctx != NULL: the last
+ * error any caller thread met on ctx).  The string belongs to the calling
+ * thread and stays valid until its next tnni_last_error call. */
 const char *tnni_last_error(const tnni_ctx *ctx);
 int tnni_abi_version(void);
 

@@ -83,6 +83,21 @@ def build_driver(verbose: bool = False) -> str:
     return out
 
 
+def build_stage_check(verbose: bool = False) -> str:
+    """tests/cpp/_build/stage_bounds_test: a host program (no device code)
+    that emulates the tile kernel's stage and label-tile indexing over the
+    engine's own tile layout (prepare_tile, in libterranni_hip.so)."""
+    tdir = os.path.join(ROOT, "tests", "cpp", "_build")
+    os.makedirs(tdir, exist_ok=True)
+    exe = os.path.join(tdir, "stage_bounds_test")
+    _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+          os.path.join(ROOT, "tests", "cpp", "stage_bounds_test.cpp"), "-o", exe + ".tmp",
+          "-L", LIB_DIR, "-lterranni_hip", "-Wl,-rpath," + LIB_DIR,
+          "-Wl,-rpath,$ORIGIN/../../../terranni_amd/_lib"], verbose)
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 def build_oracle(verbose: bool = False):
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], verbose)
 
@@ -91,6 +106,7 @@ def build_all(verbose: bool = False):
     build_lib(verbose)
     build_nni_api(verbose)
     build_driver(verbose)
+    build_stage_check(verbose)
     build_oracle(verbose)
 
 

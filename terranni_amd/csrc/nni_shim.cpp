@@ -29,6 +29,39 @@ std::string g_err;
 tpie::cpu_timer g_t_read, g_t_tile, g_t_write;
 bool g_have_origin64 = false;
 double g_origin64_x = 0.0, g_origin64_y = 0.0;
+// Heights go to the engine as integers of a unit chosen from the data of
+// the current work package (offset 0), as the reference picks its
+// zMultiplier from the data's range and decimal precision
+// (terranni.cpp:674-681):
+//   - the coarsest decimal unit 10^-k (k = 0..9) of which every height read
+//     so far is a whole multiple (heights that came from LAS integers with
+//     a decimal scale), if the largest |height| fits in zlim units: exact;
+//   - otherwise 2^e metres, e the smallest with every |height| so far within
+//     zlim units: at most half a unit off, 2^-31 of the largest |height|
+//     (2^-24 for FP32 two-sum).
+// zlim is 2^30, or 2^23 for FP32 two-sum (exact sums below 2^24).  The unit
+// only coarsens within a package, so its calls keep one unit (the
+// space-time cache is keyed by the tile descriptor) unless a later call
+// brings a larger or finer height.  clearRenderbuffers (called after each
+// work package) and a new work package start again.
+struct ZUnit {
+    bool any = false;     // a height has been seen in this package
+    double zmax = 0.0;    // largest |height| so far
+    bool decimal = true;  // every height so far a multiple of 10^-dec_k
+    int dec_k = 0;
+};
+ZUnit g_zu;
+nni::WorkPackage g_z_wp;
+bool g_z_wp_set = false;
+constexpr int kMaxDecimals = 9;
+
+// Whether z is a whole multiple of 10^-k (to 1e-6 of that unit, far below
+// any rounding a decimal-scaled LAS height carries).
+bool decimal_at(double z, int k)
+{
+    const double u = z * std::pow(10.0, k);
+    return std::fabs(u - std::rint(u)) <= 1e-6;
+}
 
 void set_error(int st, const std::string &msg)
 {
@@ -85,6 +118,7 @@ void setupRun(const WorkPackage &, int, int) {}
 void clearRenderbuffers()
 {
     if (g_cache) (void)tnni_time_cache_clear(g_cache);
+    g_zu = ZUnit();
 }
 
 void printTimes()
@@ -171,7 +205,7 @@ void performNNI(const WorkPackage &wp, tpie::ami::stream<OutPoint> &out,
     }
 
     // 2. the tile on the engine's grid: coordinates in its own unit
-    //    relative to the tile origin, heights in millimetres
+    //    relative to the tile origin, heights in the package's unit
     tnni_tile_desc d;
     std::memset(&d, 0, sizeof d);
     d.origin_x = have64 ? g_origin64_x : (double)wp.origin_x;
@@ -195,17 +229,38 @@ void performNNI(const WorkPackage &wp, tpie::ami::stream<OutPoint> &out,
         return;
     }
     const double upm = (double)f.upm;
+    const size_t n = pts.size();
+    if (g_z_wp_set && std::memcmp(&g_z_wp, &wp, sizeof wp) != 0) g_zu = ZUnit();
+    g_z_wp = wp;
+    g_z_wp_set = true;
+    for (size_t i = 0; i < n; ++i) {
+        const double z = pts[i].p[2];
+        if (!std::isfinite(z)) {
+            g_t_read.stop();
+            set_error(TNNI_E_INVALID, "a point height is not finite");
+            return;
+        }
+        g_zu.any = true;
+        g_zu.zmax = std::max(g_zu.zmax, std::fabs(z));
+        while (g_zu.decimal && !decimal_at(z, g_zu.dec_k))
+            if (++g_zu.dec_k > kMaxDecimals) g_zu.decimal = false;
+    }
+    const double zlim = g_mode == TNNI_ACC_FP32_2SUM ? 8388608.0 : 1073741824.0;
+    const bool use_dec = g_zu.decimal && g_zu.zmax * std::pow(10.0, g_zu.dec_k) <= zlim;
+    int zexp = -64;
+    while (!use_dec && zexp < 64 && g_zu.zmax > std::ldexp(zlim, zexp)) ++zexp;
+    const double zmul = use_dec ? std::pow(10.0, g_zu.dec_k) : std::ldexp(1.0, -zexp);
     d.las_scale[0] = d.las_scale[1] = 1.0 / upm;
-    d.las_scale[2] = 1e-3;
+    d.las_scale[2] = use_dec ? 1.0 / zmul : std::ldexp(1.0, zexp);
     d.las_offset[0] = d.origin_x;
     d.las_offset[1] = d.origin_y;
     d.las_offset[2] = 0.0;
-    const size_t n = pts.size();
     std::vector<int32_t> X(n), Y(n), Z(n), T(space_time ? n : 0);
     for (size_t i = 0; i < n; ++i) {
         X[i] = clamp_i32(std::llround((pts[i].p[0] - d.origin_x) * upm));
         Y[i] = clamp_i32(std::llround((pts[i].p[1] - d.origin_y) * upm));
-        Z[i] = clamp_i32(std::llround(pts[i].p[2] * 1e3));
+        // |Z| <= zlim; rint rounds to nearest, ties to even
+        Z[i] = (int32_t)std::rint(pts[i].p[2] * zmul);
         if (space_time) T[i] = pts[i].t;
     }
     g_t_read.stop();

@@ -27,6 +27,12 @@ using namespace tnni;
 namespace {
 
 thread_local std::string g_thread_err;
+// The scheduler thread reports errors through its requests only: it never
+// writes the context's message, which caller threads read and write.
+thread_local bool g_scheduler_thread = false;
+// tnni_last_error(ctx) hands out a copy of the context's message, owned by
+// the calling thread (valid until its next tnni_last_error call).
+thread_local std::string g_err_copy;
 
 double now_s()
 {
@@ -188,6 +194,7 @@ struct tnni_ctx {
     int device = 0;
     int n_cu = 0;
     std::mutex mu;  // the synchronous calls and the main arena
+    mutable std::mutex err_mu;  // err: written and read by any caller thread
     std::string err;
     uint64_t budget = 0;
     std::mutex mem_mu;
@@ -233,10 +240,16 @@ struct tnni_ctx {
 
 namespace {
 
+void set_ctx_error(tnni_ctx *ctx, const std::string &msg)
+{
+    std::lock_guard<std::mutex> lk(ctx->err_mu);
+    ctx->err = msg;
+}
+
 tnni_status fail(tnni_ctx *ctx, tnni_status st, const std::string &msg)
 {
-    if (ctx) ctx->err = msg;
     g_thread_err = msg;
+    if (ctx && !g_scheduler_thread) set_ctx_error(ctx, msg);
     return st;
 }
 
@@ -615,6 +628,7 @@ tnni_status launch_batch(tnni_ctx *ctx, Slot &s)
 
 void scheduler(tnni_ctx *ctx)
 {
+    g_scheduler_thread = true;
     (void)hipSetDevice(ctx->device);
     std::unique_lock<std::mutex> lk(ctx->smu);
     for (;;) {
@@ -693,8 +707,14 @@ void scheduler(tnni_ctx *ctx)
             }
             if (st == TNNI_OK) ctx->budget_full_trim = false;
             if (st != TNNI_OK) {
+                // copies and the graph may already be on the slot's stream:
+                // let them end before the requests' pinned buffers go back
+                // to the pool (tnni_wait releases them)
+                lk.unlock();
+                (void)hipStreamSynchronize(s.a.stream);
+                lk.lock();
                 for (auto &r : s.reqs) {
-                    r->err = ctx->err;
+                    r->err = g_thread_err;
                     finish_request(ctx, r, st);
                 }
                 ctx->inflight_bytes -= s.bytes;
@@ -759,8 +779,10 @@ int tnni_abi_version(void) { return TNNI_ABI_VERSION; }
 
 const char *tnni_last_error(const tnni_ctx *ctx)
 {
-    if (ctx) return ctx->err.c_str();
-    return g_thread_err.c_str();
+    if (!ctx) return g_thread_err.c_str();
+    std::lock_guard<std::mutex> lk(ctx->err_mu);
+    g_err_copy = ctx->err;
+    return g_err_copy.c_str();
 }
 
 tnni_status tnni_frame(const tnni_tile_desc *desc, tnni_frame_info *out)
@@ -1495,7 +1517,11 @@ tnni_status tnni_wait(tnni_ctx *ctx, uint64_t ticket, double *latency_s)
         r->t_done = std::max(r->t_done, now_s());  // results in the caller's buffers
     }
     if (latency_s) *latency_s = r->t_done - r->t_submit;
-    if (r->status != TNNI_OK) ctx->err = "request failed" + (r->err.empty() ? std::string() : ": " + r->err);
+    if (r->status != TNNI_OK) {
+        const std::string m = "request failed" + (r->err.empty() ? std::string() : ": " + r->err);
+        g_thread_err = m;
+        set_ctx_error(ctx, m);
+    }
     return r->status;
 }
 

@@ -11,6 +11,12 @@
 namespace tnni {
 namespace {
 
+// ceil(2^32 / d) for d >= 2; 0 marks d == 1 (no 32-bit multiplier)
+uint32_t div_magic_of(uint32_t d)
+{
+    return d <= 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + d - 1) / d);
+}
+
 int64_t isqrt64(int64_t v)
 {
     if (v <= 0) return 0;
@@ -295,13 +301,13 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
     const int64_t br_pts = std::max<int64_t>(br_min, kBandPts / per_row);
     t.br = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(nby, kBandBuckets / nbx), br_pts));
     t.nbands = (int)((nby + t.br - 1) / t.br);
-    t.inv_br = 1.0 / (double)t.br;
+    t.br_magic = div_magic_of((uint32_t)t.br);
     if (nbx > kBandBuckets || t.nbands > kMaxBands) {
         err = "bucket grid too large for the band sort: raise bucket_px";
         return TNNI_E_GEOMETRY;
     }
     t.bm_magic = bm == 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)bm - 1) / (uint64_t)bm);
-    t.inv_bs = 1.0 / (double)bs;
+    t.bs_magic = div_magic_of((uint32_t)bs);
     // A site of ring 0 or 1 around a pixel's bucket is within 2*bs of the
     // pixel on both axes (strictly less than 2*bs).
     t.ring1_covered = d.algorithm == TNNI_ALG_PLANES ? (2 * bs <= f.Rs) : (8 * bs * bs <= f.Rs2);

@@ -49,14 +49,14 @@ struct TileDev {
     uint32_t bm_magic;          // ceil(2^32 / bm), 0 for bm == 1
     int ring1_covered;          // every site of rings 0-1 covers the pixel
     int dense;                  // buckets hold several sites: ring-by-ring searches
-    double inv_bs;              // 1 / bs
+    uint32_t bs_magic;          // ceil(2^32 / bs), 0 for bs == 1
+    uint32_t br_magic;          // ceil(2^32 / br), 0 for br == 1
     int64_t bk_base;            // first bucket of this tile in the batch
     int64_t e_base;             // first byte of its distance maps (4-aligned)
     int nbx4;                   // bytes per row of the distance maps (nbx rounded up to 4)
     int pad3;
     int64_t row_base;           // first bucket row of this tile in the batch
     int br, nbands;             // bucket rows per band, bands
-    double inv_br;              // 1 / br
     int64_t band_base;          // first band of this tile in the batch
     int64_t hist_base;          // first band-histogram entry of this tile
     int chunk_base, nchunks;    // this tile's chunks of points
@@ -77,6 +77,24 @@ struct TileDev {
     int32_t *out_labels;
     uint32_t *out_d2;
 };
+
+// Per-wave LDS staging of the tile kernel: the sites around one 8x8 block
+// of pixels, i.e. the block's buckets plus one bucket on every side (rings
+// 0 and 1 of every pixel of the block).  A block spans at most 8 buckets a
+// side (buckets are at least one pixel), so the window is at most kWinMax
+// buckets a side; woff holds each window row's bucket offsets plus the end
+// of its last bucket (kWinMax + 1 a row).
+constexpr int kWinMax = 10;      // window side in buckets (8 pixels + 2)
+constexpr int kStageCap = 384;   // staged sites per wave
+template <int CAP>
+struct WaveStageT {
+    uint32_t woff[kWinMax * (kWinMax + 1)];  // bucket offsets of the window rows
+    uint32_t rowpre[kWinMax];                // first staged slot of each row
+    uint32_t pad[2];
+    int4 sites[CAP];
+};
+using WaveStage = WaveStageT<kStageCap>;
+static_assert(sizeof(WaveStage) % 16 == 0, "WaveStage must keep int4 alignment");
 
 // A run of points of one tile, the unit of work of the sort passes.
 struct Chunk {

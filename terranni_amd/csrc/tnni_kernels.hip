@@ -61,26 +61,25 @@ __device__ __forceinline__ bool slice_site(const TileDev &t, int32_t Ti, uint32_
     return in;
 }
 
-// v / d for 0 <= v < 2^30: FP64 reciprocal, then at most one correction.
-__device__ __forceinline__ int div_pos(int v, int d, double inv_d)
+// v / d for 0 <= v < 2^30 and d < 2^30, in integers: with magic =
+// ceil(2^32 / d) = (2^32 + e) / d, 0 <= e < d, v * magic / 2^32 exceeds v / d
+// by v * e / (d * 2^32) < 1/4, so its integer part is the quotient or one
+// more, and one multiply-compare corrects it.  magic == 0 marks d == 1.
+__device__ __forceinline__ int div_magic(int v, int d, uint32_t magic)
 {
-    int q = (int)((double)v * inv_d);
+    if (!magic) return v;
+    int q = (int)__umulhi((uint32_t)v, magic);
     if (q * d > v) --q;
-    else if ((q + 1) * d <= v) ++q;
     return q;
 }
 
-__device__ __forceinline__ uint32_t bucket_of(const TileDev &t, int xu, int yu)
-{
-    const int bx = div_pos(xu - t.gx0, t.bs, t.inv_bs);
-    const int by = div_pos(yu - t.gy0, t.bs, t.inv_bs);
-    return (uint32_t)by * (uint32_t)t.nbx + (uint32_t)bx;
-}
+__device__ __forceinline__ int bucket_col(const TileDev &t, int xu) { return div_magic(xu - t.gx0, t.bs, t.bs_magic); }
+__device__ __forceinline__ int bucket_row(const TileDev &t, int yu) { return div_magic(yu - t.gy0, t.bs, t.bs_magic); }
 
 // Bucket row band of a kept site: bands of br bucket rows.
 __device__ __forceinline__ int point_band(const TileDev &t, int yu)
 {
-    return div_pos(div_pos(yu - t.gy0, t.bs, t.inv_bs), t.br, t.inv_br);
+    return div_magic(bucket_row(t, yu), t.br, t.br_magic);
 }
 
 constexpr int kPtsPerStep = 4;  // points per thread with their loads in flight
@@ -327,8 +326,8 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
             for (int q = 0; q < kSortRecs; ++q) {
                 key[q] = 0u;
                 if (threadIdx.x + (uint32_t)q * kSortBlock < n) {
-                    const int bx = div_pos(rec[q].y - t.gx0, t.bs, t.inv_bs);
-                    const int by = div_pos(rec[q].z - t.gy0, t.bs, t.inv_bs);
+                    const int bx = bucket_col(t, rec[q].y);
+                    const int by = bucket_row(t, rec[q].z);
                     const uint32_t bk = (uint32_t)((by - r0) * nbx + bx);
                     key[q] = bk | (atomicAdd(&cnt[bk], 1u) << 16);
                 }
@@ -345,8 +344,8 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
 #pragma unroll
                 for (int q = 0; q < kPtsPerStep; ++q) {
                     if (p0 + (uint32_t)q * kSortBlock >= s1) break;
-                    const int bx = div_pos(r4[q].y - t.gx0, t.bs, t.inv_bs);
-                    const int by = div_pos(r4[q].z - t.gy0, t.bs, t.inv_bs);
+                    const int bx = bucket_col(t, r4[q].y);
+                    const int by = bucket_row(t, r4[q].z);
                     atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
                 }
             }
@@ -399,8 +398,8 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
 #pragma unroll
                 for (int q = 0; q < kPtsPerStep; ++q) {
                     if (p0 + (uint32_t)q * kSortBlock >= s1) break;
-                    const int bx = div_pos(r4[q].y - t.gx0, t.bs, t.inv_bs);
-                    const int by = div_pos(r4[q].z - t.gy0, t.bs, t.inv_bs);
+                    const int bx = bucket_col(t, r4[q].y);
+                    const int by = bucket_row(t, r4[q].z);
                     const uint32_t pos = atomicAdd(&cnt[(by - r0) * nbx + bx], 1u);
                     sorted[s0 + pos] = r4[q];
                 }
@@ -613,19 +612,6 @@ __device__ __forceinline__ int div_bm(const TileDev &t, int k)
 {
     return t.bm_magic ? (int)__umulhi((uint32_t)k, t.bm_magic) : k;
 }
-
-// Per-wave LDS staging of the sites around one 8x8 block of pixels: the
-// block's buckets plus one bucket on every side (rings 0 and 1 of every
-// pixel of the block).
-constexpr int kWinMax = 10;      // window side in buckets (8 pixels + 2)
-constexpr int kStageCap = 384;   // staged sites per wave
-struct WaveStage {
-    uint32_t woff[kWinMax * (kWinMax + 1)];  // bucket offsets of the window rows
-    uint32_t rowpre[kWinMax];                // first staged slot of each row
-    uint32_t pad[2];
-    int4 sites[kStageCap];
-};
-static_assert(sizeof(WaveStage) % 16 == 0, "WaveStage must keep int4 alignment");
 
 __device__ __forceinline__ void wave_sync()
 {

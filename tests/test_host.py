@@ -65,3 +65,17 @@ def test_clustered_tile_smaller_than_a_water_disc():
     import terranni_amd.synth as S
     t = S.make_tile(51, 200, 7, cell_size=0.25, scale=4, kind="clustered")
     assert t.n == 200
+
+
+def test_tile_kernel_stage_and_label_tile_indices_in_bounds():
+    """Every index the tile kernel forms into its per-wave LDS stage (window
+    offsets, row prefixes, staged slots), its label tile and search list,
+    the bucket offsets and the distance maps stays inside its buffer, for
+    every 8x8 block of every work item of the benchmark tiles, bucket and
+    cell-tile sweeps with and without the border, and 300 random
+    geometries: a host emulation over the engine's own tile layout."""
+    exe = os.path.join(ROOT, "tests", "cpp", "_build", "stage_bounds_test")
+    r = subprocess.run([exe, "300"], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stderr
+    assert "violations 0" in r.stdout
