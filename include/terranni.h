@@ -153,6 +153,26 @@ tnni_status tnni_tile_run_dev(tnni_ctx *ctx, const tnni_tile_desc *desc,
                               const int32_t *dZ, uint64_t n, float *d_out_h,
                               int32_t *d_out_labels, uint32_t *d_out_d2);
 
+/* The same as tnni_tile_run_dev (same arguments, same results bit for bit)
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  It has no
+ * counterpart in the reference.  It calls no hipMalloc, hipFree,
+ * hipHostMalloc or other device- or pinned-memory allocation: besides the
+ * caller's inputs and outputs, its scratch (descriptors, histograms, the
+ * sorted site records, bucket offsets, distance maps) lives in a fixed
+ * array of 128 MiB per device, set up when a context is first created on
+ * that device and held until the process ends.  A tile that needs more
+ * scratch (roughly 3M points and up, or a very fine bucket grid) fails with
+ * TNNI_E_BUDGET and a message giving the bytes it needs.  All calls on one
+ * device share its array and would race if they overlapped, so the library
+ * runs them one at a time per device within the process; each call writes
+ * everything it later reads, so the previous call's contents never matter.
+ * Like tnni_tile_run_dev it runs on the context stream and synchronises. */
+tnni_status tnni_tile_run_dev_noalloc(tnni_ctx *ctx, const tnni_tile_desc *desc,
+                                      const int32_t *dX, const int32_t *dY,
+                                      const int32_t *dZ, uint64_t n, float *d_out_h,
+                                      int32_t *d_out_labels, uint32_t *d_out_d2);
+
 /* ---- batched form ----------------------------------------------------- */
 /* n_tiles uneven tiles in one pipeline: one bucket sort over all points and
  * one persistent launch over a device work queue of (tile, cell-block)

@@ -171,6 +171,7 @@ def lib():
     L.tnni_ctx_destroy.restype = None
     L.tnni_tile_run.argtypes = [P, ctypes.POINTER(TileDesc), P, P, P, ctypes.c_uint64, P, P, P]
     L.tnni_tile_run_dev.argtypes = [P, ctypes.POINTER(TileDesc), P, P, P, ctypes.c_uint64, P, P, P]
+    L.tnni_tile_run_dev_noalloc.argtypes = [P, ctypes.POINTER(TileDesc), P, P, P, ctypes.c_uint64, P, P, P]
     L.tnni_batch_run_dev.argtypes = [P, ctypes.c_int, ctypes.POINTER(TileDesc), P, P, P,
                                      ctypes.POINTER(ctypes.c_uint64), P, P, P]
     L.tnni_dev_alloc.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(P)]
@@ -212,7 +213,7 @@ def lib():
     L.tnni_tile_run_time_cached_dev.argtypes = [P, P, ctypes.POINTER(TileDesc), P, P, P, P,
                                                 ctypes.c_uint64, ctypes.c_float, ctypes.c_float, P]
     for name in ("tnni_frame", "tnni_footprint", "tnni_ctx_create", "tnni_tile_run",
-                 "tnni_tile_run_dev", "tnni_batch_run_dev", "tnni_dev_alloc", "tnni_dev_free",
+                 "tnni_tile_run_dev", "tnni_tile_run_dev_noalloc", "tnni_batch_run_dev", "tnni_dev_alloc", "tnni_dev_free",
                  "tnni_h2d", "tnni_d2h", "tnni_d2d", "tnni_profile_enable", "tnni_profile_read",
                  "tnni_submit", "tnni_submit_dev", "tnni_poll", "tnni_wait",
                  "tnni_serve_stats_read", "tnni_serve_stats_reset", "tnni_shape_class", "tnni_autotune",
@@ -396,6 +397,15 @@ class Context:
         vp = lambda a: ctypes.c_void_p(a.ptr) if a is not None else None  # noqa: E731
         self._chk(lib().tnni_tile_run_dev(self.h, ctypes.byref(d), vp(dX), vp(dY), vp(dZ), n,
                                           vp(d_h), vp(d_lab), vp(d_d2)))
+
+    def tile_run_dev_noalloc(self, p: TileParams, dX: DevArray, dY: DevArray, dZ: DevArray, n: int,
+                             d_h: DevArray, d_lab: DevArray | None = None, d_d2: DevArray | None = None):
+        """tile_run_dev without any device allocation (fixed per-device
+        scratch; tiles over its size raise TnniError with status -6)."""
+        d = p.to_desc()
+        vp = lambda a: ctypes.c_void_p(a.ptr) if a is not None else None  # noqa: E731
+        self._chk(lib().tnni_tile_run_dev_noalloc(self.h, ctypes.byref(d), vp(dX), vp(dY), vp(dZ), n,
+                                                  vp(d_h), vp(d_lab), vp(d_d2)))
 
     def batch_run_dev(self, params, dXs, dYs, dZs, ns, d_hs, d_labs=None, d_d2s=None):
         k = len(params)

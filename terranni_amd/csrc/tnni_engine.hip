@@ -24,7 +24,43 @@
 
 using namespace tnni;
 
+// ---- scratch of the allocation-free entry point -----------------------------
+// tnni_tile_run_dev_noalloc keeps everything besides its caller's inputs and
+// outputs (descriptors, band histograms and scan partials, the banded and
+// sorted site records, bucket offsets, distance maps) in this array: one
+// copy of kNoAllocScratchBytes per device, held until the process ends.  In
+// host code `scratch` is only a host-side stand-in for the symbol: the
+// device address of each device's copy comes from hipGetSymbolAddress at
+// context creation (noalloc_setup) and only that address is used.
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
 namespace {
+
+// Per device: the address of its copy of `scratch`, found once with the
+// device current; mu also runs that device's allocation-free calls one at a
+// time (they share the array).
+constexpr int kNoAllocDevices = 64;
+struct NoAllocScratch {
+    std::mutex mu;
+    bool tried = false;
+    hipError_t err = hipSuccess;
+    void *ptr = nullptr;
+};
+NoAllocScratch g_noalloc[kNoAllocDevices];
+
+// Called by tnni_ctx_create with `device` current.
+void noalloc_setup(int device)
+{
+    if (device < 0 || device >= kNoAllocDevices) return;
+    NoAllocScratch &ns = g_noalloc[device];
+    std::lock_guard<std::mutex> lk(ns.mu);
+    if (ns.tried) return;
+    ns.tried = true;
+    void *ptr = nullptr;
+    ns.err = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (ns.err == hipSuccess) ns.ptr = ptr;
+}
 
 thread_local std::string g_thread_err;
 // The scheduler thread reports errors through its requests only: it never
@@ -840,6 +876,7 @@ tnni_status tnni_ctx_create(const tnni_ctx_opts *opts, tnni_ctx **out)
         return hip_fail(nullptr, e, "hipMemGetInfo");
     }
     ctx->budget = o.hbm_budget ? o.hbm_budget : (uint64_t)(0.8 * (double)fr);
+    noalloc_setup(o.device);
     tnni_status st = arena_init(ctx, ctx->main);
     if (st == TNNI_OK && ((e = hipEventCreate(&ctx->ev0)) != hipSuccess ||
                           (e = hipEventCreate(&ctx->ev1)) != hipSuccess ||
@@ -964,6 +1001,99 @@ tnni_status tnni_tile_run_dev(tnni_ctx *ctx, const tnni_tile_desc *desc, const i
     v[0].t.out_labels = d_out_labels;
     v[0].t.out_d2 = d_out_d2;
     return run_prepared(ctx, ctx->main, v, true, Launch::Direct);
+}
+
+tnni_status tnni_tile_run_dev_noalloc(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *dX,
+                                      const int32_t *dY, const int32_t *dZ, uint64_t n, float *d_out_h,
+                                      int32_t *d_out_labels, uint32_t *d_out_d2)
+{
+    if (!ctx || !desc || !d_out_h || (n && (!dX || !dY || !dZ)))
+        return fail(ctx, TNNI_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (ctx->device >= kNoAllocDevices) return fail(ctx, TNNI_E_INVALID, "device ordinal above 63");
+    NoAllocScratch &ns = g_noalloc[ctx->device];
+    std::lock_guard<std::mutex> lk_dev(ns.mu);
+    if (!ns.ptr)
+        return fail(ctx, TNNI_E_HIP, std::string("no scratch address for this device: hipGetSymbolAddress: ") +
+                                         hipGetErrorString(ns.err));
+    Prepared p;
+    std::string err;
+    tnni_status st = prepare_tile(*desc, n, ctx->n_cu, false, &ctx->tuned, p.t, p.info, err);
+    if (st != TNNI_OK) return fail(ctx, st, err);
+    TileDev &t = p.t;
+    t.X = dX;
+    t.Y = dY;
+    t.Z = dZ;
+    t.out_h = d_out_h;
+    t.out_labels = d_out_labels;
+    t.out_d2 = d_out_d2;
+    // the layout of a batch of this one tile (run_prepared's, with every
+    // base at 0): at least one chunk, a histogram entry per (band, chunk)
+    const int64_t nch = std::max<int64_t>(1, (t.n + kChunkPts - 1) / kChunkPts);
+    const int64_t bks = (int64_t)t.nbx * t.nby, hents = (int64_t)t.nbands * nch;
+    t.nbx4 = (t.nbx + 3) & ~3;
+    const int64_t ebytes = (int64_t)t.nbx4 * t.nby;
+    t.chunk_base = 0;
+    t.nchunks = (int)nch;
+    if (bks + 1 > 0xFFFFFFFFll || hents + 1 > 0xFFFFFFFFll)
+        return fail(ctx, TNNI_E_INVALID, "tile exceeds 2^32 buckets");
+    // carve the scratch: every buffer on a 256-byte boundary
+    size_t off = 0;
+    auto carve = [&off](size_t bytes) {
+        const size_t at = off;
+        off += (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+        return at;
+    };
+    const size_t o_batch = carve(sizeof(BatchDev)), o_tiles = carve(sizeof(TileDev));
+    const size_t o_chunks = carve((size_t)nch * sizeof(Chunk)), o_hist = carve((size_t)(hents + 1) * 4);
+    const size_t o_part = carve(kScanBlocks * 4), o_banded = carve((size_t)t.n * 16);
+    const size_t o_sorted = carve((size_t)t.n * 16), o_offs = carve((size_t)(bks + 1) * 4);
+    const size_t o_hdist = carve((size_t)ebytes), o_edist = carve((size_t)ebytes);
+    if (off > kNoAllocScratchBytes)
+        return fail(ctx, TNNI_E_BUDGET,
+                    "tile needs " + std::to_string(off) + " bytes of scratch; the allocation-free entry point has " +
+                        std::to_string(kNoAllocScratchBytes) + " (use tnni_tile_run_dev)");
+    char *base = (char *)ns.ptr;
+    DevBuffers db;
+    db.batch = (BatchDev *)(base + o_batch);
+    db.tiles = (const TileDev *)(base + o_tiles);
+    db.chunks = (const Chunk *)(base + o_chunks);
+    db.hist = (uint32_t *)(base + o_hist);
+    db.partials = (uint32_t *)(base + o_part);
+    db.banded = (int4 *)(base + o_banded);
+    db.sorted = (int4 *)(base + o_sorted);
+    db.offs = (uint32_t *)(base + o_offs);
+    db.hdist = (uint8_t *)(base + o_hdist);
+    db.edist = (uint8_t *)(base + o_edist);
+    BatchDev bd;
+    std::memset(&bd, 0, sizeof bd);
+    bd.n_tiles = 1;
+    bd.n_chunks = (int)nch;
+    bd.n_scan = (uint32_t)(hents + 1);
+    bd.n_items = (int64_t)t.ntx * t.nty;
+    bd.n_rows = t.nby;
+    bd.n_bands = t.nbands;
+    bd.scan_chunk = (uint32_t)((bd.n_scan + kScanBlocks - 1) / kScanBlocks);
+    // the array keeps the previous call's contents: the setup kernel writes
+    // every descriptor and k_band_count zeroes the work counter, and the
+    // passes write every histogram entry, offset and distance they read
+    Arena &a = ctx->main;
+    HIP_TRY(ctx, launch_single_tile_setup(t, bd, (TileDev *)db.tiles, (Chunk *)db.chunks, db.batch, a.stream));
+    LaunchCfg cfg;
+    int occ = tile_kernel_occupancy(p.info.lds);
+    if (p.info.wg_per_cu > 0) occ = std::min(occ, p.info.wg_per_cu);
+    cfg.main_lds = p.info.lds;
+    cfg.algorithm = t.algorithm;
+    cfg.mode = t.mode;
+    cfg.time = 0;
+    cfg.sort_grid = (int)std::min<int64_t>(4096, nch);
+    cfg.band_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, t.nbands));
+    cfg.row_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, t.nby));
+    cfg.main_grid = (int)std::max<int64_t>(1, std::min<int64_t>(bd.n_items, (int64_t)ctx->n_cu * occ));
+    HIP_TRY(ctx, enqueue_pipeline(db, cfg, a.stream, nullptr, nullptr));
+    HIP_TRY(ctx, hipStreamSynchronize(a.stream));
+    return TNNI_OK;
 }
 
 tnni_status tnni_tile_run(tnni_ctx *ctx, const tnni_tile_desc *desc, const int32_t *X,

@@ -193,6 +193,11 @@ hipError_t enqueue_pipeline(const DevBuffers &b, const LaunchCfg &cfg,
                             hipStream_t stream, hipEvent_t ev_main0,
                             hipEvent_t ev_main1);
 
+// Write the descriptors of a one-tile batch (tile, chunks, batch header)
+// into device memory with a kernel (no host staging, no copies).
+hipError_t launch_single_tile_setup(const TileDev &t, const BatchDev &bd, TileDev *tiles, Chunk *chunks,
+                                    BatchDev *batch, hipStream_t s);
+
 // Space-time year layers: a label raster turned into site values in place;
 // a slice's values and squared distances combined from the layers of its
 // times (ascending, dt2 each); the query gather of a slice (values and

@@ -1209,7 +1209,36 @@ hipError_t launch_gather_slice_t(const TileDev &t, const int32_t *val, const uin
     return hipGetLastError();
 }
 
+// The descriptors of a one-tile batch written on the device from kernel
+// arguments (no host staging): the tile, the batch header, and the tile's
+// chunks of kChunkPts points (at least one, possibly empty).
+__global__ void __launch_bounds__(kBlock)
+k_single_tile_setup(const TileDev t, const BatchDev bd, TileDev *__restrict__ tiles,
+                    Chunk *__restrict__ chunks, BatchDev *__restrict__ batch)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *tiles = t;
+        *batch = bd;
+    }
+    for (int c = blockIdx.x * kBlock + threadIdx.x; c < bd.n_chunks; c += gridDim.x * kBlock) {
+        Chunk ch;
+        ch.tile = 0;
+        ch.pad = 0;
+        ch.begin = (int64_t)c * kChunkPts;
+        ch.end = min(t.n, ch.begin + kChunkPts);
+        chunks[c] = ch;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_single_tile_setup(const TileDev &t, const BatchDev &bd, TileDev *tiles, Chunk *chunks,
+                                    BatchDev *batch, hipStream_t s)
+{
+    const int grid = std::max(1, std::min((bd.n_chunks + kBlock - 1) / kBlock, 1024));
+    hipLaunchKernelGGL(k_single_tile_setup, dim3(grid), dim3(kBlock), 0, s, t, bd, tiles, chunks, batch);
+    return hipGetLastError();
+}
 
 hipError_t launch_label_values(const TileDev &t, int32_t *lab, uint64_t pix, hipStream_t s)
 {
