@@ -31,6 +31,30 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def compact(x, sig: int = 4):
+    """x with every float rounded to `sig` significant digits (the printed
+    line stays short enough for a log tail to hold it whole)."""
+    if isinstance(x, float):
+        return float(f"{x:.{sig}g}") if np.isfinite(x) else x
+    if isinstance(x, dict):
+        return {k: compact(v, sig) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [compact(v, sig) for v in x]
+    return x
+
+
+def write_full_record(out: dict, name: str):
+    """The whole record, unrounded and with the per-class autotune tables,
+    to gpurun_out/<name> (next to the runs' other outputs)."""
+    try:
+        d = os.path.join(ROOT, "gpurun_out")
+        os.makedirs(d, exist_ok=True)
+        with open(os.path.join(d, name), "w") as f:
+            json.dump(out, f, indent=1)
+    except OSError as e:
+        log(f"could not write the full record: {e}")
+
+
 def cpu_baseline(tile, budget_s: float = 12.0):
     """The CPU oracle (site-major raster + query gather, OpenMP) on the same
     tile, repeated until about budget_s of work; returns (cells/s, threads,
@@ -161,27 +185,28 @@ def bench_batch(T, S, ctx, reps: int = 3, tune: bool = True, cpu: bool = True):
     def pct(v):
         return {"p50": float(np.percentile(v, 50) * 1e3), "p99": float(np.percentile(v, 99) * 1e3)}
 
+    choices = {"batched": [{k: e[k] for k in ("shape_class", "bucket_px", "tile_cells", "wg_per_cu",
+                                              "ms_default", "ms_tuned")} for e in tuned_b],
+               "plain": [{k: e[k] for k in ("shape_class", "bucket_px", "tile_cells", "wg_per_cu",
+                                            "ms_default", "ms_tuned")} for e in tuned_p]}
     return {
         "workload": "256 uneven tiles, log-normal points (median 200k, 10k-2M), grids "
                     "256^2-2048^2 at 0.5 m, half uniform / half clustered",
         "tiles": len(tiles), "points": pts, "cells": cells,
         "plain_cells_per_s": cells / plain_s, "plain_points_per_s": pts / plain_s,
-        "plain_tile_latency_ms": pct(lat),
+        "plain_tile_ms": pct(lat),
         "batched_cells_per_s": cells / batch_s, "batched_points_per_s": pts / batch_s,
         "batched_ms": batch_s * 1e3,
         "speedup": plain_s / batch_s,
         "tuned": {"classes_batched": len(tuned_b), "classes_plain": len(tuned_p),
                   "tuning_s": tune_s,
-                  "plain_cells_per_s": cells / plain_t_s, "plain_tile_latency_ms": pct(lat_t),
+                  "plain_cells_per_s": cells / plain_t_s, "plain_tile_ms": pct(lat_t),
                   "batched_cells_per_s": cells / batch_t_s, "batched_ms": batch_t_s * 1e3,
-                  "speedup": plain_t_s / batch_t_s,
-                  "batched_choices": [{k: e[k] for k in ("bucket_px", "tile_cells", "wg_per_cu",
-                                                          "ms_default", "ms_tuned")}
-                                      for e in tuned_b]} if tune else None,
+                  "speedup": plain_t_s / batch_t_s} if tune else None,
         "batched_equals_plain": bool(same(plain_out, batch_out) and same(plain_out, plain_t_out)
                                      and same(plain_out, batch_t_out)),
         "cpu_baseline": cpu_base,
-    }, tiles, ctx.tune_table()
+    }, tiles, ctx.tune_table(), choices
 
 
 def bench_uniform_100k(T, S, ctx, steps: int = 20, cpu: bool = True):
@@ -221,31 +246,37 @@ def bench_served(T, tiles, rates, n_req, resident: bool, seed=4, tune_table=None
     """Served form: Poisson arrivals at each rate of tiles drawn from the
     uneven-tile pool; one thread submits, another waits in order.  resident:
     the tiles' points and the output rasters live in HBM (tnni_submit_dev);
-    otherwise host buffers in and out (tnni_submit).  Latency is submit ->
-    results in device memory (resident) or in the caller's host buffers,
-    timed by the engine."""
+    otherwise host buffers in and out (tnni_submit).  Either way the caller
+    keeps a ring of 512 output rasters and reuses them, as a server would.
+    Latency is submit -> results in device memory (resident) or in the
+    caller's host buffers, timed by the engine."""
     import threading
     out = []
     rng = np.random.default_rng(seed)
-    ring = 512  # output rasters in flight (resident)
+    ring = 512  # output rasters in flight
     # admission by HBM footprint within a 200 GB budget; batches of up to 256
     # tiles (the engine's default), three in flight
     with T.Context(0, n_slots=3, max_batch_tiles=256, hbm_budget=200 << 30) as c:
         if tune_table:
             c.tune_load(tune_table)
+        big = max(t.cells for t in tiles)
         if resident:
             dev = [(c.upload(t.X), c.upload(t.Y), c.upload(t.Z)) for t in tiles]
-            big = max(t.cells for t in tiles)
             outs = [c.alloc((big,), np.float32) for _ in range(ring)]
+        else:
+            outs = [np.zeros(big, np.float32) for _ in range(ring)]
 
         def submit(k, i):
             t = tiles[i]
             if resident:
                 d = dev[i]
                 return c.submit_dev(t.params, d[0], d[1], d[2], t.n, outs[k % ring])
-            return c.submit(t.params, t.X, t.Y, t.Z)
+            return c.submit(t.params, t.X, t.Y, t.Z, out=outs[k % ring])
 
-        for p in [submit(k, k) for k in range(16)]:  # warm the slots and graphs
+        # warm the slots, graphs and (host buffers) the pinned staging pool:
+        # one pass of ring requests in flight at once
+        warm = [submit(k, k % len(tiles)) for k in range(16 if resident else ring)]
+        for p in warm:
             p.wait()
         for rate in rates:
             c.serve_stats_reset()
@@ -293,12 +324,14 @@ def bench_served(T, tiles, rates, n_req, resident: bool, seed=4, tune_table=None
             span = state["t_end"] - t0
             cells = float(sum(tiles[i].cells for i in picks))
             st = c.serve_stats()
-            out.append({"rate_tiles_per_s": rate, "requests": per_rate,
-                        "achieved_tiles_per_s": per_rate / span, "cells_per_s": cells / span,
-                        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
-                                       "p99": float(np.percentile(lat, 99) * 1e3)},
-                        "max_inflight_tiles": st["max_inflight_tiles"],
-                        "max_inflight_batches": st["max_inflight_batches"]})
+            p99 = float(np.percentile(lat, 99))
+            out.append({"rate": rate, "requests": per_rate,
+                        "tiles_per_s": per_rate / span, "cells_per_s": cells / span,
+                        "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": p99 * 1e3,
+                        "p99_tail_n": int(np.count_nonzero(np.asarray(lat) >= p99)),
+                        "max_ms": float(np.max(lat) * 1e3),
+                        "inflight_tiles": st["max_inflight_tiles"],
+                        "inflight_batches": st["max_inflight_batches"]})
             log(f"served ({'HBM-resident' if resident else 'host buffers'}) rate {rate}/s: {out[-1]}")
         if resident:
             for d in dev:
@@ -307,6 +340,48 @@ def bench_served(T, tiles, rates, n_req, resident: bool, seed=4, tune_table=None
             for o in outs:
                 o.free()
     return out
+
+
+def bench_ref100k(T, S, args):
+    """--workload ref100k: only the reference's CPU-runnable case (100k
+    uniform points in 1000 m, 512^2 DEM at 2 m, scale 5: 2584^2 pixels),
+    plain form on device buffers, K steps timed, k_tiles timed with HIP
+    events on its stream (for profiling this shape alone)."""
+    t = S.workload_uniform_100k(tile_cells=args.tile_cells, bucket_px=args.bucket_px)
+    fr = T.frame(t.params)
+    ctx = T.Context(0)
+    dX, dY, dZ = ctx.upload(t.X), ctx.upload(t.Y), ctx.upload(t.Z)
+    dH = ctx.alloc((t.params.nrows, t.params.ncols), np.float32)
+    for _ in range(max(args.warmup, 1)):
+        ctx.tile_run_dev(t.params, dX, dY, dZ, t.n, dH)
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    s0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.tile_run_dev(t.params, dX, dY, dZ, t.n, dH)
+    el = time.perf_counter() - s0
+    launches, kern_ms = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    h = dH.to_host()
+    for a in (dX, dY, dZ, dH):
+        a.free()
+    ctx.close()
+    k_ms = kern_ms / max(launches, 1)
+    pix = fr["W"] * fr["H"]
+    out = {"metric": "DEM cells/s (100k uniform points, 512^2 DEM at 2 m, scale 5, plain form)",
+           "value": t.cells * args.steps / el, "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+           "data": "synthetic (seeded uniform LiDAR, LAS int32 at 0.01 m)",
+           "config": {"workload": "100k uniform points in 1000 x 1000 m, 512x512 cells at 2 m, scale 5",
+                      "points": t.n, "cells": t.cells, "pixels": pix, "form": "plain"},
+           "pixels_per_s": pix * args.steps / el, "k_tiles_ms": k_ms}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        ref = O.run(t.params, t.X, t.Y, t.Z, nthreads=min(len(os.sched_getaffinity(0)), 16))
+        out["bits_differ_from_oracle"] = int(np.count_nonzero(h.view(np.uint32) != ref["h_fixed"].view(np.uint32)))
+    print(json.dumps(compact(out)))
 
 
 def bench_batched_only(T, S, args):
@@ -623,11 +698,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-cells", type=int, default=0, help="cell-tile side (0 = auto)")
     ap.add_argument("--bucket-px", type=int, default=0, help="bucket side in pixels (0 = auto)")
-    ap.add_argument("--workload", choices=("single", "batch", "stream", "spacetime"), default="single",
+    ap.add_argument("--workload", choices=("single", "batch", "stream", "spacetime", "ref100k"),
+                    default="single",
                     help="single: the 4096^2 tile (default line, plus the batched and served "
                          "comparisons); batch: only the batched 256-tile form; stream: the "
                          "streamed-terrain driver pipeline (down-scaled); spacetime: one work "
-                         "package stepped through time, Delta cache against Delta^2")
+                         "package stepped through time, Delta cache against Delta^2; ref100k: "
+                         "the reference's CPU case (100k uniform points, 512^2 at 2 m, scale 5) alone")
     ap.add_argument("--time-radius", type=float, default=2.0, help="spacetime: time radius")
     ap.add_argument("--st-grid", type=int, default=1024, help="spacetime: tile side in cells")
     ap.add_argument("--st-points-per-year", type=int, default=400_000)
@@ -641,9 +718,12 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-ref-case", action="store_true",
                     help="skip the 100k-point 512^2 case (profiler runs of the 4M tile)")
-    ap.add_argument("--served-rates", default="50,200,500,1000,2000,0",
-                    help="offered tiles/s (0: saturated, back-to-back submits)")
-    ap.add_argument("--served-requests", type=int, default=300,
+    ap.add_argument("--served-rates", default="50,100,200,500,1000,2000,0",
+                    help="offered tiles/s, HBM-resident requests (0: saturated, back-to-back "
+                         "submits)")
+    ap.add_argument("--served-host-rates", default="200,1000,0",
+                    help="offered tiles/s, host-buffer requests")
+    ap.add_argument("--served-requests", type=int, default=2000,
                     help="requests per arrival rate")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates "
@@ -665,6 +745,9 @@ def main():
         return
     if args.workload == "spacetime":
         bench_spacetime(T, S, args)
+        return
+    if args.workload == "ref100k":
+        bench_ref100k(T, S, args)
         return
     t0 = time.perf_counter()
     tile = S.workload_single_4m(seed=1, n=args.points, grid=args.grid,
@@ -765,21 +848,24 @@ def main():
         out["roofline"]["lds"] = lds
     for d in (dX, dY, dZ, dH):
         d.free()
-    pool, table = None, None
+    pool, table, choices = None, None, None
     if not args.no_ref_case:
         out["uniform_100k"] = bench_uniform_100k(T, S, ctx, cpu=not args.no_cpu)
     if not args.no_batch:
-        out["batched_vs_plain"], pool, table = bench_batch(T, S, ctx, tune=not args.no_tune, cpu=not args.no_cpu)
+        out["batched_vs_plain"], pool, table, choices = bench_batch(T, S, ctx, tune=not args.no_tune,
+                                                                     cpu=not args.no_cpu)
     if not args.no_served:
         if pool is None:
             pool = S.workload_batch()
         rates = [float(r) for r in args.served_rates.split(",") if r]
+        host_rates = [float(r) for r in args.served_host_rates.split(",") if r]
         out["served"] = {
-            "workload": "Poisson arrivals of tiles drawn from the 256 uneven tiles",
+            "workload": "Poisson arrivals of tiles drawn from the 256 uneven tiles; rate 0: "
+                        "saturated (back-to-back submits); latency submit -> results",
             "hbm_resident": bench_served(T, pool, rates, args.served_requests, True,
                                          tune_table=table),
-            "host_buffers": bench_served(T, pool, [r for r in rates if r > 0][:2] + [0.0],
-                                         args.served_requests, False, tune_table=table),
+            "host_buffers": bench_served(T, pool, host_rates, args.served_requests, False,
+                                         tune_table=table),
         }
     if not args.no_cpu:
         c_cells, threads, per, reps, ref = cpu_baseline(tile)
@@ -800,7 +886,11 @@ def main():
                            "stolen_pairs": ref["S"], "candidate_pairs": ref["K"],
                            "stolen_pairs_per_s": ref["S"] / (elapsed / args.steps)}
     ctx.close()
-    print(json.dumps(out))
+    full = dict(out)
+    if choices is not None:
+        full["autotune_choices"] = choices
+    write_full_record(full, "bench_full.json")
+    print(json.dumps(compact(out)))
 
 
 if __name__ == "__main__":

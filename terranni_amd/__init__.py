@@ -482,14 +482,22 @@ class Context:
         self._chk(lib().tnni_tune_clear(self.h))
 
     # ---- served form ----
-    def submit(self, p: TileParams, X, Y, Z, labels: bool = False):
+    def submit(self, p: TileParams, X, Y, Z, labels: bool = False, out: np.ndarray | None = None):
         """Queue a tile; returns a Pending whose arrays are filled once
-        wait() returns.  The input arrays are kept alive until then."""
+        wait() returns.  The input arrays are kept alive until then.  out
+        (optional): a float32 array of at least nrows*ncols elements that
+        receives the heights (row-major), e.g. one of a caller's ring of
+        reused buffers."""
         X = np.ascontiguousarray(X, dtype=np.int32)
         Y = np.ascontiguousarray(Y, dtype=np.int32)
         Z = np.ascontiguousarray(Z, dtype=np.int32)
         fr = frame(p)
-        h = np.empty((p.nrows, p.ncols), dtype=np.float32)
+        if out is not None:
+            if out.dtype != np.float32 or not out.flags.c_contiguous or out.size < p.nrows * p.ncols:
+                raise ValueError("out must be a contiguous float32 array of at least nrows*ncols")
+            h = out.reshape(-1)[:p.nrows * p.ncols].reshape(p.nrows, p.ncols)
+        else:
+            h = np.empty((p.nrows, p.ncols), dtype=np.float32)
         lab = np.empty((fr["H"], fr["W"]), dtype=np.int32) if labels else None
         d2 = np.empty((fr["H"], fr["W"]), dtype=np.uint32) if labels else None
         d = p.to_desc()
