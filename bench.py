@@ -347,7 +347,7 @@ def bench_ref100k(T, S, args):
     uniform points in 1000 m, 512^2 DEM at 2 m, scale 5: 2584^2 pixels),
     plain form on device buffers, K steps timed, k_tiles timed with HIP
     events on its stream (for profiling this shape alone)."""
-    t = S.workload_uniform_100k(tile_cells=args.tile_cells, bucket_px=args.bucket_px)
+    t = S.workload_uniform_100k(tile_cells=args.tile_cells, bucket_px=args.bucket_px, form=args.form)
     fr = T.frame(t.params)
     ctx = T.Context(0)
     dX, dY, dZ = ctx.upload(t.X), ctx.upload(t.Y), ctx.upload(t.Z)
@@ -698,6 +698,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-cells", type=int, default=0, help="cell-tile side (0 = auto)")
     ap.add_argument("--bucket-px", type=int, default=0, help="bucket side in pixels (0 = auto)")
+    ap.add_argument("--form", type=int, default=0,
+                    help="0: automatic, 1: fused, 2: split (label raster, then a gather pass)")
     ap.add_argument("--workload", choices=("single", "batch", "stream", "spacetime", "ref100k"),
                     default="single",
                     help="single: the 4096^2 tile (default line, plus the batched and served "
@@ -751,7 +753,7 @@ def main():
         return
     t0 = time.perf_counter()
     tile = S.workload_single_4m(seed=1, n=args.points, grid=args.grid,
-                                tile_cells=args.tile_cells, bucket_px=args.bucket_px)
+                                tile_cells=args.tile_cells, bucket_px=args.bucket_px, form=args.form)
     log(f"generated {tile.n} points, {tile.cells} cells in {time.perf_counter() - t0:.1f}s")
     fr = T.frame(tile.params)
 

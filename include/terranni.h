@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define TNNI_ABI_VERSION 3
+#define TNNI_ABI_VERSION 4
 
 typedef enum tnni_status {
     TNNI_OK = 0,
@@ -89,8 +89,22 @@ typedef struct tnni_tile_desc {
     int point_min;              /* Settings::pointMin   (TNNI_ACC_S0 only) */
     int multiplier;             /* Settings::multiplier (TNNI_ACC_S0 only) */
     int bucket_px;              /* bucket side in pixels, 0 = tuned        */
-    int tile_cells;             /* cell-tile side per workgroup, 0 = auto  */
+    int tile_cells;             /* cell-tile side per workgroup (in the
+                                   split form: the label tile side in
+                                   pixels, at least 8), 0 = auto           */
+    int form;                   /* TNNI_FORM_*: how labels reach the
+                                   accumulation, 0 = tuned / automatic     */
 } tnni_tile_desc;
+
+/* tnni_tile_desc::form.  Either gives the same results bit for bit.
+ *  FUSED: each work item labels its cells' pixels plus the query halo into
+ *    LDS and accumulates its cells from there in the same kernel (the halo
+ *    is labelled by every item that needs it);
+ *  SPLIT: each pixel is labelled once (items without a halo) into a raster
+ *    of site values and squared distances in device memory, then a second
+ *    kernel accumulates every cell from that raster.  Pays where the query
+ *    window is wide in pixels (large scale), so that the halo dominates. */
+enum { TNNI_FORM_AUTO = 0, TNNI_FORM_FUSED = 1, TNNI_FORM_SPLIT = 2 };
 
 /* Derived geometry of a tile (all lengths in fixed-point units). */
 typedef struct tnni_frame_info {
@@ -310,6 +324,8 @@ typedef struct tnni_tune_entry {
     int32_t candidates;  /* settings timed (tnni_autotune only)             */
     float ms_default;    /* sample time, built-in choice (tnni_autotune)    */
     float ms_tuned;      /* sample time, chosen setting (tnni_autotune)     */
+    int32_t form;        /* TNNI_FORM_FUSED or TNNI_FORM_SPLIT (0: built-in) */
+    int32_t pad;
 } tnni_tune_entry;
 
 /* Shape class of a tile (batched != 0: the batched and served forms). */

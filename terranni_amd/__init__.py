@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(LIB_DIR, "libterranni_hip.so")
 
 ALG_PLANES, ALG_CONES = 0, 1
 ACC_FIXED64, ACC_FP32_2SUM, ACC_S0 = 0, 1, 2
+FORM_AUTO, FORM_FUSED, FORM_SPLIT = 0, 1, 2
 NODATA = -9999.0
 NO_D2 = 0xFFFFFFFF
 
@@ -51,6 +52,7 @@ class TileDesc(ctypes.Structure):
         ("units_per_m", ctypes.c_int), ("pixel_offset", ctypes.c_int),
         ("point_min", ctypes.c_int), ("multiplier", ctypes.c_int),
         ("bucket_px", ctypes.c_int), ("tile_cells", ctypes.c_int),
+        ("form", ctypes.c_int),
     ]
 
 
@@ -90,7 +92,7 @@ class TuneEntry(ctypes.Structure):
         ("shape_class", ctypes.c_uint64), ("bucket_px", ctypes.c_int32),
         ("tile_cells", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
         ("candidates", ctypes.c_int32), ("ms_default", ctypes.c_float),
-        ("ms_tuned", ctypes.c_float),
+        ("ms_tuned", ctypes.c_float), ("form", ctypes.c_int32), ("pad", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -118,6 +120,7 @@ class TileParams:
     multiplier: int = 1
     bucket_px: int = 0
     tile_cells: int = 0
+    form: int = 0   # FORM_AUTO / FORM_FUSED / FORM_SPLIT
 
     def qr(self) -> float:
         if self.query_radius is not None:
@@ -143,6 +146,7 @@ class TileParams:
         d.multiplier = self.multiplier
         d.bucket_px = self.bucket_px
         d.tile_cells = self.tile_cells
+        d.form = self.form
         return d
 
 

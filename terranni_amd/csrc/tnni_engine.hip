@@ -90,11 +90,13 @@ struct HostBuf {
 struct GraphKey {
     const void *ptrs[10];
     size_t lds;
-    int alg, mode, main_grid;
+    int alg, mode, main_grid, gather_grid, label_only;
+    size_t gather_lds;
     bool operator==(const GraphKey &o) const
     {
         return std::memcmp(ptrs, o.ptrs, sizeof ptrs) == 0 && lds == o.lds && alg == o.alg &&
-               mode == o.mode && main_grid == o.main_grid;
+               mode == o.mode && main_grid == o.main_grid && gather_grid == o.gather_grid &&
+               gather_lds == o.gather_lds && label_only == o.label_only;
     }
 };
 
@@ -105,6 +107,7 @@ struct Arena {
     DevBuf sx, sy, sz, sh, sl, sd;  // device staging of host-pointer calls
     DevBuf st, acc;                 // space-time: point times, per-cell sums
     DevBuf cv, cd, flag;            // space-time: a slice combined from layers, sortedness flag
+    DevBuf lv, ld;                  // split form: site-value and squared-distance rasters
     HostBuf h_desc;                 // pinned descriptors
     HostBuf h_in, h_out;            // pinned staging of the served form
     hipGraphExec_t gexec = nullptr;
@@ -117,6 +120,7 @@ struct Arena {
 constexpr int kGraphSortGrid = 2048;
 constexpr int kGraphBandGrid = 2048;
 constexpr int kGraphRowGrid = 2048;
+constexpr int kGraphGatherGrid = 2048;
 
 struct Request {
     uint64_t id = 0;
@@ -360,7 +364,7 @@ void arena_free(tnni_ctx *ctx, Arena &a)
     if (a.stream) (void)hipStreamSynchronize(a.stream);
     DevBuf *bufs[] = {&a.tiles, &a.chunks, &a.batch, &a.hist, &a.partials, &a.banded, &a.sorted,
                       &a.offs, &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd,
-                      &a.st, &a.acc, &a.cv, &a.cd, &a.flag};
+                      &a.st, &a.acc, &a.cv, &a.cd, &a.flag, &a.lv, &a.ld};
     for (DevBuf *b : bufs) release(ctx, *b);
     HostBuf *hb[] = {&a.h_desc, &a.h_in, &a.h_out};
     for (HostBuf *b : hb) {
@@ -395,9 +399,12 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     const bool time = tiles[0].t.T != nullptr;
     if (time && how == Launch::Graph) return fail(ctx, TNNI_E_INVALID, "space-time tiles run direct");
     int64_t pts = 0, bks = 0, ebytes = 0, items = 0, rows = 0, bands = 0, hents = 0;
+    int64_t gather_items = 0, split_pix = 0, split_d2_pix = 0;  // the split tiles' gather blocks and rasters
+    size_t gather_lds = 0;
     size_t lds = 0;
     int wg_cap = 0;       // tuned workgroups per CU of the tile kernel
     bool wg_any = false;  // some tile has no tuned cap
+    bool any_label_only = false;
     std::vector<Chunk> chunks;
     for (size_t k = 0; k < tiles.size(); ++k) {
         TileDev &t = tiles[k].t;
@@ -413,6 +420,17 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
         t.band_base = bands;
         t.hist_base = hents;
         t.chunk_base = (int)chunks.size();
+        // a split tile with heights wanted: its cells are gathered from its
+        // value and squared-distance rasters (scratch unless the caller
+        // takes the squared distances); other tiles get an empty range
+        t.gitem_base = gather_items;
+        any_label_only = any_label_only || t.label_only;
+        if (t.label_only && t.out_h) {
+            gather_items += (int64_t)t.gntx * t.gnty;
+            gather_lds = std::max(gather_lds, gather_window_bytes(t, t.gb));
+            split_pix += (int64_t)t.W * t.H;
+            if (!t.out_d2) split_d2_pix += (int64_t)t.W * t.H;
+        }
         // every tile has at least one (possibly empty) chunk
         for (int64_t b = 0; b < std::max<int64_t>(t.n, 1); b += kChunkPts) {
             Chunk c;
@@ -446,6 +464,21 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     if ((st = ensure(ctx, a.offs, (size_t)(bks + 1) * 4)) != TNNI_OK) return st;
     if ((st = ensure(ctx, a.hdist, (size_t)ebytes)) != TNNI_OK) return st;
     if ((st = ensure(ctx, a.edist, (size_t)ebytes)) != TNNI_OK) return st;
+    if (gather_items) {
+        if ((st = ensure(ctx, a.lv, (size_t)split_pix * 4)) != TNNI_OK) return st;
+        if (split_d2_pix && (st = ensure(ctx, a.ld, (size_t)split_d2_pix * 4)) != TNNI_OK) return st;
+        int64_t pv = 0, pd = 0;
+        for (Prepared &p : tiles) {
+            TileDev &t = p.t;
+            if (!(t.label_only && t.out_h)) continue;
+            t.out_val = (int32_t *)a.lv.p + pv;
+            pv += (int64_t)t.W * t.H;
+            if (!t.out_d2) {
+                t.out_d2 = (uint32_t *)a.ld.p + pd;
+                pd += (int64_t)t.W * t.H;
+            }
+        }
+    }
 
     BatchDev bd;
     std::memset(&bd, 0, sizeof bd);
@@ -456,6 +489,7 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     bd.n_rows = rows;
     bd.n_bands = bands;
     bd.scan_chunk = (uint32_t)((bd.n_scan + kScanBlocks - 1) / kScanBlocks);
+    bd.n_gather_items = gather_items;
 
     // descriptors through one pinned staging block; it may still feed the
     // copies of the previous batch on this stream
@@ -479,6 +513,7 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     cfg.algorithm = alg;
     cfg.mode = mode;
     cfg.time = time ? 1 : 0;
+    cfg.label_only = any_label_only ? 1 : 0;
     DevBuffers db;
     db.batch = (BatchDev *)a.batch.p;
     db.tiles = (const TileDev *)a.tiles.p;
@@ -496,6 +531,8 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
         cfg.band_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, bands));
         cfg.row_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, rows));
         cfg.main_grid = (int)std::max<int64_t>(1, std::min<int64_t>(items, (int64_t)ctx->n_cu * occ));
+        cfg.gather_grid = (int)std::min<int64_t>(gather_items, 8192);
+        cfg.gather_lds = gather_lds;
         HIP_TRY(ctx, enqueue_pipeline(db, cfg, a.stream, prof ? ctx->ev0 : nullptr,
                                       prof ? ctx->ev1 : nullptr));
     } else {
@@ -503,6 +540,8 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
         cfg.band_grid = kGraphBandGrid;
         cfg.row_grid = kGraphRowGrid;
         cfg.main_grid = ctx->n_cu * occ;
+        cfg.gather_grid = gather_items ? kGraphGatherGrid : 0;
+        cfg.gather_lds = gather_lds;
         GraphKey key;
         const void *p[10] = {a.batch.p, a.tiles.p, a.chunks.p, a.hist.p, a.partials.p,
                              a.banded.p, a.sorted.p, a.offs.p, a.hdist.p, a.edist.p};
@@ -511,6 +550,9 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
         key.alg = alg;
         key.mode = mode;
         key.main_grid = cfg.main_grid;
+        key.gather_grid = cfg.gather_grid;
+        key.gather_lds = cfg.gather_lds;
+        key.label_only = cfg.label_only;
         if (!a.gexec || !(key == a.gkey)) {
             if (a.gexec) HIP_TRY(ctx, hipGraphExecDestroy(a.gexec));
             a.gexec = nullptr;
@@ -561,6 +603,7 @@ uint64_t request_bytes(const TileDev &t, bool labels, bool d2)
                  nbx4 * t.nby * 2 + cells * 4;
     if (labels) b += pix * 4;
     if (d2) b += pix * 4;
+    if (t.label_only) b += pix * 8;  // split form: value and squared-distance rasters
     const uint64_t chunks = (uint64_t)t.n / kChunkPts + 1;
     // the band histograms: one entry per (band, chunk)
     return b + sizeof(TileDev) + chunks * sizeof(Chunk) + ((uint64_t)t.nbands * chunks + 1) * 4;
@@ -578,7 +621,7 @@ void arena_trim(tnni_ctx *ctx, Arena &a)
 {
     DevBuf *bufs[] = {&a.tiles, &a.chunks, &a.batch, &a.hist, &a.partials, &a.banded, &a.sorted,
                       &a.offs, &a.hdist, &a.edist, &a.sx, &a.sy, &a.sz, &a.sh, &a.sl, &a.sd,
-                      &a.st, &a.acc, &a.cv, &a.cd, &a.flag};
+                      &a.st, &a.acc, &a.cv, &a.cd, &a.flag, &a.lv, &a.ld};
     for (DevBuf *b : bufs) release(ctx, *b);
     if (a.gexec) (void)hipGraphExecDestroy(a.gexec);
     a.gexec = nullptr;
@@ -1050,6 +1093,11 @@ tnni_status tnni_tile_run_dev_noalloc(tnni_ctx *ctx, const tnni_tile_desc *desc,
     const size_t o_part = carve(kScanBlocks * 4), o_banded = carve((size_t)t.n * 16);
     const size_t o_sorted = carve((size_t)t.n * 16), o_offs = carve((size_t)(bks + 1) * 4);
     const size_t o_hdist = carve((size_t)ebytes), o_edist = carve((size_t)ebytes);
+    // the split form's value raster (and squared-distance raster, unless the
+    // caller takes the squared distances)
+    const size_t pix = (size_t)t.W * t.H;
+    const size_t o_val = t.label_only ? carve(pix * 4) : 0;
+    const size_t o_sd2 = t.label_only && !d_out_d2 ? carve(pix * 4) : 0;
     if (off > kNoAllocScratchBytes)
         return fail(ctx, TNNI_E_BUDGET,
                     "tile needs " + std::to_string(off) + " bytes of scratch; the allocation-free entry point has " +
@@ -1066,6 +1114,12 @@ tnni_status tnni_tile_run_dev_noalloc(tnni_ctx *ctx, const tnni_tile_desc *desc,
     db.offs = (uint32_t *)(base + o_offs);
     db.hdist = (uint8_t *)(base + o_hdist);
     db.edist = (uint8_t *)(base + o_edist);
+    t.gitem_base = 0;
+    if (t.label_only) {
+        t.out_val = (int32_t *)(base + o_val);
+        if (!d_out_d2) t.out_d2 = (uint32_t *)(base + o_sd2);
+    }
+    const int64_t gather_items = t.label_only ? (int64_t)t.gntx * t.gnty : 0;
     BatchDev bd;
     std::memset(&bd, 0, sizeof bd);
     bd.n_tiles = 1;
@@ -1075,6 +1129,7 @@ tnni_status tnni_tile_run_dev_noalloc(tnni_ctx *ctx, const tnni_tile_desc *desc,
     bd.n_rows = t.nby;
     bd.n_bands = t.nbands;
     bd.scan_chunk = (uint32_t)((bd.n_scan + kScanBlocks - 1) / kScanBlocks);
+    bd.n_gather_items = gather_items;
     // the array keeps the previous call's contents: the setup kernel writes
     // every descriptor and k_band_count zeroes the work counter, and the
     // passes write every histogram entry, offset and distance they read
@@ -1087,10 +1142,13 @@ tnni_status tnni_tile_run_dev_noalloc(tnni_ctx *ctx, const tnni_tile_desc *desc,
     cfg.algorithm = t.algorithm;
     cfg.mode = t.mode;
     cfg.time = 0;
+    cfg.label_only = t.label_only;
     cfg.sort_grid = (int)std::min<int64_t>(4096, nch);
     cfg.band_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, t.nbands));
     cfg.row_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, t.nby));
     cfg.main_grid = (int)std::max<int64_t>(1, std::min<int64_t>(bd.n_items, (int64_t)ctx->n_cu * occ));
+    cfg.gather_grid = (int)std::min<int64_t>(gather_items, 8192);
+    cfg.gather_lds = t.label_only ? gather_window_bytes(t, t.gb) : 0;
     HIP_TRY(ctx, enqueue_pipeline(db, cfg, a.stream, nullptr, nullptr));
     HIP_TRY(ctx, hipStreamSynchronize(a.stream));
     return TNNI_OK;
@@ -1153,6 +1211,16 @@ namespace {
 // turned into site values, then each slice combines its layers
 // (k_combine_layers) and is gathered (k_gather_slice).  desc is the
 // cache's key.
+// A space-time tile's descriptor for its slices: the fused form (the
+// slices' kernels accumulate across slices in the tile kernel's gather).
+tnni_status prepare_time_tile(tnni_ctx *ctx, const tnni_tile_desc &desc, uint64_t n, Prepared &p,
+                              std::string &err)
+{
+    tnni_tile_desc d = desc;
+    d.form = TNNI_FORM_FUSED;
+    return prepare_tile(d, n, ctx->n_cu, false, &ctx->tuned, p.t, p.info, err);
+}
+
 tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_time,
                           float time_radius, tnni_time_cache *cache = nullptr,
                           const tnni_tile_desc *desc = nullptr)
@@ -1286,7 +1354,8 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
     size_t n_missing = 0;
     for (size_t k = 0; k < years.size(); ++k) n_missing += lay[k] ? 0 : 1;
     const double item_target = 16.0 * ctx->n_cu / (double)std::max<size_t>(n_missing, 1);
-    const int tc_small = std::max(8, (int)std::floor(std::sqrt((double)t.nrows * t.ncols / item_target)));
+    // (label-only layer tiles: pixels a side, whole 8x8 wave blocks)
+    const int tc_small = std::max(8, (int)std::floor(std::sqrt((double)t.W * t.H / item_target) / 8.0) * 8);
     for (size_t k = 0; k < years.size(); ++k) {
         if (lay[k]) continue;
         // a slot this call uses (held, or already taken for another missing
@@ -1303,12 +1372,16 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
         // the 2-D Voronoi of the points of this time: the plain tile of
         // their run (its heights go to scratch that the slices overwrite)
         const uint32_t lo = runs[1 + k], hi = std::max(runs[1 + k], runs[1 + kMaxLayers + k]);
+        // label-only items (no query halo, no accumulation): the layer takes
+        // each pixel's site value and squared distance straight from them
+        tnni_tile_desc lyr = *desc;
+        lyr.form = TNNI_FORM_SPLIT;
         Prepared q;
         std::string err;
-        if ((st = prepare_tile(*desc, hi - lo, ctx->n_cu, false, &ctx->tuned, q.t, q.info, err)) != TNNI_OK)
+        if ((st = prepare_tile(lyr, hi - lo, ctx->n_cu, false, &ctx->tuned, q.t, q.info, err)) != TNNI_OK)
             return fail(ctx, st, err);
         if (desc->tile_cells <= 0 && (double)q.t.ntx * q.t.nty < item_target && tc_small < q.t.tc) {
-            tnni_tile_desc ld = *desc;
+            tnni_tile_desc ld = lyr;
             ld.tile_cells = tc_small;
             Prepared qs;
             if (prepare_tile(ld, hi - lo, ctx->n_cu, false, &ctx->tuned, qs.t, qs.info, err) == TNNI_OK) q = qs;
@@ -1316,8 +1389,8 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
         q.t.X = t.X + lo;
         q.t.Y = t.Y + lo;
         q.t.Z = t.Z + lo;
-        q.t.out_h = (float *)a.cv.p;
-        q.t.out_labels = (int32_t *)victim->val.p;
+        q.t.out_h = nullptr;
+        q.t.out_val = (int32_t *)victim->val.p;
         q.t.out_d2 = (uint32_t *)victim->d2.p;
         v.push_back(q);
         vslot.push_back(victim);
@@ -1328,10 +1401,7 @@ tnni_status run_time_tile(tnni_ctx *ctx, Arena &a, const Prepared &p, float cur_
     }
     if (!v.empty()) {
         if ((st = run_prepared(ctx, a, v, false, Launch::Direct)) != TNNI_OK) return st;
-        for (size_t i = 0; i < v.size(); ++i) {
-            HIP_TRY(ctx, launch_label_values(v[i].t, (int32_t *)vslot[i]->val.p, pix, a.stream));
-            vslot[i]->full = true;
-        }
+        for (size_t i = 0; i < v.size(); ++i) vslot[i]->full = true;
     }
     // each slice: its layers combined, then gathered
     for (int j = j0; j <= j1; ++j) {
@@ -1435,7 +1505,7 @@ tnni_status tnni_tile_run_time_cached_dev(tnni_ctx *ctx, tnni_time_cache *cache,
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     Prepared p;
     std::string err;
-    tnni_status st = prepare_tile(*desc, n, ctx->n_cu, false, &ctx->tuned, p.t, p.info, err);
+    tnni_status st = prepare_time_tile(ctx, *desc, n, p, err);
     if (st != TNNI_OK) return fail(ctx, st, err);
     p.t.X = dX;
     p.t.Y = dY;
@@ -1467,7 +1537,7 @@ tnni_status tnni_tile_run_time_cached(tnni_ctx *ctx, tnni_time_cache *cache, con
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     Prepared p;
     std::string err;
-    tnni_status st = prepare_tile(*desc, n, ctx->n_cu, false, &ctx->tuned, p.t, p.info, err);
+    tnni_status st = prepare_time_tile(ctx, *desc, n, p, err);
     if (st != TNNI_OK) return fail(ctx, st, err);
     Arena &a = ctx->main;
     const size_t cells = (size_t)p.t.nrows * p.t.ncols;
@@ -1708,7 +1778,7 @@ namespace {
 tnni_status time_setting(tnni_ctx *ctx, const std::vector<int> &idx, const tnni_tile_desc *descs,
                          const int32_t *const *dX, const int32_t *const *dY,
                          const int32_t *const *dZ, const uint64_t *n, bool batched, int replicas,
-                         int bm, int tc, int wg, int reps, float &best_ms)
+                         int bm, int tc, int wg, int form, int reps, float &best_ms)
 {
     Arena &a = ctx->main;
     std::vector<Prepared> v;
@@ -1720,6 +1790,7 @@ tnni_status time_setting(tnni_ctx *ctx, const std::vector<int> &idx, const tnni_
             tnni_tile_desc d = descs[k];
             d.bucket_px = bm;
             d.tile_cells = tc;
+            d.form = form;
             Prepared p;
             tnni_status st = prepare_tile(d, n[k], ctx->n_cu, batched, nullptr, p.t, p.info, err);
             if (st != TNNI_OK) return fail(ctx, st, err);
@@ -1828,27 +1899,54 @@ tnni_status tnni_autotune(tnni_ctx *ctx, int n_tiles, const tnni_tile_desc *desc
             if (tcs.size() >= 5) break;
             if (std::find(tcs.begin(), tcs.end(), c) == tcs.end()) tcs.push_back(c);
         }
+        const int f0 = i0.form;
         float ms_default = 0.f;
-        st = time_setting(ctx, idx, descs, dX, dY, dZ, n, bat, replicas, bm0, tc0, 0, reps, ms_default);
+        st = time_setting(ctx, idx, descs, dX, dY, dZ, n, bat, replicas, bm0, tc0, 0, f0, reps, ms_default);
         if (st != TNNI_OK) return st;
         TuneChoice best;
         best.bucket_px = bm0;
         best.tile_cells = tc0;
+        best.form = f0;
         float best_ms = ms_default;
         int tried = 1;
-        for (int b : bms) {
+        auto try_setting = [&](int b, int c, int form) -> tnni_status {
+            float ms = 0.f;
+            tnni_status s2 = time_setting(ctx, idx, descs, dX, dY, dZ, n, bat, replicas, b, c, 0, form, reps, ms);
+            if (s2 == TNNI_E_GEOMETRY) return TNNI_OK;  // e.g. a bucket grid too large
+            if (s2 != TNNI_OK) return s2;
+            ++tried;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.bucket_px = b;
+                best.tile_cells = c;
+                best.form = form;
+            }
+            return TNNI_OK;
+        };
+        for (int b : bms)
             for (int c : tcs) {
                 if (b == bm0 && c == tc0) continue;
-                float ms = 0.f;
-                st = time_setting(ctx, idx, descs, dX, dY, dZ, n, bat, replicas, b, c, 0, reps, ms);
-                if (st == TNNI_E_GEOMETRY) continue;  // e.g. a bucket grid too large
-                if (st != TNNI_OK) return st;
-                ++tried;
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best.bucket_px = b;
-                    best.tile_cells = c;
+                if ((st = try_setting(b, c, f0)) != TNNI_OK) return st;
+            }
+        // the other form (fused or split) at the built-in and the best
+        // bucket side, with its own three most lane-efficient cell tiles
+        {
+            const int f1 = f0 == TNNI_FORM_SPLIT ? TNNI_FORM_FUSED : TNNI_FORM_SPLIT;
+            tnni_tile_desc d1 = d0;
+            d1.form = f1;
+            TileDev t1;
+            PrepInfo i1;
+            if (prepare_tile(d1, n[idx[0]], ctx->n_cu, bat, nullptr, t1, i1, err) == TNNI_OK) {
+                std::vector<int> tcs1{t1.tc};
+                for (int c : rank_tile_cells(t1, bat ? kLdsBudget4 : kLdsBudget, bat ? 0 : ctx->n_cu)) {
+                    if (tcs1.size() >= 3) break;
+                    if (std::find(tcs1.begin(), tcs1.end(), c) == tcs1.end()) tcs1.push_back(c);
                 }
+                std::vector<int> bms1{bm0};
+                if (best.bucket_px != bm0) bms1.push_back(best.bucket_px);
+                for (int b : bms1)
+                    for (int c : tcs1)
+                        if ((st = try_setting(b, c, f1)) != TNNI_OK) return st;
             }
         }
         // fewer workgroups per CU at the best sizes (kept only if clearly faster)
@@ -1856,6 +1954,7 @@ tnni_status tnni_autotune(tnni_ctx *ctx, int n_tiles, const tnni_tile_desc *desc
             tnni_tile_desc db = d0;
             db.bucket_px = best.bucket_px;
             db.tile_cells = best.tile_cells;
+            db.form = best.form;
             TileDev tb;
             PrepInfo ib;
             if ((st = prepare_tile(db, n[idx[0]], ctx->n_cu, bat, nullptr, tb, ib, err)) != TNNI_OK)
@@ -1864,7 +1963,7 @@ tnni_status tnni_autotune(tnni_ctx *ctx, int n_tiles, const tnni_tile_desc *desc
             for (int wg = occ - 1; wg >= std::max(1, occ - 2); --wg) {
                 float ms = 0.f;
                 st = time_setting(ctx, idx, descs, dX, dY, dZ, n, bat, replicas, best.bucket_px,
-                                  best.tile_cells, wg, reps, ms);
+                                  best.tile_cells, wg, best.form, reps, ms);
                 if (st != TNNI_OK) return st;
                 ++tried;
                 if (ms < 0.98f * best_ms) {
@@ -1880,6 +1979,8 @@ tnni_status tnni_autotune(tnni_ctx *ctx, int n_tiles, const tnni_tile_desc *desc
             e.bucket_px = best.bucket_px;
             e.tile_cells = best.tile_cells;
             e.wg_per_cu = best.wg_per_cu;
+            e.form = best.form;
+            e.pad = 0;
             e.candidates = tried;
             e.ms_default = ms_default;
             e.ms_tuned = best_ms;
@@ -1901,6 +2002,7 @@ tnni_status tnni_tune_read(tnni_ctx *ctx, tnni_tune_entry *out, int cap, int *n)
         out[i].bucket_px = e[(size_t)i].second.bucket_px;
         out[i].tile_cells = e[(size_t)i].second.tile_cells;
         out[i].wg_per_cu = e[(size_t)i].second.wg_per_cu;
+        out[i].form = e[(size_t)i].second.form;
     }
     return TNNI_OK;
 }
@@ -1909,13 +2011,15 @@ tnni_status tnni_tune_load(tnni_ctx *ctx, const tnni_tune_entry *in, int n)
 {
     if (!ctx || n < 0 || (n && !in)) return fail(ctx, TNNI_E_INVALID, "null argument");
     for (int i = 0; i < n; ++i)
-        if (in[i].bucket_px < 0 || in[i].tile_cells < 0 || in[i].tile_cells > 64 || in[i].wg_per_cu < 0)
+        if (in[i].bucket_px < 0 || in[i].tile_cells < 0 || in[i].tile_cells > 64 || in[i].wg_per_cu < 0 ||
+            in[i].form < 0 || in[i].form > TNNI_FORM_SPLIT)
             return fail(ctx, TNNI_E_INVALID, "tune entry " + std::to_string(i) + " out of range");
     for (int i = 0; i < n; ++i) {
         TuneChoice c;
         c.bucket_px = in[i].bucket_px;
         c.tile_cells = in[i].tile_cells;
         c.wg_per_cu = in[i].wg_per_cu;
+        c.form = in[i].form;
         ctx->tuned.put(in[i].shape_class, c);
     }
     return TNNI_OK;

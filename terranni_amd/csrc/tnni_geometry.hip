@@ -127,6 +127,12 @@ uint64_t shape_class(const tnni_tile_desc &d, uint64_t n, const tnni_frame_info 
 size_t tile_lds_bytes(TileDev &t, int tc)
 {
     t.tc = tc;
+    if (t.label_only) {  // tc x tc pixel tiles
+        t.ntx = (t.W + tc - 1) / tc;
+        t.nty = (t.H + tc - 1) / tc;
+        const size_t rw = (size_t)std::min(tc, t.W), rh = (size_t)std::min(tc, t.H);
+        return (rw * rh * 10 + 3) / 4 * 4 + tile_kernel_stage_bytes();
+    }
     t.ntx = (t.ncols + tc - 1) / tc;
     t.nty = (t.nrows + tc - 1) / tc;
     // an item's pixel width depends on its column only, its height on its
@@ -155,9 +161,18 @@ size_t tile_lds_bytes(TileDev &t, int tc)
     return ((size_t)rw * rh * 10 + 3) / 4 * 4 + tile_kernel_stage_bytes();
 }
 
+int query_span(const TileDev &t) { return std::max(1, query_hi(t) - query_lo(t) + 1); }
+
+size_t gather_window_bytes(const TileDev &t, int gb)
+{
+    const size_t side = (size_t)(gb - 1) * t.scale + query_span(t);
+    return side * side * 8;
+}
+
 // Useful share of the label lanes of an interior item of tc cells: it labels
 // (tc-1)*scale + span pixels a side in 8x8 wave blocks, four waves a
-// workgroup, for the tc*scale pixels a side its cells own.
+// workgroup, for the tc*scale pixels a side its cells own (a label-only
+// item, which has no query halo, passes span = scale).
 double tile_efficiency(int tc, int scale, int span)
 {
     const int P = (tc - 1) * scale + span;
@@ -171,8 +186,21 @@ double tile_efficiency(int tc, int scale, int span)
 // sizes above 16 that leave fewer than 2 * min_items_cu items go last.
 std::vector<int> rank_tile_cells(TileDev &t, size_t budget, int min_items_cu)
 {
-    const int qa = query_lo(t), qb = query_hi(t);
-    const int span = std::max(1, qb - qa + 1);
+    if (t.label_only) {
+        // pixel tiles of 8..64 pixels a side in whole 8x8 wave blocks; the
+        // largest that fits, the plain form first those leaving at least
+        // twice the CUs in items
+        std::vector<int> big, few;
+        for (int tc = 64; tc >= 8; tc -= 8) {
+            if (tile_lds_bytes(t, tc) > budget) continue;
+            const bool f = min_items_cu > 0 && (int64_t)t.ntx * t.nty < 2ll * min_items_cu;
+            (f ? few : big).push_back(tc);
+        }
+        big.insert(big.end(), few.begin(), few.end());
+        if (!big.empty()) tile_lds_bytes(t, big[0]);
+        return big;
+    }
+    const int span = query_span(t);
     struct C {
         int tc;
         double e;
@@ -259,7 +287,7 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
     info = PrepInfo();
     info.shape_class = shape_class(d, n, f, batched);
     TuneChoice tc_tuned;
-    if (tuned && (d.bucket_px <= 0 || d.tile_cells <= 0) && tuned->lookup(info.shape_class, tc_tuned))
+    if (tuned && (d.bucket_px <= 0 || d.tile_cells <= 0 || d.form <= 0) && tuned->lookup(info.shape_class, tc_tuned))
         info.wg_per_cu = tc_tuned.wg_per_cu;
     int bm = d.bucket_px > 0 ? d.bucket_px : tc_tuned.bucket_px;
     if (bm <= 0) {
@@ -316,10 +344,40 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
     // by ring and stop early; otherwise ring 1 is nearly always needed and
     // the searches visit whole squares of buckets (fewer, longer ranges).
     t.dense = (double)n * (double)bm * (double)bm > kDenseSites * (double)f.W * (double)f.H ? 1 : 0;
+    // the form: fused (labels with the query halo, cells accumulated in the
+    // same kernel) or split (each pixel labelled once into a raster, then a
+    // gather pass).  Built in: split where label-only items use their lanes
+    // at least kSplitGain times better than fused ones, i.e. where the
+    // query halo (wide query windows at large scale) dominates the labelling
+    int form = d.form > 0 ? d.form : tc_tuned.form;
+    if (form != TNNI_FORM_FUSED && form != TNNI_FORM_SPLIT) {
+        // (label-only pixel tiles use every lane of their 8x8 blocks)
+        TileDev a = t;
+        a.label_only = 0;
+        std::vector<int> order = rank_tile_cells(a, kLdsBudget4, batched ? 0 : n_cu);
+        const double eff = order.empty() ? 0.0 : tile_efficiency(order[0], t.scale, query_span(t));
+        form = 1.0 >= kSplitGain * eff ? TNNI_FORM_SPLIT : TNNI_FORM_FUSED;
+    }
+    t.label_only = form == TNNI_FORM_SPLIT ? 1 : 0;
+    t.gitem_base = 0;
+    if (t.label_only) {
+        // gather blocks: the largest of 16 .. 1 cells a side whose pixel
+        // window fits kGatherLds
+        t.gb = 16;
+        while (t.gb > 1 && gather_window_bytes(t, t.gb) > kGatherLds) t.gb /= 2;
+        if (gather_window_bytes(t, t.gb) > kGatherLds) {
+            err = "query window too wide for the split form's gather";
+            return TNNI_E_GEOMETRY;
+        }
+        t.gntx = (t.ncols + t.gb - 1) / t.gb;
+        t.gnty = (t.nrows + t.gb - 1) / t.gb;
+    }
+    info.form = form;
     // cell tiles: the most lane-efficient size that fits the LDS budget
     // of four workgroups per CU; the plain form avoids sizes that leave a
     // lone small tile with fewer items than twice the CUs
-    const int tc_fixed = d.tile_cells > 0 ? d.tile_cells : tc_tuned.tile_cells;
+    int tc_fixed = d.tile_cells > 0 ? d.tile_cells : tc_tuned.tile_cells;
+    if (t.label_only && tc_fixed > 0) tc_fixed = std::max(8, tc_fixed);  // pixels: at least one wave block
     size_t lds = 0;
     if (tc_fixed > 0) {
         lds = tile_lds_bytes(t, tc_fixed);

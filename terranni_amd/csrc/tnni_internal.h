@@ -26,6 +26,7 @@ constexpr int kMaxBands = 8192;        // bucket-row bands per tile (LDS histogr
 constexpr int kBandBuckets = 8192;     // buckets per band (LDS counters)
 constexpr int kBandPts = 4096;         // points per band aimed at (mean density)
 constexpr double kDenseSites = 2.0;    // sites per bucket above which searches go ring by ring
+constexpr double kSplitGain = 1.6;     // label-lane efficiency gain above which tiles take the split form
 
 // Everything a kernel needs to know about one tile of a batch.  Lengths are
 // in the tile-local fixed-point unit (1/upm metre).
@@ -61,8 +62,18 @@ struct TileDev {
     int64_t hist_base;          // first band-histogram entry of this tile
     int chunk_base, nchunks;    // this tile's chunks of points
     int tc, ntx, nty;           // cell-tile side and tiles along x / y
-    int pad0;
+    // label_only: items are tc x tc pixel tiles (tc in pixels, no query
+    // halo) that label their pixels and write the site values (out_val) and
+    // squared distances (out_d2) to device memory instead of accumulating
+    // cells; the split form then accumulates the cells from those rasters
+    // (k_gather_split) in blocks of gb x gb cells (gntx x gnty blocks), its
+    // items [gitem_base, gitem_base + gntx * gnty) of the batch's gather items
+    int label_only;
     int64_t item_base;          // first work item of this tile
+    int64_t gitem_base;         // first gather item (tiles without: an empty range)
+    int32_t *out_val;
+    int gb, gntx, gnty;         // gather blocks: side in cells, blocks along x / y
+    int pad4;
     // space-time slice (T != nullptr): the sites are the points with
     // |slice_t - T[i]| <= time_radius + 1e-6, at squared time distance dt2
     // (units^2) added to their squared distance; queries add jj2; acc holds
@@ -113,6 +124,7 @@ struct BatchDev {
     int64_t n_bands;       // bands of the whole batch
     uint32_t scan_chunk;   // elements per scan block
     uint32_t pad;
+    int64_t n_gather_items;  // cell blocks the split form's gather pass accumulates
 };
 
 __host__ __device__ inline int floordiv_i(int a, int b)  // b > 0
@@ -137,6 +149,14 @@ __host__ __device__ inline int query_hi(const TileDev &t) { return floordiv_i(t.
 __host__ __device__ inline Region region_of(const TileDev &t, int ti, int tj)
 {
     Region g;
+    if (t.label_only) {  // a tc x tc pixel tile: no cells, no query halo
+        g.i0 = g.i1 = g.j0 = g.j1 = 0;
+        g.rk0 = g.ok0 = ti * t.tc;
+        g.rk1 = g.ok1 = g.rk0 + t.tc - 1 < t.W - 1 ? g.rk0 + t.tc - 1 : t.W - 1;
+        g.rl0 = g.ol0 = tj * t.tc;
+        g.rl1 = g.ol1 = g.rl0 + t.tc - 1 < t.H - 1 ? g.rl0 + t.tc - 1 : t.H - 1;
+        return g;
+    }
     g.i0 = ti * t.tc;
     g.i1 = g.i0 + t.tc < t.ncols ? g.i0 + t.tc : t.ncols;
     g.j0 = tj * t.tc;
@@ -171,6 +191,9 @@ struct LaunchCfg {
     int algorithm;      // TNNI_ALG_* of every tile in the batch
     int mode;           // TNNI_ACC_* of every tile in the batch
     int time;           // space-time slices (every tile has T)
+    int label_only;     // some tile of the batch is a label-only tile
+    int gather_grid;    // blocks of the split form's gather pass (0: none)
+    size_t gather_lds;  // its dynamic LDS: the largest pixel window of a cell block
 };
 
 struct DevBuffers {
@@ -198,8 +221,8 @@ hipError_t enqueue_pipeline(const DevBuffers &b, const LaunchCfg &cfg,
 hipError_t launch_single_tile_setup(const TileDev &t, const BatchDev &bd, TileDev *tiles, Chunk *chunks,
                                     BatchDev *batch, hipStream_t s);
 
-// Space-time year layers: a label raster turned into site values in place;
-// a slice's values and squared distances combined from the layers of its
+// Space-time year layers: a slice's values and squared distances combined
+// from the layers of its
 // times (ascending, dt2 each); the query gather of a slice (values and
 // squared distances of its W x H pixels) into t.acc / t.out_h; whether the
 // points' times are out of order, and the run of each layer's time.
@@ -210,7 +233,6 @@ struct LayerSet {
     const int32_t *val[kMaxLayers];
     const uint32_t *d2[kMaxLayers];
 };
-hipError_t launch_label_values(const TileDev &t, int32_t *lab, uint64_t pix, hipStream_t s);
 hipError_t launch_combine_layers(const LayerSet &ls, uint64_t pix, int32_t *val, uint32_t *d2, hipStream_t s);
 hipError_t launch_gather_slice(const TileDev &t, const int32_t *val, const uint32_t *d2, hipStream_t s);
 struct LayerTimes {
@@ -232,6 +254,11 @@ size_t tile_kernel_stage_bytes();
 // 64 KB per-group limit (static LDS takes a few more bytes)
 constexpr size_t kLdsBudget = 60 * 1024;
 constexpr size_t kLdsBudget4 = 40 * 1024;  // four workgroups per CU (160 KB LDS)
+constexpr size_t kGatherLds = 80 * 1024;   // a gather block's pixel window (two workgroups per CU)
+
+// The pixel window (value and squared distance, 8 bytes a pixel) of an
+// interior block of gb x gb cells of the split form's gather.
+size_t gather_window_bytes(const TileDev &t, int gb);
 
 tnni_status compute_frame(const tnni_tile_desc &d, tnni_frame_info &f, std::string &err);
 
@@ -239,6 +266,7 @@ tnni_status compute_frame(const tnni_tile_desc &d, tnni_frame_info &f, std::stri
 // class (0 = the built-in choice).
 struct TuneChoice {
     int bucket_px = 0, tile_cells = 0, wg_per_cu = 0;
+    int form = 0;  // TNNI_FORM_*, 0: the built-in rule
 };
 
 // Measured choices per shape class; shared by the forms of one context.
@@ -278,11 +306,14 @@ struct PrepInfo {
     size_t lds = 0;            // dynamic LDS of the tile kernel for this tile
     int wg_per_cu = 0;         // tuned workgroups per CU (0 = all that fit)
     uint64_t shape_class = 0;  // the tile's shape class
+    int form = 0;              // TNNI_FORM_FUSED or TNNI_FORM_SPLIT
 };
 
 uint64_t shape_class(const tnni_tile_desc &d, uint64_t n, const tnni_frame_info &f, bool batched);
 size_t tile_lds_bytes(TileDev &t, int tc);
 double tile_efficiency(int tc, int scale, int span);
+// The query window of a cell in pixels (a side).
+int query_span(const TileDev &t);
 std::vector<int> rank_tile_cells(TileDev &t, size_t budget, int min_items_cu);
 
 // Fill everything of a TileDev that depends on the descriptor and the point

@@ -862,6 +862,30 @@ struct Acc {
         }
     }
 
+    // add(v) where take, else nothing, without a branch: adding an integer
+    // 0 changes no sum bit in any mode (FP32 two-sum: neither hi nor lo is
+    // ever -0.0, since they start at +0.0 and an exact zero sum of
+    // non-negative-zero terms rounds to +0.0), and only taken pixels count.
+    __device__ __forceinline__ void add_if(bool take, int32_t v)
+    {
+        if (MODE == TNNI_ACC_FIXED64) {
+            s64 += take ? v : 0;
+            cnt += take ? 1u : 0u;
+        } else if (MODE == TNNI_ACC_FP32_2SUM) {
+            const float x = take ? (float)v : 0.0f;
+            const float s = hi + x;
+            const float bb = s - hi;
+            const float err = (hi - (s - bb)) + (x - bb);
+            hi = s;
+            lo += err;
+            cnt += take ? 1u : 0u;
+        } else {
+            const bool t = take && v != 0;  // cuda_nni.cu:423
+            numer += t ? (uint32_t)v : 0u;
+            cnt += t ? 1u : 0u;
+        }
+    }
+
     // The running sums as 16 bytes, carried between the slices of a
     // space-time tile.
     __device__ __forceinline__ void load(const uint4 &v)
@@ -913,7 +937,10 @@ struct Acc {
     }
 };
 
-template <int ALG, int MODE, bool TIME>
+// LABEL_ONLY: the batch holds label-only tiles of the split form (checked
+// per tile); false for batches of fused tiles only, which then carry none
+// of that code.
+template <int ALG, int MODE, bool TIME, bool LABEL_ONLY>
 __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, uint32_t *smem,
                                               uint32_t *s_next, uint32_t *s_npend, uint32_t share,
                                               const uint32_t *__restrict__ offs,
@@ -966,6 +993,7 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
                 const int64_t o = (int64_t)l * t.W + k;
                 if (t.out_labels) t.out_labels[o] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
                 if (t.out_d2) t.out_d2[o] = d2;
+                if (LABEL_ONLY && t.out_val) t.out_val[o] = bval;
             }
         }
         uint32_t nxt = 0;
@@ -1011,8 +1039,12 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
             const int64_t o = (int64_t)l * t.W + k;
             if (t.out_labels) t.out_labels[o] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
             if (t.out_d2) t.out_d2[o] = d2;
+            if (LABEL_ONLY && t.out_val) t.out_val[o] = bval;
         }
     }
+    // a label-only item has written its pixels' values and squared
+    // distances: the split form's gather pass accumulates the cells
+    if (LABEL_ONLY && t.label_only) return;
     __syncthreads();
 
     // 2. query gather: cell q steals pixel x iff x is in q's region and
@@ -1047,7 +1079,7 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
                 const uint32_t d2 = s_d2[row + m];
                 bool steal = d2 != kNoD2 && dq + (TIME ? t.jj2 : 0u) <= d2;
                 if (ALG == TNNI_ALG_CONES) steal = steal && dq <= t.Rq2;
-                if (steal) acc.add(s_val[row + m]);
+                acc.add_if(steal, s_val[row + m]);
             }
         }
         if (!TIME || t.acc_final) t.out_h[cell] = acc.result(t);
@@ -1055,7 +1087,7 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
     }
 }
 
-template <int ALG, int MODE, bool TIME>
+template <int ALG, int MODE, bool TIME, bool LABEL_ONLY>
 __global__ void __launch_bounds__(kBlock)
 k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
         const uint8_t *__restrict__ edist, const int4 *__restrict__ sorted)
@@ -1092,32 +1124,107 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
         const TileDev t = tiles[lo];
         const int64_t local = item - t.item_base;
         const int tj = (int)(local / t.ntx), ti = (int)(local % t.ntx);
-        process_block<ALG, MODE, TIME>(t, ti, tj, smem, &s_next, &s_npend, share, offs, edist, sorted);
+        process_block<ALG, MODE, TIME, LABEL_ONLY>(t, ti, tj, smem, &s_next, &s_npend, share, offs, edist, sorted);
         __syncthreads();
     }
 }
 
-template <int ALG, int MODE, bool TIME>
+// The split form's gather pass: the batch's split tiles in blocks of
+// gb x gb cells, one workgroup a block.  The block's pixel window (every
+// pixel some cell of the block may steal) is copied from the value and
+// squared-distance rasters of the label-only items into LDS, then each
+// thread accumulates one cell from it, in the pixel order of
+// process_block's gather, so FP32 two-sum sums are bit-equal to the fused
+// form's.  The blocks of tile k are [gitem_base, gitem_base + gntx * gnty)
+// of the n_gather_items; other tiles' ranges are empty.
+template <int ALG, int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_gather_split(const BatchDev *b, const TileDev *tiles)
+{
+    extern __shared__ __align__(16) uint2 win[];  // {d2, value} per window pixel
+    const int64_t n_items = b->n_gather_items;
+    const int n_tiles = b->n_tiles;
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        int lo = 0, hi = n_tiles - 1;  // the last tile whose range starts at or before item
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tiles[mid].gitem_base <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const TileDev &t = tiles[lo];
+        const int gb = t.gb, scale = t.scale, W = t.W, H = t.H;
+        const int64_t local = item - t.gitem_base;
+        const int bj = (int)(local / t.gntx), bi = (int)(local - (int64_t)bj * t.gntx);
+        const int i0 = bi * gb, j0 = bj * gb;
+        const int i1 = min(i0 + gb, t.ncols), j1 = min(j0 + gb, t.nrows);
+        const int qa = query_lo(t), qb = query_hi(t);
+        const int kx0 = max(0, i0 * scale + qa), kx1 = min(W - 1, (i1 - 1) * scale + qb);
+        const int ky0 = max(0, j0 * scale + qa), ky1 = min(H - 1, (j1 - 1) * scale + qb);
+        const int ww = kx1 - kx0 + 1, wh = ky1 - ky0 + 1;
+        typedef const __attribute__((address_space(1))) int32_t gi32;
+        typedef const __attribute__((address_space(1))) uint32_t gu32;
+        gi32 *val = (gi32 *)t.out_val;
+        gu32 *d2r = (gu32 *)t.out_d2;
+        // the window, four pixels a thread in flight at once
+        const int npx = ww > 0 && wh > 0 ? ww * wh : 0;
+        for (int p0 = threadIdx.x; p0 < npx; p0 += 4 * kBlock) {
+            uint32_t dv[4], vv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int p = p0 + q * kBlock;
+                dv[q] = kNoD2;
+                vv[q] = 0u;
+                if (p < npx) {
+                    const int r = p / ww, c = p - r * ww;
+                    const int64_t o = (int64_t)(ky0 + r) * W + kx0 + c;
+                    dv[q] = d2r[o];
+                    vv[q] = (uint32_t)val[o];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (p0 + q * kBlock < npx) win[p0 + q * kBlock] = make_uint2(dv[q], vv[q]);
+        }
+        __syncthreads();
+        const int ci = (int)threadIdx.x % gb, cj = (int)threadIdx.x / gb;
+        const int i = i0 + ci, j = j0 + cj;
+        if (cj < gb && i < i1 && j < j1) {
+            const int ma = max(qa, -i * scale), mb = min(qb, W - 1 - i * scale);
+            const int na = max(qa, -j * scale), nb = min(qb, H - 1 - j * scale);
+            const int x0 = t.x0, y0 = t.y0, ps = t.ps_u;
+            const uint32_t Rq2 = t.Rq2;
+            Acc<MODE> acc;
+            for (int n = na; n <= nb; ++n) {
+                const uint32_t uy = (uint32_t)(y0 + n * ps);
+                const uint2 *wr = win + (j * scale + n - ky0) * ww + (i * scale - kx0);
+#pragma unroll 2
+                for (int m = ma; m <= mb; ++m) {
+                    const uint32_t ux = (uint32_t)(x0 + m * ps);
+                    const uint32_t dq = ux * ux + uy * uy;
+                    const uint2 e = wr[m];
+                    bool steal = e.x != kNoD2 && dq <= e.x;
+                    if (ALG == TNNI_ALG_CONES) steal = steal && dq <= Rq2;
+                    acc.add_if(steal, (int32_t)e.y);
+                }
+            }
+            t.out_h[(int64_t)j * t.ncols + i] = acc.result(t);
+        }
+        __syncthreads();
+    }
+}
+
+template <int ALG, int MODE, bool TIME, bool LABEL_ONLY>
 hipError_t launch_tiles(const DevBuffers &d, const LaunchCfg &cfg, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_tiles<ALG, MODE, TIME>), dim3(cfg.main_grid), dim3(kBlock), cfg.main_lds, s,
+    hipLaunchKernelGGL((k_tiles<ALG, MODE, TIME, LABEL_ONLY>), dim3(cfg.main_grid), dim3(kBlock), cfg.main_lds, s,
                        d.batch, d.tiles, d.offs, d.edist, d.sorted);
+    if (!TIME && cfg.gather_grid > 0)
+        hipLaunchKernelGGL((k_gather_split<ALG, MODE>), dim3(cfg.gather_grid), dim3(kBlock), cfg.gather_lds, s,
+                           d.batch, d.tiles);
     return hipGetLastError();
 }
 
 // ---- space-time cache of per-time layers ---------------------------------
-
-// A layer's label raster (written by k_tiles) turned into the labels' site
-// values in place (0 where no site covers the pixel), so that the layer
-// serves later calls whose point arrays index differently.
-__global__ void __launch_bounds__(kBlock)
-k_label_values(const TileDev t, int32_t *__restrict__ lab, uint64_t pix)
-{
-    for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < pix; p += (uint64_t)gridDim.x * kBlock) {
-        const int32_t l = lab[p];
-        lab[p] = l < 0 ? 0 : site_value(t, t.Z[l]);
-    }
-}
 
 // The query gather of one cached slice (values and squared distances of
 // its pixels in HBM): the cell loop of process_block's gather, in the same
@@ -1144,7 +1251,7 @@ k_gather_slice(const TileDev t, const int32_t *__restrict__ val, const uint32_t 
                 const uint32_t d2 = d2r[row + m];
                 bool steal = d2 != kNoD2 && dq + t.jj2 <= d2;
                 if (ALG == TNNI_ALG_CONES) steal = steal && dq <= t.Rq2;
-                if (steal) acc.add(val[row + m]);
+                acc.add_if(steal, val[row + m]);
             }
         }
         if (t.acc_final) t.out_h[cell] = acc.result(t);
@@ -1240,13 +1347,6 @@ hipError_t launch_single_tile_setup(const TileDev &t, const BatchDev &bd, TileDe
     return hipGetLastError();
 }
 
-hipError_t launch_label_values(const TileDev &t, int32_t *lab, uint64_t pix, hipStream_t s)
-{
-    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((pix + kBlock - 1) / kBlock, 8192));
-    hipLaunchKernelGGL(k_label_values, dim3(grid), dim3(kBlock), 0, s, t, lab, pix);
-    return hipGetLastError();
-}
-
 hipError_t launch_combine_layers(const LayerSet &ls, uint64_t pix, int32_t *val, uint32_t *d2, hipStream_t s)
 {
     if (ls.n < 0 || ls.n > kMaxLayers) return hipErrorInvalidValue;
@@ -1296,20 +1396,28 @@ hipError_t enqueue_pipeline(const DevBuffers &d, const LaunchCfg &cfg, hipStream
                        d.hdist, d.edist);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
-    const int sel = (cfg.time ? 6 : 0) + cfg.algorithm * 3 + cfg.mode;
+    // space-time slices never take the split form
+    if (cfg.time && cfg.label_only) return hipErrorInvalidValue;
+    const int sel = (cfg.time ? 6 : cfg.label_only ? 12 : 0) + cfg.algorithm * 3 + cfg.mode;
     switch (sel) {
-    case 0: e = launch_tiles<0, 0, false>(d, cfg, s); break;
-    case 1: e = launch_tiles<0, 1, false>(d, cfg, s); break;
-    case 2: e = launch_tiles<0, 2, false>(d, cfg, s); break;
-    case 3: e = launch_tiles<1, 0, false>(d, cfg, s); break;
-    case 4: e = launch_tiles<1, 1, false>(d, cfg, s); break;
-    case 5: e = launch_tiles<1, 2, false>(d, cfg, s); break;
-    case 6: e = launch_tiles<0, 0, true>(d, cfg, s); break;
-    case 7: e = launch_tiles<0, 1, true>(d, cfg, s); break;
-    case 8: e = launch_tiles<0, 2, true>(d, cfg, s); break;
-    case 9: e = launch_tiles<1, 0, true>(d, cfg, s); break;
-    case 10: e = launch_tiles<1, 1, true>(d, cfg, s); break;
-    case 11: e = launch_tiles<1, 2, true>(d, cfg, s); break;
+    case 0: e = launch_tiles<0, 0, false, false>(d, cfg, s); break;
+    case 1: e = launch_tiles<0, 1, false, false>(d, cfg, s); break;
+    case 2: e = launch_tiles<0, 2, false, false>(d, cfg, s); break;
+    case 3: e = launch_tiles<1, 0, false, false>(d, cfg, s); break;
+    case 4: e = launch_tiles<1, 1, false, false>(d, cfg, s); break;
+    case 5: e = launch_tiles<1, 2, false, false>(d, cfg, s); break;
+    case 6: e = launch_tiles<0, 0, true, false>(d, cfg, s); break;
+    case 7: e = launch_tiles<0, 1, true, false>(d, cfg, s); break;
+    case 8: e = launch_tiles<0, 2, true, false>(d, cfg, s); break;
+    case 9: e = launch_tiles<1, 0, true, false>(d, cfg, s); break;
+    case 10: e = launch_tiles<1, 1, true, false>(d, cfg, s); break;
+    case 11: e = launch_tiles<1, 2, true, false>(d, cfg, s); break;
+    case 12: e = launch_tiles<0, 0, false, true>(d, cfg, s); break;
+    case 13: e = launch_tiles<0, 1, false, true>(d, cfg, s); break;
+    case 14: e = launch_tiles<0, 2, false, true>(d, cfg, s); break;
+    case 15: e = launch_tiles<1, 0, false, true>(d, cfg, s); break;
+    case 16: e = launch_tiles<1, 1, false, true>(d, cfg, s); break;
+    case 17: e = launch_tiles<1, 2, false, true>(d, cfg, s); break;
     default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
@@ -1322,7 +1430,7 @@ size_t tile_kernel_stage_bytes() { return (size_t)kWaves * sizeof(WaveStage); }
 int tile_kernel_occupancy(size_t lds_bytes)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tiles<0, 0, false>, kBlock, lds_bytes) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tiles<0, 0, false, false>, kBlock, lds_bytes) !=
         hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;

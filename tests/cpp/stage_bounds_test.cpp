@@ -7,11 +7,15 @@
 // must stay inside its buffer, and the label tile must fit the dynamic LDS
 // that the launch gives the workgroup.  Also prints the stage's size and
 // the LDS of one workgroup for 256- and 384-site stages against the 160 KB
-// of a CU at 4 and 5 workgroups per CU.  Exit status 0: no violation.
+// of a CU at 4 and 5 workgroups per CU.  Every geometry runs in both forms:
+// fused (items label their cells' pixels plus the query halo and gather
+// from LDS) and split (label-only items, which must own every pixel of the
+// raster exactly once).  Exit status 0: no violation.
 //
 //   stage_bounds_test [n_random_geometries]
 #include <algorithm>
 #include <cstdio>
+#include <vector>
 #include <cstdlib>
 #include <random>
 #include <string>
@@ -35,13 +39,27 @@ void violation(const std::string &what, const tnni_tile_desc &d)
 
 int div_bm(const TileDev &t, int k) { return k / t.bm; }
 
-// One tile: every item, every block, every lane.
-void check_tile(const tnni_tile_desc &d, uint64_t n)
+void check_form(const tnni_tile_desc &d, uint64_t n);
+
+// One tile in both forms.
+void check_tile(tnni_tile_desc d, uint64_t n)
+{
+    for (int form : {TNNI_FORM_FUSED, TNNI_FORM_SPLIT}) {
+        d.form = form;
+        check_form(d, n);
+    }
+}
+
+// One tile in one form: every item, every block, every lane.
+void check_form(const tnni_tile_desc &d, uint64_t n)
 {
     TileDev t;
     PrepInfo info;
     std::string err;
     if (prepare_tile(d, n, 256, false, nullptr, t, info, err) != TNNI_OK) return;
+    if (t.label_only != (d.form == TNNI_FORM_SPLIT)) violation("form not taken", d);
+    // label-only items: how often each pixel is written
+    std::vector<uint8_t> owned(t.label_only ? (size_t)t.W * t.H : 0, 0);
     ++g_tiles;
     t.nbx4 = (t.nbx + 3) & ~3;
     const int64_t n_offs = (int64_t)t.nbx * t.nby + 1;  // offsets plus the sentinel
@@ -90,8 +108,14 @@ void check_tile(const tnni_tile_desc &d, uint64_t n)
                     if (c0 < 0 || c1 > ww) violation("staged column", d);
                     const int p = (l - g.rl0) * RW + (k - g.rk0);
                     if (p < 0 || p >= RW * RH) violation("label tile index", d);
+                    if (t.label_only) {
+                        if (k < g.ok0 || k > g.ok1 || l < g.ol0 || l > g.ol1) violation("label-only pixel not owned", d);
+                        else if (k < 0 || k >= t.W || l < 0 || l >= t.H) violation("label-only pixel off the raster", d);
+                        else owned[(size_t)l * t.W + k]++;
+                    }
                 }
             }
+            if (t.label_only) continue;  // no gather in the tile kernel
             // the gather: every pixel of every cell's query window inside the
             // item's label tile
             for (int j = g.j0; j < g.j1; ++j)
@@ -103,6 +127,43 @@ void check_tile(const tnni_tile_desc &d, uint64_t n)
                         j * t.scale + nb > g.rl1)
                         violation("gather window outside the label tile", d);
                 }
+        }
+    for (uint8_t c : owned)
+        if (c != 1) {
+            violation("label-only items do not own every pixel once", d);
+            break;
+        }
+    if (!t.label_only) return;
+    // the split form's gather: each block's pixel window within its LDS
+    // (gather_window_bytes of an interior block, under kGatherLds), every
+    // cell's reads inside the window, every cell of the tile in one block
+    const size_t wmax = gather_window_bytes(t, t.gb);
+    if (wmax > kGatherLds) violation("gather window past its LDS budget", d);
+    std::vector<uint8_t> cell_seen((size_t)t.nrows * t.ncols, 0);
+    for (int bj = 0; bj < t.gnty; ++bj)
+        for (int bi = 0; bi < t.gntx; ++bi) {
+            const int i0 = bi * t.gb, j0 = bj * t.gb;
+            const int i1 = std::min(i0 + t.gb, t.ncols), j1 = std::min(j0 + t.gb, t.nrows);
+            const int kx0 = std::max(0, i0 * t.scale + qa), kx1 = std::min(t.W - 1, (i1 - 1) * t.scale + qb);
+            const int ky0 = std::max(0, j0 * t.scale + qa), ky1 = std::min(t.H - 1, (j1 - 1) * t.scale + qb);
+            const size_t ww = (size_t)std::max(0, kx1 - kx0 + 1), wh = (size_t)std::max(0, ky1 - ky0 + 1);
+            if (ww * wh * 8 > wmax) violation("gather block window past the launch's LDS", d);
+            if (t.gb * t.gb > kBlock) violation("gather block larger than the workgroup", d);
+            for (int j = j0; j < j1; ++j)
+                for (int i = i0; i < i1; ++i) {
+                    cell_seen[(size_t)j * t.ncols + i]++;
+                    const int ma = std::max(qa, -i * t.scale), mb = std::min(qb, t.W - 1 - i * t.scale);
+                    const int na = std::max(qa, -j * t.scale), nb = std::min(qb, t.H - 1 - j * t.scale);
+                    if (ma > mb || na > nb) continue;
+                    if (i * t.scale + ma < kx0 || i * t.scale + mb > kx1 || j * t.scale + na < ky0 ||
+                        j * t.scale + nb > ky1)
+                        violation("gather reads outside the block's window", d);
+                }
+        }
+    for (uint8_t c : cell_seen)
+        if (c != 1) {
+            violation("gather blocks do not cover every cell once", d);
+            break;
         }
 }
 

@@ -40,10 +40,40 @@ def test_modes(ctx, mode, alg):
     check_tile(ctx, p, t.X, t.Y, t.Z, s0=(mode == T.ACC_S0))
 
 
-def test_uniform_100k_512(ctx):
-    _say("uniform 100k, 512^2 at 2 m, scale 5")
+@pytest.mark.parametrize("form", [T.FORM_AUTO, T.FORM_FUSED, T.FORM_SPLIT])
+def test_uniform_100k_512(ctx, form):
+    """The reference's CPU case (scale 5: a 17-pixel query window) in both
+    forms; automatic picks the split form here."""
+    _say("uniform 100k, 512^2 at 2 m, scale 5, form", form)
     t = S.workload_uniform_100k()
-    check_tile(ctx, t.params, t.X, t.Y, t.Z)
+    p = dataclasses.replace(t.params, form=form)
+    check_tile(ctx, p, t.X, t.Y, t.Z)
+
+
+@pytest.mark.parametrize("mode", [T.ACC_FIXED64, T.ACC_FP32_2SUM, T.ACC_S0])
+@pytest.mark.parametrize("alg", [T.ALG_PLANES, T.ALG_CONES])
+def test_split_form_without_label_outputs_equals_fused(ctx, mode, alg):
+    """Heights only (the split form's rasters then live in the engine's
+    scratch), on host and device buffers: bit-equal to the fused form, in
+    every height mode and region shape (the gather sums the same pixels in
+    the same order)."""
+    t = S.make_tile(12, 30000, 120, cell_size=2.0, scale=4, kind="clustered")
+    base = dataclasses.replace(t.params, accum_mode=mode, algorithm=alg, point_min=-300, multiplier=10000)
+    hs = []
+    for form in (T.FORM_FUSED, T.FORM_SPLIT):
+        p = dataclasses.replace(base, form=form)
+        h, _, _ = ctx.tile_run(p, t.X, t.Y, t.Z)
+        dX, dY, dZ = ctx.upload(t.X), ctx.upload(t.Y), ctx.upload(t.Z)
+        dh = ctx.alloc((p.nrows, p.ncols), np.float32)
+        ctx.tile_run_dev(p, dX, dY, dZ, t.n, dh)
+        hd = dh.to_host()
+        for a in (dX, dY, dZ, dh):
+            a.free()
+        assert np.array_equal(h.view(np.uint32), hd.view(np.uint32))
+        hs.append(h)
+    assert np.array_equal(hs[0].view(np.uint32), hs[1].view(np.uint32))
+    check_tile(ctx, dataclasses.replace(base, form=T.FORM_SPLIT), t.X, t.Y, t.Z,
+               s0=mode == T.ACC_S0)
 
 
 @pytest.mark.parametrize("bucket", [0, 1, 2, 3, 5, 8])

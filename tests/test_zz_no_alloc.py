@@ -44,6 +44,19 @@ def test_noalloc_equals_tile_run_dev_on_two_calls():
             assert (ha != T.NODATA).mean() > 0.5
 
 
+def test_noalloc_split_form_equals_tile_run_dev():
+    """A scale-5 tile takes the split form (its value and squared-distance
+    rasters then also come from the fixed scratch): equal to
+    tnni_tile_run_dev, after a larger fused call."""
+    with T.Context(0) as ctx:
+        fused = S.make_tile(34, 200_000, 512, cell_size=0.5, kind="clustered")
+        split = S.workload_uniform_100k()
+        for t in (fused, split):
+            ha, la, da, hb, lb, db = _run_both(ctx, t)
+            assert np.array_equal(la, lb) and np.array_equal(da, db)
+            assert np.array_equal(ha.view(np.uint32), hb.view(np.uint32))
+
+
 def test_noalloc_refuses_a_tile_over_its_scratch():
     with T.Context(0) as ctx:
         n = 5_000_000
