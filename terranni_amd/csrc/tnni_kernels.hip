@@ -1051,6 +1051,22 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
     //    |x - q|^2 <= d2(x); accumulate the label's value, then divide.
     const int qa = query_lo(t), qb = query_hi(t);
     const int ncx = g.i1 - g.i0, ncy = g.j1 - g.j0;
+    // A 3-pixel query window (scale 1): a cell whose window lies inside the
+    // raster tests the nine pixels at the same offsets as every other, so
+    // their squared query distances (plus a space-time slice's offset, or
+    // "never" outside a cone's disk) are one uniform table, and a test is
+    // dqt < d2 + 1 (unsigned: d2 + 1 wraps to 0 where no site covers the
+    // pixel, and is at most 2^32 - 1 otherwise, so this is dq <= d2)
+    const bool span3 = qb - qa == 2;
+    uint32_t dqt[9];
+#pragma unroll
+    for (int n = 0; n < 3; ++n)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const uint32_t ux = (uint32_t)(t.x0 + (qa + m) * t.ps_u), uy = (uint32_t)(t.y0 + (qa + n) * t.ps_u);
+            const uint32_t dq = ux * ux + uy * uy;
+            dqt[n * 3 + m] = (ALG == TNNI_ALG_CONES && dq > t.Rq2) ? 0xFFFFFFFFu : dq + (TIME ? t.jj2 : 0u);
+        }
     // cell c = (ci, cj) of the block, stepped by kBlock cells without a
     // division per cell
     const int step_j = kBlock / ncx, step_i = kBlock - step_j * ncx;
@@ -1068,6 +1084,20 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
         Acc<MODE> acc;
         const int64_t cell = (int64_t)j * t.ncols + i;
         if (TIME && t.acc_load) acc.load(t.acc[cell]);
+        if (span3 && ma == qa && mb == qb && na == qa && nb == qb) {
+            // the same pixels in the same order as the loop below
+            const int p0 = (j * t.scale + qa - g.rl0) * RW - g.rk0 + i * t.scale + qa;
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const int q = p0 + n * RW + m;
+                    acc.add_if(dqt[n * 3 + m] < s_d2[q] + 1u, s_val[q]);
+                }
+            if (!TIME || t.acc_final) t.out_h[cell] = acc.result(t);
+            else t.acc[cell] = acc.save();
+            continue;
+        }
         for (int n = na; n <= nb; ++n) {
             const uint32_t uy = (uint32_t)(t.y0 + n * t.ps_u);
             const int row = (j * t.scale + n - g.rl0) * RW - g.rk0 + i * t.scale;
