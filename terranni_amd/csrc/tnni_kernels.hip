@@ -1140,7 +1140,8 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
             s_npend = 0u;
         }
         __syncthreads();
-        const int64_t item = (int64_t)__builtin_amdgcn_readfirstlane(s_item);
+        // (readfirstlane is a signed int: read the counter back unsigned)
+        const int64_t item = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)s_item);
         __syncthreads();
         if (item >= n_items) break;
         int lo = 0, hi = n_tiles - 1;
