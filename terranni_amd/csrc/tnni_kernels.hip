@@ -581,7 +581,9 @@ __device__ __forceinline__ void scan_range_part(uint32_t s, uint32_t e, uint32_t
 }
 
 // square_global's sites shared by the G lanes of a group: lane sub visits
-// every G-th site of each bucket row's range.
+// every G-th site of each bucket row's range.  The row offsets come nine
+// rows at a time (the square of a long search at the built-in bucket side
+// is 9 rows: one round trip instead of three).
 template <int ALG, bool TIME>
 __device__ __forceinline__ void square_global_part(const TileDev &t, const uint32_t *__restrict__ offs,
                                                    const int4 *__restrict__ sorted, int bx, int by, int R,
@@ -591,16 +593,16 @@ __device__ __forceinline__ void square_global_part(const TileDev &t, const uint3
     const uint32_t *base = offs + t.bk_base;
     const int c0 = max(bx - R, 0), c1 = min(bx + R, t.nbx - 1);
     const int r0 = max(by - R, 0), r1 = min(by + R, t.nby - 1);
-    for (int row = r0; row <= r1; row += 4) {
-        uint32_t a[4], e[4];
+    for (int row = r0; row <= r1; row += 9) {
+        uint32_t a[9], e[9];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < 9; ++q) {
             const uint32_t *rs = base + (int64_t)min(row + q, r1) * t.nbx;
             a[q] = row + q <= r1 ? rs[c0] : 0u;
             e[q] = row + q <= r1 ? rs[c1 + 1] : 0u;
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 9; ++q)
             scan_range_part<ALG, TIME>(a[q], e[q], sub, G, sorted, px, py, Rr, t.Rs2, best, bval);
     }
 }
