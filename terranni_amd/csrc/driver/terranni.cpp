@@ -88,6 +88,7 @@ struct Options {
     std::string engine = "plain", format = "asc", accum = "fixed64";
     int max_size = 5000;
     bool bin_only = false;
+    long bin_block = 1L << 22;
 };
 
 const char *kUsage =
@@ -121,7 +122,9 @@ const char *kUsage =
     "  --accum fixed64|fp32|s0  height arithmetic (default fixed64)\n"
     "  --max-size N        largest raster side in pixels (default 5000, as the\n"
     "                      reference forces at terranni.cpp:697-698)\n"
-    "  --bin-only          stop after binning\n";
+    "  --bin-only          stop after binning\n"
+    "  --bin-block N       points a binning pass takes from a file at a time\n"
+    "                      (default 4194304)\n";
 
 Options parse(int argc, char **argv)
 {
@@ -184,12 +187,14 @@ Options parse(int argc, char **argv)
             else if (name == "time-length") o.time_length = (float)d;
             else if (name == "time-step") o.time_step = (float)d;
             else if (name == "max-size") o.max_size = (int)d;
+            else if (name == "bin-block") o.bin_block = (long)d;
             else die("unknown option --" + name + "\n" + kUsage);
         }
     }
     if (o.engine != "plain" && o.engine != "batched") die("--engine must be plain or batched");
     if (o.format != "asc" && o.format != "flt" && o.format != "both") die("--format must be asc, flt or both");
     if (o.accum != "fixed64" && o.accum != "fp32" && o.accum != "s0") die("--accum must be fixed64, fp32 or s0");
+    if (o.bin_block < 1 || o.bin_block > (1L << 30)) die("--bin-block must be in 1 .. 2^30");
     return o;
 }
 
@@ -274,84 +279,122 @@ struct Frame {
 };
 
 // Bin every point of `files` into the streams of the packages whose extent
-// holds it, then sort each stream by time (stable, so equal times keep the
-// file order; a stream whose times already arrived in order is left as it
-// is, which is what the stable sort would give).  At most kMaxOpen streams
-// are open at once: more packages take more passes over the input.
+// holds it, in one pass over the input, then sort each stream by time
+// (stable, so equal times keep the file order; a stream whose times already
+// arrived in order is left as it is, which is what the stable sort would
+// give).  Files go in path order, each read in blocks of kSuper points whose
+// points the threads bin side by side, thread k a contiguous part of the
+// block, into per-thread buffers per package; the buffers are appended to
+// the streams in thread order, so every stream holds its points in file
+// order, as a serial pass would.  Streams are opened on demand for
+// appending, at most kMaxOpen at once (the least recently used closed
+// first), so any number of packages takes one pass (the reference bins in
+// recursive levels of at most streamsPerPass streams, terranni.cpp:142-152,
+// 324-363).
 template <class Rec, class Make>
 void bin_points(const std::vector<std::string> &files, const Layout &L, const std::string &dir,
-                const char *ext, Make make)
+                const char *ext, size_t kSuper, Make make)
 {
-    constexpr int kMaxOpen = 256;
+    constexpr size_t kMaxOpen = 256;
     const int n_wp = L.cols * L.rows;
-    const size_t kBlock = 1 << 16;
-    // files are binned side by side, a group of them at a time, each into
-    // its own per-package buffers; the buffers then go to the streams in
-    // file order, so every stream holds its points in file order
-    const int n_par = std::max(1, omp_get_max_threads());
-    for (int g0 = 0; g0 < n_wp; g0 += kMaxOpen) {
-        const int g1 = std::min(n_wp, g0 + kMaxOpen);
-        const size_t ng = (size_t)(g1 - g0);
-        std::vector<std::unique_ptr<tpie::ami::stream<Rec>>> st;
-        for (int k = g0; k < g1; ++k) {
-            st.emplace_back(new tpie::ami::stream<Rec>(wp_file(dir, k / L.cols, k % L.cols, ext)));
-            if (!st.back()->is_valid()) die("cannot create " + wp_file(dir, k / L.cols, k % L.cols, ext));
-            st.back()->truncate(0);
+    // every package starts empty
+    for (int k = 0; k < n_wp; ++k) {
+        tpie::ami::stream<Rec> st(wp_file(dir, k / L.cols, k % L.cols, ext));
+        if (!st.is_valid()) die("cannot create " + wp_file(dir, k / L.cols, k % L.cols, ext));
+        st.truncate(0);
+    }
+    std::vector<int32_t> last_t((size_t)n_wp, INT32_MIN);
+    std::vector<char> in_order((size_t)n_wp, 1);
+    // open streams, most recently used last
+    std::map<int, std::unique_ptr<tpie::ami::stream<Rec>>> open;
+    std::vector<uint64_t> used((size_t)n_wp, 0);
+    uint64_t clock = 0;
+    auto append = [&](int k, const std::vector<Rec> &v) {
+        auto it = open.find(k);
+        if (it == open.end()) {
+            if (open.size() >= kMaxOpen) {
+                auto victim = open.begin();
+                for (auto jt = open.begin(); jt != open.end(); ++jt)
+                    if (used[(size_t)jt->first] < used[(size_t)victim->first]) victim = jt;
+                victim->second->flush();
+                open.erase(victim);
+            }
+            std::unique_ptr<tpie::ami::stream<Rec>> st(
+                new tpie::ami::stream<Rec>(wp_file(dir, k / L.cols, k % L.cols, ext)));
+            if (!st->is_valid()) die("cannot open " + wp_file(dir, k / L.cols, k % L.cols, ext));
+            it = open.emplace(k, std::move(st)).first;
         }
-        std::vector<int32_t> last_t(ng, INT32_MIN);
-        std::vector<char> in_order(ng, 1);
-        for (size_t f0 = 0; f0 < files.size(); f0 += (size_t)n_par) {
-            const size_t nf = std::min(files.size() - f0, (size_t)n_par);
-            std::vector<std::vector<std::vector<Rec>>> out(nf, std::vector<std::vector<Rec>>(ng));
-            std::vector<std::string> err(nf);
-#pragma omp parallel for schedule(dynamic, 1)
-            for (size_t fi = 0; fi < nf; ++fi) {
+        used[(size_t)k] = ++clock;
+        tpie::ami::stream<Rec> &st = *it->second;
+        if (st.seek(st.stream_len()) != tpie::ami::NO_ERROR ||
+            st.write_array(v.data(), (TPIE_OS_SIZE_T)v.size()) != tpie::ami::NO_ERROR)
+            die("writing a work package failed");
+        for (const Rec &rec : v) {
+            if ((int32_t)rec.t < last_t[(size_t)k]) in_order[(size_t)k] = 0;
+            last_t[(size_t)k] = (int32_t)rec.t;
+        }
+    };
+    const int n_par = std::max(1, omp_get_max_threads());
+    std::vector<int32_t> X(kSuper), Y(kSuper), Z(kSuper);
+    std::vector<uint16_t> I(kSuper);
+    // per thread: a buffer per package and the packages it touched
+    std::vector<std::vector<std::vector<Rec>>> buf((size_t)n_par, std::vector<std::vector<Rec>>((size_t)n_wp));
+    std::vector<std::vector<int>> touched((size_t)n_par);
+    std::vector<std::string> err((size_t)n_par);
+    const size_t fstep = std::max<size_t>(1, files.size() / 20);
+    for (size_t fi = 0; fi < files.size(); ++fi) {
+        const std::string &path = files[fi];
+        if (fi % fstep == 0) {
+            std::fprintf(stderr, "binning: file %zu of %zu\n", fi + 1, files.size());
+            std::fflush(stderr);
+        }
+        nni_io::las_file lf;
+        lf.open(path);
+        const nni_io::las_header &h = lf.header();
+        size_t got;
+        while ((got = lf.read_raw(kSuper, X.data(), Y.data(), Z.data(), I.data())) > 0) {
+#pragma omp parallel num_threads(n_par)
+            {
+                const int k = omp_get_thread_num(), nt = omp_get_num_threads();
+                const size_t b0 = got * (size_t)k / (size_t)nt, b1 = got * (size_t)(k + 1) / (size_t)nt;
                 try {
-                    std::vector<int32_t> X(kBlock), Y(kBlock), Z(kBlock);
-                    std::vector<uint16_t> I(kBlock);
-                    nni_io::las_file lf;
-                    lf.open(files[f0 + fi]);
-                    const nni_io::las_header &h = lf.header();
-                    size_t got;
-                    while ((got = lf.read_raw(kBlock, X.data(), Y.data(), Z.data(), I.data())) > 0) {
-                        for (size_t i = 0; i < got; ++i) {
-                            const double x = (double)X[i] * h.scale[0] + h.offset[0];
-                            const double y = (double)Y[i] * h.scale[1] + h.offset[1];
-                            int c0, c1, r0, r1;
-                            if (!wp_span(x - L.minx, L.wpw(), L.border, L.cols, c0, c1)) continue;
-                            if (!wp_span(y - L.miny, L.wph(), L.border, L.rows, r0, r1)) continue;
-                            const Rec rec = make(h, X[i], Y[i], Z[i], I[i], x, y);
-                            for (int r = r0; r <= r1; ++r)
-                                for (int c = c0; c <= c1; ++c) {
-                                    const int k = r * L.cols + c;
-                                    if (k >= g0 && k < g1) out[fi][(size_t)(k - g0)].push_back(rec);
-                                }
-                        }
+                    std::vector<std::vector<Rec>> &mine = buf[(size_t)k];
+                    for (size_t i = b0; i < b1; ++i) {
+                        const double x = (double)X[i] * h.scale[0] + h.offset[0];
+                        const double y = (double)Y[i] * h.scale[1] + h.offset[1];
+                        int c0, c1, r0, r1;
+                        if (!wp_span(x - L.minx, L.wpw(), L.border, L.cols, c0, c1)) continue;
+                        if (!wp_span(y - L.miny, L.wph(), L.border, L.rows, r0, r1)) continue;
+                        const Rec rec = make(h, X[i], Y[i], Z[i], I[i], x, y);
+                        for (int r = r0; r <= r1; ++r)
+                            for (int c = c0; c <= c1; ++c) {
+                                std::vector<Rec> &v = mine[(size_t)(r * L.cols + c)];
+                                if (v.empty()) touched[(size_t)k].push_back(r * L.cols + c);
+                                v.push_back(rec);
+                            }
                     }
                 } catch (const std::exception &e) {
-                    err[fi] = e.what();
+                    err[(size_t)k] = e.what();
                 }
             }
-            for (size_t fi = 0; fi < nf; ++fi) {
-                if (!err[fi].empty()) throw std::runtime_error(err[fi]);
-                for (size_t j = 0; j < ng; ++j) {
-                    const std::vector<Rec> &v = out[fi][j];
-                    if (v.empty()) continue;
-                    if (st[j]->write_array(v.data(), (TPIE_OS_SIZE_T)v.size()) != tpie::ami::NO_ERROR)
-                        die("writing a work package failed");
-                    for (const Rec &rec : v) {
-                        if ((int32_t)rec.t < last_t[j]) in_order[j] = 0;
-                        last_t[j] = (int32_t)rec.t;
-                    }
+            for (int k = 0; k < n_par; ++k) {
+                if (!err[(size_t)k].empty()) throw std::runtime_error(err[(size_t)k]);
+                for (int wp : touched[(size_t)k]) {
+                    std::vector<Rec> &v = buf[(size_t)k][(size_t)wp];
+                    append(wp, v);
+                    v.clear();
                 }
+                touched[(size_t)k].clear();
             }
         }
-        Timecmp cmp;
-        for (size_t j = 0; j < st.size(); ++j) {
-            st[j]->flush();
-            if (!in_order[j] && tpie::ami::sort(st[j].get(), &cmp) != tpie::ami::NO_ERROR)
-                die("sorting a work package failed");
-        }
+    }
+    for (auto &kv : open) kv.second->flush();
+    open.clear();
+    Timecmp cmp;
+    for (int k = 0; k < n_wp; ++k) {
+        if (in_order[(size_t)k]) continue;
+        tpie::ami::stream<Rec> st(wp_file(dir, k / L.cols, k % L.cols, ext));
+        if (!st.is_valid() || tpie::ami::sort(&st, &cmp) != tpie::ami::NO_ERROR) die("sorting a work package failed");
     }
 }
 
@@ -706,7 +749,12 @@ void run_batched(tnni_ctx *ctx, const Options &o, const Layout &L, const nni::Se
         std::vector<float> band;
         const double ts = wall();
         auto pending = submit_row(L.rows - 1);
+        const int rstep = std::max(1, L.rows / 10);
         for (int row = L.rows - 1; row >= 0; --row) {
+            if ((L.rows - 1 - row) % rstep == 0) {
+                std::fprintf(stderr, "interpolating: package row %d of %d\n", L.rows - row, L.rows);
+                std::fflush(stderr);
+            }
             std::vector<std::unique_ptr<Flight>> next;
             if (row > 0) next = submit_row(row - 1);
             for (auto &f : pending)
@@ -869,7 +917,7 @@ int run(int argc, char **argv)
         const double tb = wall();
         const bool twoD = o.two_d;
         if (batched) {
-            bin_points<RawPoint>(files, L, o.wp_path, ".raw",
+            bin_points<RawPoint>(files, L, o.wp_path, ".raw", (size_t)o.bin_block,
                                  [&frame, twoD](const nni_io::las_header &h, int32_t X, int32_t Y, int32_t Z,
                                                 uint16_t I, double x, double y) {
                                      RawPoint r;
@@ -895,7 +943,7 @@ int run(int argc, char **argv)
                                  });
             write_meta(o.wp_path, m);
         } else {
-            bin_points<nni::Point4D>(files, L, o.wp_path, ".tpie",
+            bin_points<nni::Point4D>(files, L, o.wp_path, ".tpie", (size_t)o.bin_block,
                                      [twoD](const nni_io::las_header &h, int32_t, int32_t, int32_t Z, uint16_t I,
                                             double x, double y) {
                                          return nni::Point4D(x, y, (double)Z * h.scale[2] + h.offset[2],

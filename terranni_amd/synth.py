@@ -341,51 +341,68 @@ def fbm_height(x, y, seed: int, hurst: float = 0.8, octaves: int = 12, base_m: f
     return z
 
 
+def _terrain_cell(seed: int, r: int, c: int, n: int, tile_m: float):
+    """The points of core cell (r, c) of the streamed terrain: n uniform
+    points in its tile_m square and their height noise, from the cell's own
+    seed, so that every tile that overlaps the cell regenerates the same
+    points."""
+    rng = np.random.default_rng([seed, r, c])
+    x = rng.uniform(c * tile_m, (c + 1) * tile_m, n)
+    y = rng.uniform(r * tile_m, (r + 1) * tile_m, n)
+    return x, y, rng.normal(0.0, 0.05, n)
+
+
+def _terrain_tile(job):
+    """Write LAS tile (r, c): the points of its core cell and of the
+    neighbouring cells' points within `margin` of it (the overlap, points
+    written to both tiles as exact duplicates), heights from the fBm."""
+    out_dir, seed, r, c, nt, n_cell, tile_m, margin, origin = job
+    import os
+    lo_x, hi_x = c * tile_m - margin, (c + 1) * tile_m + margin
+    lo_y, hi_y = r * tile_m - margin, (r + 1) * tile_m + margin
+    xs, ys, es = [], [], []
+    for rr in (r - 1, r, r + 1):
+        for cc in (c - 1, c, c + 1):
+            if 0 <= rr < nt and 0 <= cc < nt:
+                x, y, e = _terrain_cell(seed, rr, cc, n_cell, tile_m)
+                m = (x >= lo_x) & (x < hi_x) & (y >= lo_y) & (y < hi_y)
+                xs.append(x[m])
+                ys.append(y[m])
+                es.append(e[m])
+    x, y = np.concatenate(xs), np.concatenate(ys)
+    z = fbm_height(x, y, seed) + np.concatenate(es)
+    X = np.rint(x / LAS_SCALE[0]).astype(np.int32)
+    Y = np.rint(y / LAS_SCALE[1]).astype(np.int32)
+    Z = np.rint(z / LAS_SCALE[2]).astype(np.int32)
+    path = os.path.join(out_dir, f"tile_r{r:03d}_c{c:03d}.las")
+    write_las(path, X, Y, Z, None, offset=(origin[0], origin[1], 0.0))
+    return path, int(X.shape[0])
+
+
 def streamed_terrain_las(out_dir: str, seed: int = 5, extent_m: float = 5000.0,
                          n_points: int = 20_000_000, tile_m: float = 1000.0,
-                         overlap: float = 0.05, origin=(500000.0, 4200000.0), chunk: int = 4_000_000,
+                         overlap: float = 0.05, origin=(500000.0, 4200000.0), workers: int = 16,
                          progress=None):
     """The streamed-terrain workload: n_points uniform over an extent_m
     square of fBm terrain (Hurst 0.8), cut into LAS tiles of tile_m with
     `overlap` of the tile side shared with each neighbour (points there are
-    written to both files, exact duplicates).  Returns the list of files.
-    progress (optional): called with a line of text after each chunk of
-    points and each file written."""
-    import os
-    rng = np.random.default_rng(seed)
+    written to both files, exact duplicates).  Each tile_m core cell draws
+    its n_points / cells points from its own seed, so the tiles are written
+    independently, `workers` processes side by side.  Returns the list of
+    files (sorted).  progress (optional): called with a line of text as
+    tiles are written."""
+    import multiprocessing as mp
     nt = int(np.ceil(extent_m / tile_m))
-    margin = overlap * tile_m
-    per = [[[] for _ in range(nt)] for _ in range(nt)]
-    left = n_points
-    while left > 0:
-        k = min(chunk, left)
-        left -= k
-        x = rng.uniform(0, extent_m, k)
-        y = rng.uniform(0, extent_m, k)
-        z = fbm_height(x, y, seed) + rng.normal(0.0, 0.05, k)
-        X = np.rint(x / LAS_SCALE[0]).astype(np.int32)
-        Y = np.rint(y / LAS_SCALE[1]).astype(np.int32)
-        Z = np.rint(z / LAS_SCALE[2]).astype(np.int32)
-        for r in range(nt):
-            for c in range(nt):
-                m = ((x >= c * tile_m - margin) & (x < (c + 1) * tile_m + margin) &
-                     (y >= r * tile_m - margin) & (y < (r + 1) * tile_m + margin))
-                if m.any():
-                    per[r][c].append((X[m], Y[m], Z[m]))
-        if progress:
-            progress(f"terrain points {n_points - left} of {n_points}")
-    files = []
-    for r in range(nt):
-        for c in range(nt):
-            if not per[r][c]:
-                continue
-            X = np.concatenate([p[0] for p in per[r][c]])
-            Y = np.concatenate([p[1] for p in per[r][c]])
-            Z = np.concatenate([p[2] for p in per[r][c]])
-            path = os.path.join(out_dir, f"tile_r{r:03d}_c{c:03d}.las")
-            write_las(path, X, Y, Z, None, offset=(origin[0], origin[1], 0.0))
+    n_cell = int(round(n_points / (nt * nt)))
+    jobs = [(out_dir, seed, r, c, nt, n_cell, tile_m, overlap * tile_m, tuple(origin))
+            for r in range(nt) for c in range(nt)]
+    files, written = [], 0
+    procs = max(1, min(workers, len(jobs), len(__import__("os").sched_getaffinity(0))))
+    step = max(1, len(jobs) // 20)
+    with mp.get_context("fork").Pool(procs) as pool:
+        for k, (path, n) in enumerate(pool.imap_unordered(_terrain_tile, jobs, chunksize=1)):
             files.append(path)
-            per[r][c] = None
-            if progress:
-                progress(f"wrote {path} ({X.shape[0]} points)")
-    return files
+            written += n
+            if progress and ((k + 1) % step == 0 or k + 1 == len(jobs)):
+                progress(f"terrain: {k + 1} of {len(jobs)} LAS tiles written ({written} points)")
+    return sorted(files)

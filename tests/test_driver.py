@@ -39,13 +39,13 @@ def write_scene(d, seed=7, n=6000):
     return out
 
 
-def run_driver(tmp_path, engine, extra=()):
+def run_driver(tmp_path, engine, extra=(), grid=None):
     las = tmp_path / "las"
     wp = tmp_path / f"wp_{engine}"
     out = tmp_path / f"out_{engine}"
     wp.mkdir(exist_ok=True)
     out.mkdir(exist_ok=True)
-    g = GRID
+    g = grid or GRID
     cmd = [DRIVER, f"--tilepath={las}", f"--output={out}", f"--wp-path={wp}", f"--tmp-path={tmp_path}",
            f"--cell-size={g['cs']}", f"--scaling={g['scale']}", f"--site-radius={g['sr']}",
            f"--origin-x={OFFSET[0]}", f"--origin-y={OFFSET[1]}", f"--grid-cols={g['cols']}",
@@ -56,10 +56,10 @@ def run_driver(tmp_path, engine, extra=()):
     return wp, out
 
 
-def expected_packages(strips):
+def expected_packages(strips, grid=None):
     """Per package (row, col): the indices (file order) of the points within
     border of it, ordered by time then file order."""
-    g = GRID
+    g = grid or GRID
     cs = float(np.float32(g["cs"]))
     border = float(np.float32(5 * np.float32(g["cs"]) + np.float32(g["sr"])))
     wpw, wph = int(g["wpw"] / cs) * cs, int(g["wph"] / cs) * cs
@@ -121,6 +121,27 @@ def test_bin_batched_packages_hold_las_integers_in_one_frame(tmp_path):
     meta = dict(line.split("=", 1) for line in (wp / "wp.meta").read_text().split())
     assert float(meta["scale0"]) == S.LAS_SCALE[0] and float(meta["offset0"]) == OFFSET[0]
     assert (int(meta["cols"]), int(meta["rows"])) == (cols, rows)
+
+
+def test_bin_one_pass_over_many_packages(tmp_path):
+    """1260 packages of 6 x 5 cells (five times the streams a pass keeps
+    open) and blocks of 700 points (each file read in several blocks, each
+    block binned by all threads side by side): one pass over the input, and
+    every package still holds exactly its points, in file order, then
+    sorted by time."""
+    (tmp_path / "las").mkdir()
+    strips = write_scene(str(tmp_path / "las"))
+    grid = dict(GRID, wpw=6, wph=5)
+    wp, _ = run_driver(tmp_path, "batched", ["--bin-only", "--bin-block=700"], grid=grid)
+    pk, rows, cols = expected_packages(strips, grid)
+    assert rows * cols == 1260
+    X = np.concatenate([s[0] for s in strips])
+    Z = np.concatenate([s[2] for s in strips])
+    t = np.concatenate([s[3] for s in strips]).astype(np.int32)
+    for (r, c), idx in pk.items():
+        got = np.fromfile(wp / f"r{r}c{c}.raw", dtype=RAW)
+        assert np.array_equal(got["X"], X[idx]), (r, c)
+        assert np.array_equal(got["Z"], Z[idx]) and np.array_equal(got["t"], t[idx])
 
 
 def test_2d_flag_zeroes_the_times(tmp_path):
