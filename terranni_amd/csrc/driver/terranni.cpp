@@ -44,6 +44,7 @@
 #include <vector>
 
 #include <omp.h>
+#include <unistd.h>
 
 #include "las_reader.h"
 #include "nni.h"
@@ -434,8 +435,29 @@ public:
     // Rows of ncols heights, the first row the top one of the batch.
     void write_rows(const float *rows, int n)
     {
-        if (flt_ && std::fwrite(rows, sizeof(float), (size_t)n * ncols_, flt_) != (size_t)n * ncols_)
-            die("write failed");
+        if (flt_) {
+            // the rows' bytes at the file's end, in parts written side by
+            // side at their own offsets
+            const size_t bytes = (size_t)n * (size_t)ncols_ * sizeof(float);
+            const int fd = fileno(flt_);
+            const int np = std::max(1, std::min(omp_get_max_threads(), (int)(bytes >> 24)));
+            bool ok = true;
+#pragma omp parallel for num_threads(np) reduction(&& : ok)
+            for (int k = 0; k < np; ++k) {
+                const size_t b0 = bytes * (size_t)k / (size_t)np, b1 = bytes * (size_t)(k + 1) / (size_t)np;
+                size_t done = b0;
+                while (done < b1) {
+                    const ssize_t w = pwrite(fd, (const char *)rows + done, b1 - done, (off_t)(flt_off_ + done));
+                    if (w <= 0) {
+                        ok = false;
+                        break;
+                    }
+                    done += (size_t)w;
+                }
+            }
+            if (!ok) die("write failed");
+            flt_off_ += bytes;
+        }
         if (!asc_) return;
         // each value as the reference's `ostream << float << " "`: %g of the
         // value, six significant digits (terranni.cpp:471)
@@ -459,6 +481,7 @@ public:
 private:
     int ncols_;
     std::FILE *asc_ = nullptr, *flt_ = nullptr;
+    size_t flt_off_ = 0;  // bytes of the GridFloat file written
 };
 
 // ---- plain engine: the reference's binRun + outputASC ----------------------
@@ -589,6 +612,7 @@ void write_band(RasterOut &w, const Layout &L, int row, const std::vector<const 
     const int j_hi = std::min(L.nrrows, (row + 1) * L.wp_nr) - 1, j_lo = row * L.wp_nr;
     const int nb = j_hi - j_lo + 1;
     band.assign((size_t)std::max(nb, 0) * L.nrcols, -9999.0f);
+#pragma omp parallel for schedule(static)
     for (int jj = 0; jj < nb; ++jj) {
         const int j = j_hi - jj;       // global row
         const int lj = j - j_lo;       // row inside the packages
@@ -710,9 +734,12 @@ void run_batched(tnni_ctx *ctx, const Options &o, const Layout &L, const nni::Se
     for (float t : slices) {
         RasterOut w(slice_name(o.output, t), o.format, L.nrrows, L.nrcols, L.minx, L.miny, L.cs);
         const float tr = 0.0f;
-        // submit every package of one row of packages
+        // submit every package of one row of packages, the packages read and
+        // submitted side by side (the served form takes submits from any
+        // thread)
         auto submit_row = [&](int row) {
-            std::vector<std::unique_ptr<Flight>> fl;
+            std::vector<std::unique_ptr<Flight>> fl((size_t)L.cols);
+#pragma omp parallel for schedule(dynamic, 1)
             for (int col = 0; col < L.cols; ++col) {
                 std::unique_ptr<Flight> f(new Flight());
                 // the whole package in one read; it is sorted by time, so a
@@ -742,7 +769,7 @@ void run_batched(tnni_ctx *ctx, const Options &o, const Layout &L, const nni::Se
                 if (tnni_submit(ctx, &d, f->X.data(), f->Y.data(), f->Z.data(), f->X.size(), f->h.data(), nullptr,
                                 nullptr, &f->ticket) != TNNI_OK)
                     die(std::string("tile: ") + tnni_last_error(ctx));
-                fl.push_back(std::move(f));
+                fl[(size_t)col] = std::move(f);
             }
             return fl;
         };
@@ -757,8 +784,11 @@ void run_batched(tnni_ctx *ctx, const Options &o, const Layout &L, const nni::Se
             }
             std::vector<std::unique_ptr<Flight>> next;
             if (row > 0) next = submit_row(row - 1);
-            for (auto &f : pending)
-                if (tnni_wait(ctx, f->ticket, nullptr) != TNNI_OK) die(std::string("tile: ") + tnni_last_error(ctx));
+            // the waits copy the results out of the engine's staging, side by side
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int col = 0; col < L.cols; ++col)
+                if (tnni_wait(ctx, pending[(size_t)col]->ticket, nullptr) != TNNI_OK)
+                    die(std::string("tile: ") + tnni_last_error(ctx));
             const double tw = wall();
             std::vector<const float *> hp;
             for (auto &f : pending) hp.push_back(f->h.data());
