@@ -43,6 +43,7 @@
 #include <string>
 #include <vector>
 
+#include <fcntl.h>
 #include <omp.h>
 #include <unistd.h>
 
@@ -220,13 +221,40 @@ struct Layout {
     double origin_y(int row) const { return miny + row * wph(); }
 };
 
-// Packages c on one axis whose extent [c*w - b, (c+1)*w + b] holds v
-// (metres from the grid origin).
-bool wp_span(double v, double w, double b, int n, int &lo, int &hi)
+// The same packages from a = (v - b) / w - 1 and c = (v + b) / w:
+// max(0, ceil(a)) and min(n - 1, floor(c)) by truncation, which is exact
+// where it is used (0 < a < n, 0 <= c < n).
+bool wp_span_of(double a, double c, int n, int &lo, int &hi)
 {
-    lo = std::max(0, (int)std::ceil((v - b) / w - 1.0));
-    hi = std::min(n - 1, (int)std::floor((v + b) / w));
+    if (!(c >= 0.0) || !(a < (double)n)) return false;
+    lo = 0;
+    if (a > 0.0) {
+        lo = (int)a;
+        lo += (double)lo < a;
+    }
+    hi = c < (double)n ? std::min(n - 1, (int)c) : n - 1;
     return lo <= hi;
+}
+
+// Packages c on one axis whose extent [c*w - b, (c+1)*w + b] holds v
+// (metres from the grid origin): wp_span_of the quotients by w, each taken
+// as a product with inv_w = 1 / w where that cannot change an integer
+// part.  The product differs from the quotient by a few units in the last
+// place, so a product farther than 1e-9 (relative) from every integer lies
+// between the same two integers as the quotient, and every decision
+// wp_span_of makes depends only on that; nearer, the quotient is taken.
+bool wp_span(double v, double w, double inv_w, double b, int n, int &lo, int &hi)
+{
+    auto near_int = [](double t) {
+        if (!(std::fabs(t) < 1e15)) return true;
+        const double f = std::fabs(t - (double)(long long)t);
+        const double m = 1e-9 * (1.0 + std::fabs(t));
+        return f < m || f > 1.0 - m;
+    };
+    double a = (v - b) * inv_w - 1.0, c = (v + b) * inv_w;
+    if (near_int(a)) a = (v - b) / w - 1.0;
+    if (near_int(c)) c = (v + b) / w;
+    return wp_span_of(a, c, n, lo, hi);
 }
 
 std::vector<std::string> find_tiles(const std::string &dir)
@@ -283,118 +311,233 @@ struct Frame {
 // holds it, in one pass over the input, then sort each stream by time
 // (stable, so equal times keep the file order; a stream whose times already
 // arrived in order is left as it is, which is what the stable sort would
-// give).  Files go in path order, each read in blocks of kSuper points whose
-// points the threads bin side by side, thread k a contiguous part of the
-// block, into per-thread buffers per package; the buffers are appended to
-// the streams in thread order, so every stream holds its points in file
-// order, as a serial pass would.  Streams are opened on demand for
-// appending, at most kMaxOpen at once (the least recently used closed
-// first), so any number of packages takes one pass (the reference bins in
-// recursive levels of at most streamsPerPass streams, terranni.cpp:142-152,
-// 324-363).
+// give).  The points are binned into slots, each holding its records
+// package by package, and the slots are appended to the streams in slot
+// order, so every stream holds its points in file order, as a serial pass
+// would:
+//   - files of at most kSuper points go in groups of at most one per thread
+//     and kSuper points in all, each file read and binned by one thread
+//     into its own slot;
+//   - a larger file is read in blocks of kSuper points, the threads binning
+//     contiguous parts of a block into their slots.
+// make(header) gives the file's record maker, maker(X, Y, Z, intensity, x,
+// y) a record.  A stream is a file of raw records, so the slots are
+// appended package by package, the packages in parallel, each with one
+// positioned write per slot at the stream's end.  Any number of packages
+// takes one pass (the reference bins in recursive levels of at most
+// streamsPerPass streams, terranni.cpp:142-152, 324-363).
 template <class Rec, class Make>
 void bin_points(const std::vector<std::string> &files, const Layout &L, const std::string &dir,
                 const char *ext, size_t kSuper, Make make)
 {
-    constexpr size_t kMaxOpen = 256;
     const int n_wp = L.cols * L.rows;
-    // every package starts empty
+    if (L.border / std::min(L.wpw(), L.wph()) > 30000.0) die("the border spans too many work packages");
+    std::vector<std::string> path((size_t)n_wp);
     for (int k = 0; k < n_wp; ++k) {
-        tpie::ami::stream<Rec> st(wp_file(dir, k / L.cols, k % L.cols, ext));
-        if (!st.is_valid()) die("cannot create " + wp_file(dir, k / L.cols, k % L.cols, ext));
-        st.truncate(0);
+        // every package starts empty
+        path[(size_t)k] = wp_file(dir, k / L.cols, k % L.cols, ext);
+        const int fd = ::open(path[(size_t)k].c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+        if (fd < 0) die("cannot create " + path[(size_t)k]);
+        ::close(fd);
     }
+    std::vector<uint64_t> len((size_t)n_wp, 0);  // records in each stream
     std::vector<int32_t> last_t((size_t)n_wp, INT32_MIN);
-    std::vector<char> in_order((size_t)n_wp, 1);
-    // open streams, most recently used last
-    std::map<int, std::unique_ptr<tpie::ami::stream<Rec>>> open;
-    std::vector<uint64_t> used((size_t)n_wp, 0);
-    uint64_t clock = 0;
-    auto append = [&](int k, const std::vector<Rec> &v) {
-        auto it = open.find(k);
-        if (it == open.end()) {
-            if (open.size() >= kMaxOpen) {
-                auto victim = open.begin();
-                for (auto jt = open.begin(); jt != open.end(); ++jt)
-                    if (used[(size_t)jt->first] < used[(size_t)victim->first]) victim = jt;
-                victim->second->flush();
-                open.erase(victim);
+    std::vector<char> in_order((size_t)n_wp, 1), seen((size_t)n_wp, 0);
+    const int n_par = std::max(1, omp_get_max_threads());
+    // A slot's records lie package by package in one buffer, kept from one
+    // use to the next, so binning allocates only when a slot outgrows it.
+    // a point's packages: rows r0..r0+dr, columns c0..c0+dc, k0 = r0 * cols + c0
+    struct Span {
+        uint32_t k0;  // UINT32_MAX: none
+        uint16_t dc, dr;
+    };
+    struct Slot {
+        std::vector<int32_t> X, Y, Z;  // a whole file's points (group form)
+        std::vector<uint16_t> I;
+        std::vector<Span> span;
+        std::unique_ptr<Rec[]> recs;
+        size_t cap = 0;
+        std::vector<uint64_t> end;  // per package: one past its last record in recs
+        std::vector<uint32_t> cnt;  // per package: its records in this slot
+        std::vector<int> touched;   // the packages with records, in first-touch order
+        std::string err;
+    };
+    std::vector<Slot> slots((size_t)n_par);
+    for (Slot &sl : slots) {
+        sl.end.assign((size_t)n_wp, 0);
+        sl.cnt.assign((size_t)n_wp, 0);
+    }
+
+    // Bin points [b0, b1) into an empty slot: find each point's packages
+    // (sp[i - b0]) and count the records per package, then place them.
+    auto bin_range = [&](int slot, const nni_io::las_header &h, const int32_t *X, const int32_t *Y,
+                         const int32_t *Z, const uint16_t *I, size_t b0, size_t b1, Span *sp) {
+        Slot &sl = slots[(size_t)slot];
+        const double wpw = L.wpw(), wph = L.wph(), iw = 1.0 / wpw, ih = 1.0 / wph;
+        for (size_t i = b0; i < b1; ++i) {
+            const double x = (double)X[i] * h.scale[0] + h.offset[0];
+            const double y = (double)Y[i] * h.scale[1] + h.offset[1];
+            int c0, c1, r0, r1;
+            Span &p = sp[i - b0];
+            if (!wp_span(x - L.minx, wpw, iw, L.border, L.cols, c0, c1) ||
+                !wp_span(y - L.miny, wph, ih, L.border, L.rows, r0, r1)) {
+                p.k0 = UINT32_MAX;
+                continue;
             }
-            std::unique_ptr<tpie::ami::stream<Rec>> st(
-                new tpie::ami::stream<Rec>(wp_file(dir, k / L.cols, k % L.cols, ext)));
-            if (!st->is_valid()) die("cannot open " + wp_file(dir, k / L.cols, k % L.cols, ext));
-            it = open.emplace(k, std::move(st)).first;
+            p.k0 = (uint32_t)(r0 * L.cols + c0);
+            p.dc = (uint16_t)(c1 - c0);
+            p.dr = (uint16_t)(r1 - r0);
+            for (int r = r0; r <= r1; ++r)
+                for (int c = c0; c <= c1; ++c)
+                    if (sl.cnt[(size_t)(r * L.cols + c)]++ == 0) sl.touched.push_back(r * L.cols + c);
         }
-        used[(size_t)k] = ++clock;
-        tpie::ami::stream<Rec> &st = *it->second;
-        if (st.seek(st.stream_len()) != tpie::ami::NO_ERROR ||
-            st.write_array(v.data(), (TPIE_OS_SIZE_T)v.size()) != tpie::ami::NO_ERROR)
-            die("writing a work package failed");
-        for (const Rec &rec : v) {
-            if ((int32_t)rec.t < last_t[(size_t)k]) in_order[(size_t)k] = 0;
-            last_t[(size_t)k] = (int32_t)rec.t;
+        uint64_t total = 0;
+        for (int k : sl.touched) {
+            sl.end[(size_t)k] = total;  // the fill cursor first
+            total += sl.cnt[(size_t)k];
+        }
+        if (total > sl.cap) {
+            sl.recs.reset();
+            sl.recs.reset(new Rec[total]);
+            sl.cap = total;
+        }
+        Rec *recs = sl.recs.get();
+        const auto conv = make(h);
+        for (size_t i = b0; i < b1; ++i) {
+            const Span p = sp[i - b0];
+            if (p.k0 == UINT32_MAX) continue;
+            const double x = (double)X[i] * h.scale[0] + h.offset[0];
+            const double y = (double)Y[i] * h.scale[1] + h.offset[1];
+            const Rec rec = conv(X[i], Y[i], Z[i], I[i], x, y);
+            for (size_t r = 0, k = p.k0; r <= p.dr; ++r, k += (size_t)L.cols)
+                for (size_t c = 0; c <= p.dc; ++c) recs[sl.end[k + c]++] = rec;
         }
     };
-    const int n_par = std::max(1, omp_get_max_threads());
-    std::vector<int32_t> X(kSuper), Y(kSuper), Z(kSuper);
-    std::vector<uint16_t> I(kSuper);
-    // per thread: a buffer per package and the packages it touched
-    std::vector<std::vector<std::vector<Rec>>> buf((size_t)n_par, std::vector<std::vector<Rec>>((size_t)n_wp));
-    std::vector<std::vector<int>> touched((size_t)n_par);
-    std::vector<std::string> err((size_t)n_par);
+    // append slots [0, n_slots) to the streams, in slot order, and empty them
+    auto flush = [&](int n_slots) {
+        for (int s = 0; s < n_slots; ++s)
+            if (!slots[(size_t)s].err.empty()) die(slots[(size_t)s].err);
+        std::vector<int> wps;
+        for (int s = 0; s < n_slots; ++s)
+            for (int wp : slots[(size_t)s].touched)
+                if (!seen[(size_t)wp]) {
+                    seen[(size_t)wp] = 1;
+                    wps.push_back(wp);
+                }
+        bool ok = true;
+#pragma omp parallel for schedule(dynamic, 1) reduction(&& : ok)
+        for (size_t i = 0; i < wps.size(); ++i) {
+            const size_t wp = (size_t)wps[i];
+            seen[wp] = 0;
+            const int fd = ::open(path[wp].c_str(), O_WRONLY);
+            if (fd < 0) {
+                ok = false;
+                continue;
+            }
+            for (int s = 0; s < n_slots; ++s) {
+                Slot &sl = slots[(size_t)s];
+                const size_t n = sl.cnt[wp];
+                if (n == 0) continue;
+                const Rec *v = sl.recs.get() + (sl.end[wp] - n);
+                const size_t bytes = n * sizeof(Rec);
+                size_t done = 0;
+                while (ok && done < bytes) {
+                    const ssize_t w = ::pwrite(fd, (const char *)v + done, bytes - done,
+                                               (off_t)(len[wp] * sizeof(Rec) + done));
+                    if (w <= 0) ok = false;
+                    else done += (size_t)w;
+                }
+                len[wp] += n;
+                for (size_t j = 0; j < n; ++j) {
+                    if ((int32_t)v[j].t < last_t[wp]) in_order[wp] = 0;
+                    last_t[wp] = (int32_t)v[j].t;
+                }
+                sl.cnt[wp] = 0;
+            }
+            if (::close(fd) != 0) ok = false;
+        }
+        if (!ok) die("writing a work package failed");
+        for (int s = 0; s < n_slots; ++s) slots[(size_t)s].touched.clear();
+    };
+
     const size_t fstep = std::max<size_t>(1, files.size() / 20);
-    for (size_t fi = 0; fi < files.size(); ++fi) {
-        const std::string &path = files[fi];
-        if (fi % fstep == 0) {
-            std::fprintf(stderr, "binning: file %zu of %zu\n", fi + 1, files.size());
-            std::fflush(stderr);
+    size_t fi = 0;
+    std::unique_ptr<nni_io::las_file> next;
+    while (fi < files.size()) {
+        // the next group of small files, or one large file alone
+        std::vector<std::unique_ptr<nni_io::las_file>> grp;
+        uint64_t total = 0;
+        while (fi + grp.size() < files.size() && (int)grp.size() < n_par) {
+            if (!next) {
+                next.reset(new nni_io::las_file());
+                next->open(files[fi + grp.size()]);
+            }
+            const uint64_t n = next->header().n_points;
+            if (n > kSuper ? !grp.empty() : total + n > kSuper && !grp.empty()) break;
+            total += n;
+            grp.push_back(std::move(next));
+            if (n > kSuper) break;
         }
-        nni_io::las_file lf;
-        lf.open(path);
-        const nni_io::las_header &h = lf.header();
-        size_t got;
-        while ((got = lf.read_raw(kSuper, X.data(), Y.data(), Z.data(), I.data())) > 0) {
+        for (size_t k = fi; k < fi + grp.size(); ++k)
+            if (k % fstep == 0) {
+                std::fprintf(stderr, "binning: file %zu of %zu\n", k + 1, files.size());
+                std::fflush(stderr);
+            }
+        if (grp.size() == 1 && grp[0]->header().n_points > kSuper) {
+            nni_io::las_file &lf = *grp[0];
+            const nni_io::las_header &h = lf.header();
+            std::vector<int32_t> X(kSuper), Y(kSuper), Z(kSuper);
+            std::vector<uint16_t> I(kSuper);
+            std::vector<Span> sp(kSuper);
+            size_t got;
+            while ((got = lf.read_raw(kSuper, X.data(), Y.data(), Z.data(), I.data())) > 0) {
+                int used = 1;
 #pragma omp parallel num_threads(n_par)
-            {
-                const int k = omp_get_thread_num(), nt = omp_get_num_threads();
-                const size_t b0 = got * (size_t)k / (size_t)nt, b1 = got * (size_t)(k + 1) / (size_t)nt;
-                try {
-                    std::vector<std::vector<Rec>> &mine = buf[(size_t)k];
-                    for (size_t i = b0; i < b1; ++i) {
-                        const double x = (double)X[i] * h.scale[0] + h.offset[0];
-                        const double y = (double)Y[i] * h.scale[1] + h.offset[1];
-                        int c0, c1, r0, r1;
-                        if (!wp_span(x - L.minx, L.wpw(), L.border, L.cols, c0, c1)) continue;
-                        if (!wp_span(y - L.miny, L.wph(), L.border, L.rows, r0, r1)) continue;
-                        const Rec rec = make(h, X[i], Y[i], Z[i], I[i], x, y);
-                        for (int r = r0; r <= r1; ++r)
-                            for (int c = c0; c <= c1; ++c) {
-                                std::vector<Rec> &v = mine[(size_t)(r * L.cols + c)];
-                                if (v.empty()) touched[(size_t)k].push_back(r * L.cols + c);
-                                v.push_back(rec);
-                            }
+                {
+                    const int k = omp_get_thread_num(), nt = omp_get_num_threads();
+                    if (k == 0) used = nt;
+                    const size_t b0 = got * (size_t)k / (size_t)nt, b1 = got * (size_t)(k + 1) / (size_t)nt;
+                    try {
+                        bin_range(k, h, X.data(), Y.data(), Z.data(), I.data(), b0, b1, sp.data() + b0);
+                    } catch (const std::exception &e) {
+                        slots[(size_t)k].err = e.what();
                     }
+                }
+                flush(used);
+            }
+        } else {
+            const int g = (int)grp.size();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(std::min(n_par, g))
+            for (int s = 0; s < g; ++s) {
+                Slot &sl = slots[(size_t)s];
+                try {
+                    nni_io::las_file &lf = *grp[(size_t)s];
+                    const size_t n = (size_t)lf.header().n_points;
+                    if (sl.X.size() < n) {
+                        sl.X.resize(n);
+                        sl.Y.resize(n);
+                        sl.Z.resize(n);
+                        sl.I.resize(n);
+                        sl.span.resize(n);
+                    }
+                    size_t got = 0, k;
+                    while (got < n && (k = lf.read_raw(n - got, sl.X.data() + got, sl.Y.data() + got,
+                                                       sl.Z.data() + got, sl.I.data() + got)) > 0)
+                        got += k;
+                    bin_range(s, lf.header(), sl.X.data(), sl.Y.data(), sl.Z.data(), sl.I.data(), 0, got,
+                              sl.span.data());
                 } catch (const std::exception &e) {
-                    err[(size_t)k] = e.what();
+                    sl.err = e.what();
                 }
             }
-            for (int k = 0; k < n_par; ++k) {
-                if (!err[(size_t)k].empty()) throw std::runtime_error(err[(size_t)k]);
-                for (int wp : touched[(size_t)k]) {
-                    std::vector<Rec> &v = buf[(size_t)k][(size_t)wp];
-                    append(wp, v);
-                    v.clear();
-                }
-                touched[(size_t)k].clear();
-            }
+            flush(g);
         }
+        fi += grp.size();
     }
-    for (auto &kv : open) kv.second->flush();
-    open.clear();
     Timecmp cmp;
     for (int k = 0; k < n_wp; ++k) {
         if (in_order[(size_t)k]) continue;
-        tpie::ami::stream<Rec> st(wp_file(dir, k / L.cols, k % L.cols, ext));
+        tpie::ami::stream<Rec> st(path[(size_t)k]);
         if (!st.is_valid() || tpie::ami::sort(&st, &cmp) != tpie::ami::NO_ERROR) die("sorting a work package failed");
     }
 }
@@ -948,36 +1091,39 @@ int run(int argc, char **argv)
         const bool twoD = o.two_d;
         if (batched) {
             bin_points<RawPoint>(files, L, o.wp_path, ".raw", (size_t)o.bin_block,
-                                 [&frame, twoD](const nni_io::las_header &h, int32_t X, int32_t Y, int32_t Z,
-                                                uint16_t I, double x, double y) {
-                                     RawPoint r;
-                                     r.t = twoD ? 0 : (int)I;
+                                 [&frame, twoD](const nni_io::las_header &h) {
                                      const bool same = std::memcmp(h.scale, frame.scale, sizeof frame.scale) == 0 &&
                                                        std::memcmp(h.offset, frame.offset, sizeof frame.offset) == 0;
-                                     if (same) {
-                                         r.X = X;
-                                         r.Y = Y;
-                                         r.Z = Z;
-                                     } else {
-                                         const double z = (double)Z * h.scale[2] + h.offset[2];
-                                         auto q = [](double v, double sc, double of) {
-                                             const double u = std::nearbyint((v - of) / sc);
-                                             if (!(std::fabs(u) < 2147483647.0)) die("a point does not fit the common LAS frame");
-                                             return (int32_t)u;
-                                         };
-                                         r.X = q(x, frame.scale[0], frame.offset[0]);
-                                         r.Y = q(y, frame.scale[1], frame.offset[1]);
-                                         r.Z = q(z, frame.scale[2], frame.offset[2]);
-                                     }
-                                     return r;
+                                     return [&frame, &h, twoD, same](int32_t X, int32_t Y, int32_t Z, uint16_t I,
+                                                                     double x, double y) {
+                                         RawPoint r;
+                                         r.t = twoD ? 0 : (int)I;
+                                         if (same) {
+                                             r.X = X;
+                                             r.Y = Y;
+                                             r.Z = Z;
+                                         } else {
+                                             const double z = (double)Z * h.scale[2] + h.offset[2];
+                                             auto q = [](double v, double sc, double of) {
+                                                 const double u = std::nearbyint((v - of) / sc);
+                                                 if (!(std::fabs(u) < 2147483647.0)) die("a point does not fit the common LAS frame");
+                                                 return (int32_t)u;
+                                             };
+                                             r.X = q(x, frame.scale[0], frame.offset[0]);
+                                             r.Y = q(y, frame.scale[1], frame.offset[1]);
+                                             r.Z = q(z, frame.scale[2], frame.offset[2]);
+                                         }
+                                         return r;
+                                     };
                                  });
             write_meta(o.wp_path, m);
         } else {
             bin_points<nni::Point4D>(files, L, o.wp_path, ".tpie", (size_t)o.bin_block,
-                                     [twoD](const nni_io::las_header &h, int32_t, int32_t, int32_t Z, uint16_t I,
-                                            double x, double y) {
-                                         return nni::Point4D(x, y, (double)Z * h.scale[2] + h.offset[2],
-                                                             twoD ? 0 : (int)I);
+                                     [twoD](const nni_io::las_header &h) {
+                                         return [&h, twoD](int32_t, int32_t, int32_t Z, uint16_t I, double x, double y) {
+                                             return nni::Point4D(x, y, (double)Z * h.scale[2] + h.offset[2],
+                                                                 twoD ? 0 : (int)I);
+                                         };
                                      });
         }
         t_bin = wall() - tb;

@@ -123,16 +123,20 @@ def test_bin_batched_packages_hold_las_integers_in_one_frame(tmp_path):
     assert (int(meta["cols"]), int(meta["rows"])) == (cols, rows)
 
 
-def test_bin_one_pass_over_many_packages(tmp_path):
-    """1260 packages of 6 x 5 cells (five times the streams a pass keeps
-    open) and blocks of 700 points (each file read in several blocks, each
-    block binned by all threads side by side): one pass over the input, and
-    every package still holds exactly its points, in file order, then
-    sorted by time."""
+@pytest.mark.parametrize("block", [700, 4500, 0])
+def test_bin_one_pass_over_many_packages(tmp_path, block):
+    """1260 packages of 6 x 5 cells in one pass over the input, three files
+    of about 2000 points: blocks of 700 points (each file read in several
+    blocks, each block binned by all threads side by side), of 4500 (the
+    first two files binned side by side, the third alone) and the default
+    (all three side by side); every package holds exactly its points, in
+    file order, then sorted by time."""
     (tmp_path / "las").mkdir()
     strips = write_scene(str(tmp_path / "las"))
+    assert all(1500 < s[0].shape[0] < 2250 for s in strips)
     grid = dict(GRID, wpw=6, wph=5)
-    wp, _ = run_driver(tmp_path, "batched", ["--bin-only", "--bin-block=700"], grid=grid)
+    extra = ["--bin-only"] + ([f"--bin-block={block}"] if block else [])
+    wp, _ = run_driver(tmp_path, "batched", extra, grid=grid)
     pk, rows, cols = expected_packages(strips, grid)
     assert rows * cols == 1260
     X = np.concatenate([s[0] for s in strips])
