@@ -728,7 +728,8 @@ def main():
     ap.add_argument("--tile-cells", type=int, default=0, help="cell-tile side (0 = auto)")
     ap.add_argument("--bucket-px", type=int, default=0, help="bucket side in pixels (0 = auto)")
     ap.add_argument("--form", type=int, default=0,
-                    help="0: automatic, 1: fused, 2: split (label raster, then a gather pass)")
+                    help="0: automatic, 1: fused, 2: split (label raster, then a gather pass), "
+                         "3: scatter (label raster, then each pixel spread over its cells)")
     ap.add_argument("--workload", choices=("single", "batch", "stream", "spacetime", "ref100k"),
                     default="single",
                     help="single: the 4096^2 tile (default line, plus the batched and served "

@@ -96,15 +96,24 @@ typedef struct tnni_tile_desc {
                                    accumulation, 0 = tuned / automatic     */
 } tnni_tile_desc;
 
-/* tnni_tile_desc::form.  Either gives the same results bit for bit.
+/* tnni_tile_desc::form.  Every form gives the same results bit for bit.
  *  FUSED: each work item labels its cells' pixels plus the query halo into
  *    LDS and accumulates its cells from there in the same kernel (the halo
  *    is labelled by every item that needs it);
  *  SPLIT: each pixel is labelled once (items without a halo) into a raster
  *    of site values and squared distances in device memory, then a second
- *    kernel accumulates every cell from that raster.  Pays where the query
- *    window is wide in pixels (large scale), so that the halo dominates. */
-enum { TNNI_FORM_AUTO = 0, TNNI_FORM_FUSED = 1, TNNI_FORM_SPLIT = 2 };
+ *    kernel gathers every cell from that raster (each cell tests every
+ *    pixel of its query window).  Pays where the query window is wide in
+ *    pixels (large scale), so that the halo dominates;
+ *  SCATTER: the split form's labelling, then the accumulation the other
+ *    way round (cuda_nni.cu:407-488): each pixel adds its site value to
+ *    the cells within its nearest-site distance, visiting only the cells of
+ *    its disk's bounding box, into per-block LDS sums with lanes that hit
+ *    the same cell summed across the wavefront first.  Integer modes only
+ *    (TNNI_ACC_FIXED64, TNNI_ACC_S0), whose sums do not depend on order;
+ *    TNNI_E_INVALID with TNNI_ACC_FP32_2SUM.  Never chosen automatically
+ *    unless the autotuner measured it faster. */
+enum { TNNI_FORM_AUTO = 0, TNNI_FORM_FUSED = 1, TNNI_FORM_SPLIT = 2, TNNI_FORM_SCATTER = 3 };
 
 /* Derived geometry of a tile (all lengths in fixed-point units). */
 typedef struct tnni_frame_info {
@@ -324,7 +333,7 @@ typedef struct tnni_tune_entry {
     int32_t candidates;  /* settings timed (tnni_autotune only)             */
     float ms_default;    /* sample time, built-in choice (tnni_autotune)    */
     float ms_tuned;      /* sample time, chosen setting (tnni_autotune)     */
-    int32_t form;        /* TNNI_FORM_FUSED or TNNI_FORM_SPLIT (0: built-in) */
+    int32_t form;        /* TNNI_FORM_FUSED, _SPLIT or _SCATTER (0: built-in) */
     int32_t pad;
 } tnni_tune_entry;
 

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(LIB_DIR, "libterranni_hip.so")
 
 ALG_PLANES, ALG_CONES = 0, 1
 ACC_FIXED64, ACC_FP32_2SUM, ACC_S0 = 0, 1, 2
-FORM_AUTO, FORM_FUSED, FORM_SPLIT = 0, 1, 2
+FORM_AUTO, FORM_FUSED, FORM_SPLIT, FORM_SCATTER = 0, 1, 2, 3
 NODATA = -9999.0
 NO_D2 = 0xFFFFFFFF
 
@@ -120,7 +120,7 @@ class TileParams:
     multiplier: int = 1
     bucket_px: int = 0
     tile_cells: int = 0
-    form: int = 0   # FORM_AUTO / FORM_FUSED / FORM_SPLIT
+    form: int = 0   # FORM_AUTO / FORM_FUSED / FORM_SPLIT / FORM_SCATTER (integer modes)
 
     def qr(self) -> float:
         if self.query_radius is not None:

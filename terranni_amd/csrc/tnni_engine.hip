@@ -427,7 +427,7 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
         any_label_only = any_label_only || t.label_only;
         if (t.label_only && t.out_h) {
             gather_items += (int64_t)t.gntx * t.gnty;
-            gather_lds = std::max(gather_lds, gather_window_bytes(t, t.gb));
+            gather_lds = std::max(gather_lds, split_pass_lds(t));
             split_pix += (int64_t)t.W * t.H;
             if (!t.out_d2) split_d2_pix += (int64_t)t.W * t.H;
         }
@@ -1148,7 +1148,7 @@ tnni_status tnni_tile_run_dev_noalloc(tnni_ctx *ctx, const tnni_tile_desc *desc,
     cfg.row_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, t.nby));
     cfg.main_grid = (int)std::max<int64_t>(1, std::min<int64_t>(bd.n_items, (int64_t)ctx->n_cu * occ));
     cfg.gather_grid = (int)std::min<int64_t>(gather_items, 8192);
-    cfg.gather_lds = t.label_only ? gather_window_bytes(t, t.gb) : 0;
+    cfg.gather_lds = t.label_only ? split_pass_lds(t) : 0;
     HIP_TRY(ctx, enqueue_pipeline(db, cfg, a.stream, nullptr, nullptr));
     HIP_TRY(ctx, hipStreamSynchronize(a.stream));
     return TNNI_OK;
@@ -1930,6 +1930,7 @@ tnni_status tnni_autotune(tnni_ctx *ctx, int n_tiles, const tnni_tile_desc *desc
             }
         // the other form (fused or split) at the built-in and the best
         // bucket side, with its own three most lane-efficient cell tiles
+        int tc_split = f0 == TNNI_FORM_SPLIT ? tc0 : 0;  // the split form's built-in label tile
         {
             const int f1 = f0 == TNNI_FORM_SPLIT ? TNNI_FORM_FUSED : TNNI_FORM_SPLIT;
             tnni_tile_desc d1 = d0;
@@ -1937,6 +1938,7 @@ tnni_status tnni_autotune(tnni_ctx *ctx, int n_tiles, const tnni_tile_desc *desc
             TileDev t1;
             PrepInfo i1;
             if (prepare_tile(d1, n[idx[0]], ctx->n_cu, bat, nullptr, t1, i1, err) == TNNI_OK) {
+                if (f1 == TNNI_FORM_SPLIT) tc_split = t1.tc;
                 std::vector<int> tcs1{t1.tc};
                 for (int c : rank_tile_cells(t1, bat ? kLdsBudget4 : kLdsBudget, bat ? 0 : ctx->n_cu)) {
                     if (tcs1.size() >= 3) break;
@@ -1948,6 +1950,13 @@ tnni_status tnni_autotune(tnni_ctx *ctx, int n_tiles, const tnni_tile_desc *desc
                     for (int c : tcs1)
                         if ((st = try_setting(b, c, f1)) != TNNI_OK) return st;
             }
+        }
+        // the scatter form (integer height modes only): the split form's
+        // labelling at the best split label tile found, accumulated by
+        // spreading each pixel over the cells it reaches
+        if (d0.accum_mode != TNNI_ACC_FP32_2SUM) {
+            const int tc_s = best.form == TNNI_FORM_SPLIT ? best.tile_cells : tc_split;
+            if (tc_s > 0 && (st = try_setting(best.bucket_px, tc_s, TNNI_FORM_SCATTER)) != TNNI_OK) return st;
         }
         // fewer workgroups per CU at the best sizes (kept only if clearly faster)
         {
@@ -2012,7 +2021,7 @@ tnni_status tnni_tune_load(tnni_ctx *ctx, const tnni_tune_entry *in, int n)
     if (!ctx || n < 0 || (n && !in)) return fail(ctx, TNNI_E_INVALID, "null argument");
     for (int i = 0; i < n; ++i)
         if (in[i].bucket_px < 0 || in[i].tile_cells < 0 || in[i].tile_cells > 64 || in[i].wg_per_cu < 0 ||
-            in[i].form < 0 || in[i].form > TNNI_FORM_SPLIT)
+            in[i].form < 0 || in[i].form > TNNI_FORM_SCATTER)
             return fail(ctx, TNNI_E_INVALID, "tune entry " + std::to_string(i) + " out of range");
     for (int i = 0; i < n; ++i) {
         TuneChoice c;

@@ -169,6 +169,17 @@ size_t gather_window_bytes(const TileDev &t, int gb)
     return side * side * 8;
 }
 
+size_t split_pass_lds(const TileDev &t)
+{
+    // after the window: scatter, an 8-byte sum and a 4-byte count per cell
+    // of the block; a gather of blocks of fewer cells than threads in an
+    // integer mode, the threads' 16-byte partial sums
+    if (t.scatter) return (size_t)t.gb * t.gb * 12;  // the window is read from HBM
+    size_t extra = 0;
+    if (t.mode != TNNI_ACC_FP32_2SUM && t.gb * t.gb < kBlock) extra = (size_t)kBlock * 16;
+    return ((gather_window_bytes(t, t.gb) + 15) & ~(size_t)15) + extra;
+}
+
 // Useful share of the label lanes of an interior item of tc cells: it labels
 // (tc-1)*scale + span pixels a side in 8x8 wave blocks, four waves a
 // workgroup, for the tc*scale pixels a side its cells own (a label-only
@@ -239,6 +250,10 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
     }
     if (d.accum_mode == TNNI_ACC_S0 && d.multiplier < 1) {
         err = "TNNI_ACC_S0 needs multiplier >= 1";
+        return TNNI_E_INVALID;
+    }
+    if (d.form < TNNI_FORM_AUTO || d.form > TNNI_FORM_SCATTER) {
+        err = "form must be 0 (automatic), 1 (fused), 2 (split) or 3 (scatter)";
         return TNNI_E_INVALID;
     }
     if (n > 0x7FFFFFFFull) {
@@ -350,7 +365,14 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
     // at least kSplitGain times better than fused ones, i.e. where the
     // query halo (wide query windows at large scale) dominates the labelling
     int form = d.form > 0 ? d.form : tc_tuned.form;
-    if (form != TNNI_FORM_FUSED && form != TNNI_FORM_SPLIT) {
+    if (form == TNNI_FORM_SCATTER && d.accum_mode == TNNI_ACC_FP32_2SUM) {
+        if (d.form == TNNI_FORM_SCATTER) {
+            err = "the scatter form needs an integer accumulation mode (TNNI_ACC_FIXED64 or TNNI_ACC_S0)";
+            return TNNI_E_INVALID;
+        }
+        form = 0;  // a tuned choice for another mode: decide below
+    }
+    if (form != TNNI_FORM_FUSED && form != TNNI_FORM_SPLIT && form != TNNI_FORM_SCATTER) {
         // (label-only pixel tiles use every lane of their 8x8 blocks)
         TileDev a = t;
         a.label_only = 0;
@@ -358,14 +380,34 @@ tnni_status prepare_tile(const tnni_tile_desc &d, uint64_t n, int n_cu, bool bat
         const double eff = order.empty() ? 0.0 : tile_efficiency(order[0], t.scale, query_span(t));
         form = 1.0 >= kSplitGain * eff ? TNNI_FORM_SPLIT : TNNI_FORM_FUSED;
     }
-    t.label_only = form == TNNI_FORM_SPLIT ? 1 : 0;
+    t.label_only = form == TNNI_FORM_SPLIT || form == TNNI_FORM_SCATTER ? 1 : 0;
+    t.scatter = form == TNNI_FORM_SCATTER ? 1 : 0;
     t.gitem_base = 0;
-    if (t.label_only) {
+    if (t.scatter) {
+        // scatter blocks: only their sums in LDS; the largest of 32 .. 4
+        // cells a side that leaves two blocks per CU on a tile of its own
+        // (the window overlap of neighbouring blocks falls with the side,
+        // the parallelism with it), with a window under 2^24 pixels
+        auto blocks = [&](int gb) { return (int64_t)((t.ncols + gb - 1) / gb) * ((t.nrows + gb - 1) / gb); };
+        auto window = [&](int gb) {
+            const int64_t side = (int64_t)(gb - 1) * t.scale + query_span(t);
+            return side * side;
+        };
+        t.gb = 32;
+        while (t.gb > 4 && blocks(t.gb) < 2 * (int64_t)std::max(n_cu, 1)) t.gb /= 2;
+        while (t.gb > 1 && window(t.gb) >= (int64_t)1 << 24) t.gb /= 2;
+        if (window(t.gb) >= (int64_t)1 << 24) {
+            err = "query window too wide for the scatter form";
+            return TNNI_E_GEOMETRY;
+        }
+        t.gntx = (t.ncols + t.gb - 1) / t.gb;
+        t.gnty = (t.nrows + t.gb - 1) / t.gb;
+    } else if (t.label_only) {
         // gather blocks: the largest of 16 .. 1 cells a side whose pixel
         // window fits kGatherLds
         t.gb = 16;
-        while (t.gb > 1 && gather_window_bytes(t, t.gb) > kGatherLds) t.gb /= 2;
-        if (gather_window_bytes(t, t.gb) > kGatherLds) {
+        while (t.gb > 1 && split_pass_lds(t) > kGatherLds) t.gb /= 2;
+        if (split_pass_lds(t) > kGatherLds) {
             err = "query window too wide for the split form's gather";
             return TNNI_E_GEOMETRY;
         }

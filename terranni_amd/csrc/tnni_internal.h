@@ -67,13 +67,16 @@ struct TileDev {
     // squared distances (out_d2) to device memory instead of accumulating
     // cells; the split form then accumulates the cells from those rasters
     // (k_gather_split) in blocks of gb x gb cells (gntx x gnty blocks), its
-    // items [gitem_base, gitem_base + gntx * gnty) of the batch's gather items
+    // items [gitem_base, gitem_base + gntx * gnty) of the batch's gather
+    // items, by gathering (each cell tests its query window) or, where
+    // scatter is set, by scattering (each pixel adds to the cells its
+    // nearest-site distance reaches)
     int label_only;
     int64_t item_base;          // first work item of this tile
     int64_t gitem_base;         // first gather item (tiles without: an empty range)
     int32_t *out_val;
     int gb, gntx, gnty;         // gather blocks: side in cells, blocks along x / y
-    int pad4;
+    int scatter;                // split tile accumulated by the scatter pass
     // space-time slice (T != nullptr): the sites are the points with
     // |slice_t - T[i]| <= time_radius + 1e-6, at squared time distance dt2
     // (units^2) added to their squared distance; queries add jj2; acc holds
@@ -259,6 +262,9 @@ constexpr size_t kGatherLds = 80 * 1024;   // a gather block's pixel window (two
 // The pixel window (value and squared distance, 8 bytes a pixel) of an
 // interior block of gb x gb cells of the split form's gather.
 size_t gather_window_bytes(const TileDev &t, int gb);
+// LDS of the split form's accumulation pass for one block of t.gb cells:
+// the pixel window, plus the block's cell sums where it scatters.
+size_t split_pass_lds(const TileDev &t);
 
 tnni_status compute_frame(const tnni_tile_desc &d, tnni_frame_info &f, std::string &err);
 

@@ -1162,14 +1162,156 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
     }
 }
 
-// The split form's gather pass: the batch's split tiles in blocks of
+// 8-byte slots of an interior block's pixel window in the split pass's
+// LDS, rounded up to an even count: what follows the window (the scatter's
+// sums, the gather's partial sums) starts 16-byte aligned.
+__device__ __forceinline__ size_t split_window_slots(const TileDev &t, int qa, int qb)
+{
+    const size_t side = (size_t)(t.gb - 1) * t.scale + (size_t)max(1, qb - qa + 1);
+    return (side * side + 1) & ~(size_t)1;
+}
+
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// The scatter form's accumulation of one block of cells [i0, i1) x
+// [j0, j1): every pixel of the block's window (ww x wh pixels from raster
+// pixel (kx0, ky0), read once each from the squared-distance and value
+// rasters) adds its site value to the block's cells whose query point it
+// would give up, |x - q|^2 <= d2(x) (the steal test of the gather), the way
+// cuda_nni.cu:407-488 spreads a pixel over its disk, there with two global
+// atomics per stolen pair.  Here:
+//   - a pixel visits only the cells of its disk's bounding box (a radius of
+//     at least sqrt(d2), clipped to the query window [qa, qb]);
+//   - each pixel is read once, so the window is not staged in LDS and a
+//     block can be larger than a gather block (less window overlap);
+//   - the block's sums live in LDS (s_sum, s_cnt), so nothing is added
+//     to device memory but the finished heights;
+//   - a wavefront walks the union of its lanes' boxes as offsets from
+//     each lane's first possible cell, ceil((x - qb) / scale), so the
+//     loop is uniform and for one offset up to `scale` neighbouring lanes
+//     hit one cell.  They add to LDS directly: summing them across the
+//     wavefront first (a segmented suffix sum, by shuffles or by DPP row
+//     shifts) was measured slower on gfx950 than the LDS's own handling
+//     of the conflicting atomics.
+// Integer sums only (FIXED64 or S0), so the order of the additions does
+// not change a bit of the result.
+template <int ALG, int MODE>
+__device__ __forceinline__ void scatter_block(const TileDev &t, int kx0, int ky0, int ww, int wh, int i0,
+                                              int i1, int j0, int j1, int qa, int qb,
+                                              unsigned long long *s_sum, uint32_t *s_cnt)
+{
+    static_assert(MODE != TNNI_ACC_FP32_2SUM, "the scatter form sums integers");
+    typedef const __attribute__((address_space(1))) uint32_t gu32;
+    gu32 *val = (gu32 *)t.out_val;
+    gu32 *d2r = (gu32 *)t.out_d2;
+    const int gb = t.gb, s = t.scale, ps = t.ps_u, x0 = t.x0, y0 = t.y0, W = t.W;
+    const int lane = (int)threadIdx.x & 63;
+    for (int k = threadIdx.x; k < gb * gb; k += kBlock) {
+        s_sum[k] = 0ull;
+        s_cnt[k] = 0u;
+    }
+    __syncthreads();
+    constexpr int kBig = 1 << 30;
+    const int npx = ww > 0 && wh > 0 ? ww * wh : 0;
+    // window pixel p: row p / ww by a float reciprocal, corrected (p < 2^24)
+    const float inv_ww = 1.0f / (float)max(ww, 1);
+    auto fetch = [&](int p, int &wx, int &wy) -> uint2 {
+        wy = (int)((float)p * inv_ww);
+        wx = p - wy * ww;
+        if (wx < 0) {
+            --wy;
+            wx += ww;
+        } else if (wx >= ww) {
+            ++wy;
+            wx -= ww;
+        }
+        if (p >= npx) return make_uint2(kNoD2, 0u);
+        const int64_t o = (int64_t)(ky0 + wy) * W + kx0 + wx;
+        return make_uint2(d2r[o], val[o]);
+    };
+    int nwx, nwy;
+    uint2 nxt = fetch((int)threadIdx.x, nwx, nwy);  // the next pixel, loaded a round ahead
+    for (int p0 = (int)threadIdx.x - lane; p0 < npx; p0 += kBlock) {  // wave-uniform
+        const int p = p0 + lane;
+        const bool live = p < npx;
+        const uint2 e = nxt;
+        const int wx = nwx, wy = nwy;
+        nxt = fetch(p + kBlock, nwx, nwy);
+        const int px = kx0 + wx, py = ky0 + wy;
+        bool has = live && e.x != kNoD2;
+        if (MODE == TNNI_ACC_S0) has = has && (int32_t)e.y != 0;  // cuda_nni.cu:423
+        // the disk's box in pixel offsets from a cell's base pixel, then in
+        // cells: m = x - i * scale within [mlo, mhi]
+        const int ib = -floordiv_i(qb - px, s), jb = -floordiv_i(qb - py, s);
+        int ilo = kBig, ihi = -kBig, jlo = kBig, jhi = -kBig;
+        if (has) {
+            const int r = (int)sqrtf((float)e.x) + 2;  // >= sqrt(d2)
+            const int mlo = max(qa, -floordiv_i(r + x0, ps)), mhi = min(qb, floordiv_i(r - x0, ps));
+            const int nlo = max(qa, -floordiv_i(r + y0, ps)), nhi = min(qb, floordiv_i(r - y0, ps));
+            ilo = max(i0, -floordiv_i(mhi - px, s));
+            ihi = min(i1 - 1, floordiv_i(px - mlo, s));
+            jlo = max(j0, -floordiv_i(nhi - py, s));
+            jhi = min(j1 - 1, floordiv_i(py - nlo, s));
+            if (ilo > ihi || jlo > jhi) {  // reaches no cell of the block
+                ilo = jlo = kBig;
+                ihi = jhi = -kBig;
+            }
+        }
+        const int cx0 = wave_min(ilo - ib), cx1 = wave_max(ihi - ib);
+        const int cy0 = wave_min(jlo - jb), cy1 = wave_max(jhi - jb);
+        for (int cy = cy0; cy <= cy1; ++cy) {
+            const int j = jb + cy;
+            const uint32_t uy = (uint32_t)(y0 + (py - j * s) * ps);
+            const uint32_t uy2 = uy * uy;
+            const bool row_in = j >= jlo && j <= jhi;
+            for (int cx = cx0; cx <= cx1; ++cx) {
+                const int i = ib + cx;
+                const uint32_t ux = (uint32_t)(x0 + (px - i * s) * ps);
+                const uint32_t dq = ux * ux + uy2;
+                bool take = row_in && i >= ilo && i <= ihi && dq <= e.x;
+                if (ALG == TNNI_ALG_CONES) take = take && dq <= t.Rq2;
+                if (take) {
+                    const int cell = (j - j0) * gb + (i - i0);
+                    if (MODE == TNNI_ACC_FIXED64) atomicAdd(&s_sum[cell], (unsigned long long)(int64_t)(int32_t)e.y);
+                    else atomicAdd((uint32_t *)&s_sum[cell], e.y);
+                    atomicAdd(&s_cnt[cell], 1u);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < gb * gb; k += kBlock) {
+        const int i = i0 + k % gb, j = j0 + k / gb;
+        if (i >= i1 || j >= j1) continue;
+        Acc<MODE> acc;
+        acc.s64 = (int64_t)s_sum[k];
+        acc.numer = (uint32_t)s_sum[k];
+        acc.cnt = s_cnt[k];
+        t.out_h[(int64_t)j * t.ncols + i] = acc.result(t);
+    }
+}
+
+// The split form's accumulation pass: the batch's split tiles in blocks of
 // gb x gb cells, one workgroup a block.  The block's pixel window (every
 // pixel some cell of the block may steal) is copied from the value and
-// squared-distance rasters of the label-only items into LDS, then each
-// thread accumulates one cell from it, in the pixel order of
-// process_block's gather, so FP32 two-sum sums are bit-equal to the fused
-// form's.  The blocks of tile k are [gitem_base, gitem_base + gntx * gnty)
-// of the n_gather_items; other tiles' ranges are empty.
+// squared-distance rasters of the label-only items into LDS.  Then, in a
+// tile with scatter set, scatter_block spreads the window's pixels over
+// the block's cells; otherwise each thread gathers one cell from the
+// window, in the pixel order of process_block's gather, so FP32 two-sum
+// sums are bit-equal to the fused form's.  The blocks of tile k are
+// [gitem_base, gitem_base + gntx * gnty) of the n_gather_items; other
+// tiles' ranges are empty.
 template <int ALG, int MODE>
 __global__ void __launch_bounds__(kBlock)
 k_gather_split(const BatchDev *b, const TileDev *tiles)
@@ -1194,6 +1336,15 @@ k_gather_split(const BatchDev *b, const TileDev *tiles)
         const int kx0 = max(0, i0 * scale + qa), kx1 = min(W - 1, (i1 - 1) * scale + qb);
         const int ky0 = max(0, j0 * scale + qa), ky1 = min(H - 1, (j1 - 1) * scale + qb);
         const int ww = kx1 - kx0 + 1, wh = ky1 - ky0 + 1;
+        if constexpr (MODE != TNNI_ACC_FP32_2SUM) {
+            if (t.scatter) {  // the block's sums are all its LDS
+                unsigned long long *s_sum = (unsigned long long *)win;
+                scatter_block<ALG, MODE>(t, kx0, ky0, ww, wh, i0, i1, j0, j1, qa, qb, s_sum,
+                                         (uint32_t *)(s_sum + gb * gb));
+                __syncthreads();
+                continue;
+            }
+        }
         typedef const __attribute__((address_space(1))) int32_t gi32;
         typedef const __attribute__((address_space(1))) uint32_t gu32;
         gi32 *val = (gi32 *)t.out_val;
@@ -1219,15 +1370,24 @@ k_gather_split(const BatchDev *b, const TileDev *tiles)
                 if (p0 + q * kBlock < npx) win[p0 + q * kBlock] = make_uint2(dv[q], vv[q]);
         }
         __syncthreads();
-        const int ci = (int)threadIdx.x % gb, cj = (int)threadIdx.x / gb;
+        // Blocks of fewer cells than threads (wide query windows): in the
+        // integer modes, whose partial sums add up exactly, the window rows
+        // of a cell are shared by `parts` threads (a power of two) and the
+        // partial sums added in LDS; FP32 two-sum keeps one thread a cell,
+        // summing in process_block's order.
+        const int cells = gb * gb;
+        const int parts = MODE != TNNI_ACC_FP32_2SUM && cells < kBlock ? kBlock / cells : 1;
+        const int cell = (int)threadIdx.x % cells, part = (int)threadIdx.x / cells;
+        const int ci = cell % gb, cj = cell / gb;
         const int i = i0 + ci, j = j0 + cj;
-        if (cj < gb && i < i1 && j < j1) {
+        Acc<MODE> acc;
+        const bool mine = part < parts && i < i1 && j < j1;
+        if (mine) {
             const int ma = max(qa, -i * scale), mb = min(qb, W - 1 - i * scale);
             const int na = max(qa, -j * scale), nb = min(qb, H - 1 - j * scale);
             const int x0 = t.x0, y0 = t.y0, ps = t.ps_u;
             const uint32_t Rq2 = t.Rq2;
-            Acc<MODE> acc;
-            for (int n = na; n <= nb; ++n) {
+            for (int n = na + part; n <= nb; n += parts) {
                 const uint32_t uy = (uint32_t)(y0 + n * ps);
                 const uint2 *wr = win + (j * scale + n - ky0) * ww + (i * scale - kx0);
 #pragma unroll 2
@@ -1240,8 +1400,23 @@ k_gather_split(const BatchDev *b, const TileDev *tiles)
                     acc.add_if(steal, (int32_t)e.y);
                 }
             }
-            t.out_h[(int64_t)j * t.ncols + i] = acc.result(t);
         }
+        if (parts > 1) {
+            // the partial sums after the window, one 16-byte slot a thread
+            uint4 *s_part = (uint4 *)(win + split_window_slots(t, qa, qb));
+            s_part[threadIdx.x] = acc.save();
+            __syncthreads();
+            if (part == 0 && mine) {
+                for (int k = 1; k < parts; ++k) {
+                    Acc<MODE> o;
+                    o.load(s_part[k * cells + cell]);
+                    acc.s64 += o.s64;
+                    acc.numer += o.numer;
+                    acc.cnt += o.cnt;
+                }
+            }
+        }
+        if (part == 0 && mine) t.out_h[(int64_t)j * t.ncols + i] = acc.result(t);
         __syncthreads();
     }
 }

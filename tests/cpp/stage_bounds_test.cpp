@@ -7,10 +7,12 @@
 // must stay inside its buffer, and the label tile must fit the dynamic LDS
 // that the launch gives the workgroup.  Also prints the stage's size and
 // the LDS of one workgroup for 256- and 384-site stages against the 160 KB
-// of a CU at 4 and 5 workgroups per CU.  Every geometry runs in both forms:
+// of a CU at 4 and 5 workgroups per CU.  Every geometry runs in every form:
 // fused (items label their cells' pixels plus the query halo and gather
-// from LDS) and split (label-only items, which must own every pixel of the
-// raster exactly once).  Exit status 0: no violation.
+// from LDS), split and scatter (label-only items, which must own every
+// pixel of the raster exactly once, then blocks of cells whose pixel
+// window, and for the scatter the cells' sums after it, fit the pass's
+// LDS).  Exit status 0: no violation.
 //
 //   stage_bounds_test [n_random_geometries]
 #include <algorithm>
@@ -41,10 +43,11 @@ int div_bm(const TileDev &t, int k) { return k / t.bm; }
 
 void check_form(const tnni_tile_desc &d, uint64_t n);
 
-// One tile in both forms.
+// One tile in every form (the scatter form in int64 fixed point).
 void check_tile(tnni_tile_desc d, uint64_t n)
 {
-    for (int form : {TNNI_FORM_FUSED, TNNI_FORM_SPLIT}) {
+    for (int form : {TNNI_FORM_FUSED, TNNI_FORM_SPLIT, TNNI_FORM_SCATTER}) {
+        if (form == TNNI_FORM_SCATTER) d.accum_mode = TNNI_ACC_FIXED64;
         d.form = form;
         check_form(d, n);
     }
@@ -57,7 +60,9 @@ void check_form(const tnni_tile_desc &d, uint64_t n)
     PrepInfo info;
     std::string err;
     if (prepare_tile(d, n, 256, false, nullptr, t, info, err) != TNNI_OK) return;
-    if (t.label_only != (d.form == TNNI_FORM_SPLIT)) violation("form not taken", d);
+    if (t.label_only != (d.form == TNNI_FORM_SPLIT || d.form == TNNI_FORM_SCATTER) ||
+        t.scatter != (d.form == TNNI_FORM_SCATTER))
+        violation("form not taken", d);
     // label-only items: how often each pixel is written
     std::vector<uint8_t> owned(t.label_only ? (size_t)t.W * t.H : 0, 0);
     ++g_tiles;
@@ -138,7 +143,20 @@ void check_form(const tnni_tile_desc &d, uint64_t n)
     // (gather_window_bytes of an interior block, under kGatherLds), every
     // cell's reads inside the window, every cell of the tile in one block
     const size_t wmax = gather_window_bytes(t, t.gb);
-    if (wmax > kGatherLds) violation("gather window past its LDS budget", d);
+    if (split_pass_lds(t) > kGatherLds) violation("split pass past its LDS budget", d);
+    // what follows the window (the scatter's sums, the gather's partial
+    // sums) starts where k_gather_split puts it, after the widest window,
+    // 16-aligned, and ends within the pass's LDS
+    const size_t side = (size_t)(t.gb - 1) * t.scale + (size_t)std::max(1, qb - qa + 1);
+    const size_t after = ((side * side + 1) & ~(size_t)1) * 8;
+    if (side * side * 8 != wmax || after % 16 || after < wmax) violation("LDS after the window misplaced", d);
+    // the scatter reads its window from HBM: only its sums in LDS, the
+    // window's pixel index below 2^24 (its row is found by a float product)
+    if (t.scatter && ((size_t)t.gb * t.gb * 12 > split_pass_lds(t) || side * side + kBlock >= (size_t)1 << 24))
+        violation("scatter sums past the pass's LDS or window too large", d);
+    const bool shared = !t.scatter && t.mode != TNNI_ACC_FP32_2SUM && t.gb * t.gb < kBlock;
+    if (shared && after + (size_t)kBlock * 16 > split_pass_lds(t)) violation("partial sums past the pass's LDS", d);
+    if (t.gb & (t.gb - 1)) violation("gather block side not a power of two", d);
     std::vector<uint8_t> cell_seen((size_t)t.nrows * t.ncols, 0);
     for (int bj = 0; bj < t.gnty; ++bj)
         for (int bi = 0; bi < t.gntx; ++bi) {
@@ -148,7 +166,7 @@ void check_form(const tnni_tile_desc &d, uint64_t n)
             const int ky0 = std::max(0, j0 * t.scale + qa), ky1 = std::min(t.H - 1, (j1 - 1) * t.scale + qb);
             const size_t ww = (size_t)std::max(0, kx1 - kx0 + 1), wh = (size_t)std::max(0, ky1 - ky0 + 1);
             if (ww * wh * 8 > wmax) violation("gather block window past the launch's LDS", d);
-            if (t.gb * t.gb > kBlock) violation("gather block larger than the workgroup", d);
+            if (!t.scatter && t.gb * t.gb > kBlock) violation("gather block larger than the workgroup", d);
             for (int j = j0; j < j1; ++j)
                 for (int i = i0; i < i1; ++i) {
                     cell_seen[(size_t)j * t.ncols + i]++;

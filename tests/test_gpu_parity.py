@@ -40,10 +40,10 @@ def test_modes(ctx, mode, alg):
     check_tile(ctx, p, t.X, t.Y, t.Z, s0=(mode == T.ACC_S0))
 
 
-@pytest.mark.parametrize("form", [T.FORM_AUTO, T.FORM_FUSED, T.FORM_SPLIT])
+@pytest.mark.parametrize("form", [T.FORM_AUTO, T.FORM_FUSED, T.FORM_SPLIT, T.FORM_SCATTER])
 def test_uniform_100k_512(ctx, form):
-    """The reference's CPU case (scale 5: a 17-pixel query window) in both
-    forms; automatic picks the split form here."""
+    """The reference's CPU case (scale 5: a 17-pixel query window) in every
+    form; automatic picks the split form here."""
     _say("uniform 100k, 512^2 at 2 m, scale 5, form", form)
     t = S.workload_uniform_100k()
     p = dataclasses.replace(t.params, form=form)
@@ -56,11 +56,12 @@ def test_split_form_without_label_outputs_equals_fused(ctx, mode, alg):
     """Heights only (the split form's rasters then live in the engine's
     scratch), on host and device buffers: bit-equal to the fused form, in
     every height mode and region shape (the gather sums the same pixels in
-    the same order)."""
+    the same order), and so is the scatter form in the integer modes."""
     t = S.make_tile(12, 30000, 120, cell_size=2.0, scale=4, kind="clustered")
     base = dataclasses.replace(t.params, accum_mode=mode, algorithm=alg, point_min=-300, multiplier=10000)
     hs = []
-    for form in (T.FORM_FUSED, T.FORM_SPLIT):
+    forms = (T.FORM_FUSED, T.FORM_SPLIT) + ((T.FORM_SCATTER,) if mode != T.ACC_FP32_2SUM else ())
+    for form in forms:
         p = dataclasses.replace(base, form=form)
         h, _, _ = ctx.tile_run(p, t.X, t.Y, t.Z)
         dX, dY, dZ = ctx.upload(t.X), ctx.upload(t.Y), ctx.upload(t.Z)
@@ -71,9 +72,33 @@ def test_split_form_without_label_outputs_equals_fused(ctx, mode, alg):
             a.free()
         assert np.array_equal(h.view(np.uint32), hd.view(np.uint32))
         hs.append(h)
-    assert np.array_equal(hs[0].view(np.uint32), hs[1].view(np.uint32))
+    for h in hs[1:]:
+        assert np.array_equal(hs[0].view(np.uint32), h.view(np.uint32))
     check_tile(ctx, dataclasses.replace(base, form=T.FORM_SPLIT), t.X, t.Y, t.Z,
                s0=mode == T.ACC_S0)
+
+
+@pytest.mark.parametrize("scale", [1, 2, 3, 5, 7])
+@pytest.mark.parametrize("kind", ["clustered", "uniform"])
+def test_scatter_form_matches_the_oracle(ctx, scale, kind):
+    """The scatter form (each pixel spread over the cells its nearest-site
+    distance reaches, lanes that hit one cell summed across the wavefront
+    first when the scale puts several pixels on a cell) at scales 1-7,
+    with labels: labels, squared distances and int64 heights bit-equal to
+    the oracle, S0 heights too, cones as well as planes."""
+    _say("scatter", scale, kind)
+    t = S.make_tile(40 + scale, 20000, 90, cell_size=1.0, scale=scale, kind=kind)
+    for mode, alg in ((T.ACC_FIXED64, T.ALG_PLANES), (T.ACC_S0, T.ALG_CONES)):
+        p = dataclasses.replace(t.params, accum_mode=mode, algorithm=alg, point_min=-300, multiplier=10000,
+                                form=T.FORM_SCATTER)
+        check_tile(ctx, p, t.X, t.Y, t.Z, s0=mode == T.ACC_S0)
+
+
+def test_scatter_form_refuses_fp32_two_sum(ctx):
+    t = S.make_tile(3, 500, 32, cell_size=0.5, scale=1, kind="uniform")
+    p = dataclasses.replace(t.params, accum_mode=T.ACC_FP32_2SUM, form=T.FORM_SCATTER)
+    with pytest.raises(T.TnniError, match="scatter form needs an integer"):
+        ctx.tile_run(p, t.X, t.Y, t.Z)
 
 
 @pytest.mark.parametrize("bucket", [0, 1, 2, 3, 5, 8])

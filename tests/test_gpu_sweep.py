@@ -5,9 +5,10 @@ region shape (planes / cones), height mode, bucket and cell-tile sizes,
 pixel offset, grid shape (rows may differ from columns), point count and
 distribution (uniform, clustered, a few, none), and for a quarter of the
 cases a space-time tile, run with its points in random time order and
-sorted by time.  2-D cases run in both forms (fused: labels with the query
+sorted by time.  2-D cases run in every form (fused: labels with the query
 halo and accumulation in one kernel; split: each pixel labelled once into a
-raster, then a gather pass).  Labels and their squared distances must be
+raster, then a gather pass; scatter, in the integer height modes: the split
+form's raster, then each pixel spread over the cells it reaches).  Labels and their squared distances must be
 bit-equal to the oracle's, heights bit-equal to the exact formula (int64
 fixed point, S0) or within 1e-5 relative of FP64 (FP32 two-sum), the
 NODATA sets equal.  A frame the fixed-point grid cannot hold must be
@@ -95,7 +96,8 @@ def test_random_geometry_matches_oracle(ctx, k):
     mode = p.accum_mode
     if c["time_r"] is None:
         ref, hs = None, []
-        for form in (T.FORM_FUSED, T.FORM_SPLIT):
+        forms = (T.FORM_FUSED, T.FORM_SPLIT) + ((T.FORM_SCATTER,) if mode != T.ACC_FP32_2SUM else ())
+        for form in forms:
             q = dataclasses.replace(p, form=form)
             (h, lab, d2), q = run_or_auto(lambda r: ctx.tile_run(r, X, Y, Z, labels=True), q)
             if ref is None:
@@ -104,8 +106,9 @@ def test_random_geometry_matches_oracle(ctx, k):
             assert np.array_equal(d2, ref["d2"]), form
             check_heights(mode, h, ref)
             hs.append(h)
-        # the two forms sum the same pixels in the same order
-        assert np.array_equal(hs[0].view(np.uint32), hs[1].view(np.uint32))
+        # the forms sum the same pixels (in the same order: FP32 two-sum)
+        for h in hs[1:]:
+            assert np.array_equal(hs[0].view(np.uint32), h.view(np.uint32))
     else:
         # the points in random time order (every slice searched), then
         # sorted by time (per-time layers, slices combined from them)
@@ -143,8 +146,9 @@ def test_random_geometries_in_one_batch(ctx, alg):
     shape share the sort passes and the work queue): every tile's labels,
     d2 and heights equal the oracle's as when it runs alone.  Cell-tile
     sizes are left to the engine (one explicit size too large for the LDS
-    would refuse the whole batch).  Every other tile takes the split form,
-    so fused and split tiles share the launch."""
+    would refuse the whole batch).  The tiles take the fused, split and (in
+    the integer modes) scatter forms in turn, so tiles of every form share
+    the launch."""
     cases = [draw_case(k) for k in range(N_CASES)]
     for mode in (T.ACC_FIXED64, T.ACC_FP32_2SUM, T.ACC_S0):
         group = []
@@ -153,8 +157,8 @@ def test_random_geometries_in_one_batch(ctx, alg):
                 continue
             t = S.make_tile(c["seed"], max(c["n"], 1), c["grid"], cell_size=c["cs"], scale=c["scale"],
                             kind=c["kind"], **c["params"])
-            p = dataclasses.replace(t.params, nrows=c["rows"], tile_cells=0,
-                                    form=(T.FORM_FUSED, T.FORM_SPLIT)[len(group) % 2])
+            forms = (T.FORM_FUSED, T.FORM_SPLIT) + ((T.FORM_SCATTER,) if mode != T.ACC_FP32_2SUM else ())
+            p = dataclasses.replace(t.params, nrows=c["rows"], tile_cells=0, form=forms[len(group) % len(forms)])
             group.append((p, t.X[:c["n"]], t.Y[:c["n"]], t.Z[:c["n"]]))
         _say("batch", alg, mode, len(group), "tiles")
         assert len(group) >= 3
