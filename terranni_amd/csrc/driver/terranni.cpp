@@ -90,7 +90,7 @@ struct Options {
     std::string engine = "plain", format = "asc", accum = "fixed64";
     int max_size = 5000;
     bool bin_only = false;
-    long bin_block = 1L << 22;
+    long bin_block = 1L << 26;
 };
 
 const char *kUsage =
@@ -125,8 +125,9 @@ const char *kUsage =
     "  --max-size N        largest raster side in pixels (default 5000, as the\n"
     "                      reference forces at terranni.cpp:697-698)\n"
     "  --bin-only          stop after binning\n"
-    "  --bin-block N       points a binning pass takes from a file at a time\n"
-    "                      (default 4194304)\n";
+    "  --bin-block N       points binned at a time: files of at most N points\n"
+    "                      go side by side, N points in all, larger files in\n"
+    "                      blocks of N (default 67108864)\n";
 
 Options parse(int argc, char **argv)
 {
