@@ -395,9 +395,11 @@ class Context:
     def _free(self, ptr):
         self._chk(lib().tnni_dev_free(self.h, ctypes.c_void_p(ptr)))
 
-    def tile_run_dev(self, p: TileParams, dX: DevArray, dY: DevArray, dZ: DevArray, n: int,
+    def tile_run_dev(self, p, dX: DevArray, dY: DevArray, dZ: DevArray, n: int,
                      d_h: DevArray, d_lab: DevArray | None = None, d_d2: DevArray | None = None):
-        d = p.to_desc()
+        """p: TileParams, or its TileDesc made once (p.to_desc()) for a
+        caller that runs the same tile many times."""
+        d = p if isinstance(p, TileDesc) else p.to_desc()
         vp = lambda a: ctypes.c_void_p(a.ptr) if a is not None else None  # noqa: E731
         self._chk(lib().tnni_tile_run_dev(self.h, ctypes.byref(d), vp(dX), vp(dY), vp(dZ), n,
                                           vp(d_h), vp(d_lab), vp(d_d2)))
@@ -411,15 +413,23 @@ class Context:
         self._chk(lib().tnni_tile_run_dev_noalloc(self.h, ctypes.byref(d), vp(dX), vp(dY), vp(dZ), n,
                                                   vp(d_h), vp(d_lab), vp(d_d2)))
 
-    def batch_run_dev(self, params, dXs, dYs, dZs, ns, d_hs, d_labs=None, d_d2s=None):
+    @staticmethod
+    def batch_args(params, dXs, dYs, dZs, ns, d_hs, d_labs=None, d_d2s=None):
+        """The C arguments of one tnni_batch_run_dev call, made once for a
+        caller that runs the same batch many times (batch_run_args)."""
         k = len(params)
         descs = (TileDesc * k)(*[p.to_desc() for p in params])
         arr = lambda xs: (ctypes.c_void_p * k)(*[x.ptr if x is not None else None for x in xs])  # noqa
         nn = (ctypes.c_uint64 * k)(*ns)
-        self._chk(lib().tnni_batch_run_dev(
-            self.h, k, descs, arr(dXs), arr(dYs), arr(dZs), nn, arr(d_hs),
-            arr(d_labs) if d_labs is not None else None,
-            arr(d_d2s) if d_d2s is not None else None))
+        return (k, descs, arr(dXs), arr(dYs), arr(dZs), nn, arr(d_hs),
+                arr(d_labs) if d_labs is not None else None,
+                arr(d_d2s) if d_d2s is not None else None)
+
+    def batch_run_args(self, args):
+        self._chk(lib().tnni_batch_run_dev(self.h, *args))
+
+    def batch_run_dev(self, params, dXs, dYs, dZs, ns, d_hs, d_labs=None, d_d2s=None):
+        self.batch_run_args(self.batch_args(params, dXs, dYs, dZs, ns, d_hs, d_labs, d_d2s))
 
     # ---- space-time ----
     def tile_run_time(self, p: TileParams, X, Y, Z, T, cur_time: float, time_radius: float,

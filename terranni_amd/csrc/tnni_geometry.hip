@@ -3,7 +3,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "tnni_internal.h"
@@ -195,7 +198,44 @@ double tile_efficiency(int tc, int scale, int span)
 // Cell-tile sizes 1..64 whose items fit `budget` bytes of LDS, best lane
 // efficiency first (ties: the larger tile).  min_items_cu > 0 (plain form):
 // sizes above 16 that leave fewer than 2 * min_items_cu items go last.
+namespace {
+
+std::vector<int> rank_tile_cells_uncached(TileDev &t, size_t budget, int min_items_cu);
+
+// The ranking reads only these fields of the tile (region_of,
+// tile_lds_bytes, tile_efficiency), so tiles of one shape share it: a
+// batch or a stream of work packages mostly repeats a few shapes, and the
+// ranking is most of a tile's host preparation.  At most kRankMemo
+// entries (then it starts afresh).
+using RankKey = std::tuple<int, int, int, int, int, int, int, int, int, int, size_t, int>;
+constexpr size_t kRankMemo = 4096;
+std::mutex g_rank_mu;
+std::map<RankKey, std::vector<int>> g_rank_memo;
+
+}  // namespace
+
 std::vector<int> rank_tile_cells(TileDev &t, size_t budget, int min_items_cu)
+{
+    const RankKey key(t.label_only, t.ncols, t.nrows, t.scale, t.W, t.H, t.Rqb, t.x0, t.ps_u, t.po, budget,
+                      min_items_cu);
+    {
+        std::lock_guard<std::mutex> lk(g_rank_mu);
+        auto it = g_rank_memo.find(key);
+        if (it != g_rank_memo.end()) {
+            if (!it->second.empty()) tile_lds_bytes(t, it->second[0]);
+            return it->second;
+        }
+    }
+    std::vector<int> out = rank_tile_cells_uncached(t, budget, min_items_cu);
+    std::lock_guard<std::mutex> lk(g_rank_mu);
+    if (g_rank_memo.size() >= kRankMemo) g_rank_memo.clear();
+    g_rank_memo.emplace(key, out);
+    return out;
+}
+
+namespace {
+
+std::vector<int> rank_tile_cells_uncached(TileDev &t, size_t budget, int min_items_cu)
 {
     if (t.label_only) {
         // pixel tiles of 8..64 pixels a side in whole 8x8 wave blocks; the
@@ -233,6 +273,8 @@ std::vector<int> rank_tile_cells(TileDev &t, size_t budget, int min_items_cu)
     if (!out.empty()) tile_lds_bytes(t, out[0]);
     return out;
 }
+
+}  // namespace
 
 // Fill everything of a TileDev that depends on the descriptor and the
 // point count (not on the batch layout).  tuned (nullable): bucket and
