@@ -415,7 +415,14 @@ k_edist_cols(const BatchDev *b, const TileDev *tiles, const uint8_t *__restrict_
 {
     const int64_t nr = b->n_rows;
     const int n_tiles = b->n_tiles;
-    for (int64_t item = blockIdx.x; item < nr; item += gridDim.x) {
+    // Workgroups go to the 8 XCDs in turn (blockIdx mod 8), each XCD with
+    // its own L2, and a row reads its neighbours' rows: so XCD x takes the
+    // contiguous x-th eighth of the rows, where those reads hit its L2.
+    const int G = (int)gridDim.x;
+    const int xcd = G >= 8 ? (int)blockIdx.x % 8 : 0;
+    const int64_t lo = G >= 8 ? nr * xcd / 8 : 0, hi = G >= 8 ? nr * (xcd + 1) / 8 : nr;
+    const int64_t step = G >= 8 ? (G - xcd + 7) / 8 : G;
+    for (int64_t item = lo + (G >= 8 ? (int)blockIdx.x / 8 : (int)blockIdx.x); item < hi; item += step) {
         const TileDev &t = tiles[tile_of_row(tiles, n_tiles, item)];
         const int row = (int)(item - t.row_base);
         const int nbx = t.nbx, nby = t.nby;
