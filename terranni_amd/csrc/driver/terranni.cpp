@@ -45,6 +45,10 @@
 
 #include <fcntl.h>
 #include <omp.h>
+#include <climits>
+#include <sys/mman.h>
+#include <sys/uio.h>
+#include <thread>
 #include <unistd.h>
 
 #include "las_reader.h"
@@ -561,8 +565,19 @@ public:
                          nrows, ncols, xll, yll, cs);
         }
         if (format == "flt" || format == "both") {
-            flt_ = std::fopen((base + ".flt").c_str(), "wb");
-            if (!flt_) die("cannot write " + base + ".flt");
+            // the whole GridFloat file mapped for writing: rows are copied
+            // in side by side (positioned writes to one file take its inode
+            // lock in turn; page faults on a shared mapping do not)
+            const std::string path = base + ".flt";
+            flt_fd_ = ::open(path.c_str(), O_RDWR | O_CREAT | O_TRUNC, 0644);
+            if (flt_fd_ < 0) die("cannot write " + path);
+            flt_bytes_ = (size_t)nrows * (size_t)ncols * sizeof(float);
+            if (flt_bytes_ > 0) {
+                if (::ftruncate(flt_fd_, (off_t)flt_bytes_) != 0) die("cannot size " + path);
+                void *m = ::mmap(nullptr, flt_bytes_, PROT_WRITE, MAP_SHARED, flt_fd_, 0);
+                if (m == MAP_FAILED) die("cannot map " + path);
+                flt_map_ = (char *)m;
+            }
             std::FILE *h = std::fopen((base + ".hdr").c_str(), "wb");
             if (!h) die("cannot write " + base + ".hdr");
             std::fprintf(h, "ncols %d\nnrows %d\nxllcorner %.15g\nyllcorner %.15g\ncellsize %.15g\nNODATA_value -9999\nbyteorder LSBFIRST\n",
@@ -573,33 +588,45 @@ public:
     ~RasterOut()
     {
         if (asc_) std::fclose(asc_);
-        if (flt_) std::fclose(flt_);
+        if (flt_map_) ::munmap(flt_map_, flt_bytes_);
+        if (flt_fd_ >= 0) ::close(flt_fd_);
+    }
+
+    bool text() const { return asc_ != nullptr; }
+
+    // The next n rows (top down) of a GridFloat-only raster without
+    // assembling them: seg(r, iov) gives row r as pieces in column order,
+    // copied into the mapped file, the rows side by side.
+    template <class Seg>
+    void write_rows_direct(int n, Seg seg)
+    {
+        const size_t row_bytes = (size_t)ncols_ * sizeof(float);
+        if (flt_off_ + (size_t)n * row_bytes > flt_bytes_) die("raster rows past the grid");
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int r = 0; r < n; ++r) {
+            std::vector<iovec> iov;
+            seg(r, iov);
+            char *dst = flt_map_ + flt_off_ + (size_t)r * row_bytes;
+            for (const iovec &v : iov) {
+                std::memcpy(dst, v.iov_base, v.iov_len);
+                dst += v.iov_len;
+            }
+        }
+        flt_off_ += (size_t)n * row_bytes;
     }
 
     // Rows of ncols heights, the first row the top one of the batch.
     void write_rows(const float *rows, int n)
     {
-        if (flt_) {
-            // the rows' bytes at the file's end, in parts written side by
-            // side at their own offsets
+        if (flt_fd_ >= 0) {
             const size_t bytes = (size_t)n * (size_t)ncols_ * sizeof(float);
-            const int fd = fileno(flt_);
-            const int np = std::max(1, std::min(omp_get_max_threads(), (int)(bytes >> 24)));
-            bool ok = true;
-#pragma omp parallel for num_threads(np) reduction(&& : ok)
+            if (flt_off_ + bytes > flt_bytes_) die("raster rows past the grid");
+            const int np = std::max(1, std::min(omp_get_max_threads(), (int)(bytes >> 22)));
+#pragma omp parallel for num_threads(np)
             for (int k = 0; k < np; ++k) {
                 const size_t b0 = bytes * (size_t)k / (size_t)np, b1 = bytes * (size_t)(k + 1) / (size_t)np;
-                size_t done = b0;
-                while (done < b1) {
-                    const ssize_t w = pwrite(fd, (const char *)rows + done, b1 - done, (off_t)(flt_off_ + done));
-                    if (w <= 0) {
-                        ok = false;
-                        break;
-                    }
-                    done += (size_t)w;
-                }
+                std::memcpy(flt_map_ + flt_off_ + b0, (const char *)rows + b0, b1 - b0);
             }
-            if (!ok) die("write failed");
             flt_off_ += bytes;
         }
         if (!asc_) return;
@@ -624,8 +651,11 @@ public:
 
 private:
     int ncols_;
-    std::FILE *asc_ = nullptr, *flt_ = nullptr;
-    size_t flt_off_ = 0;  // bytes of the GridFloat file written
+    std::FILE *asc_ = nullptr;
+    int flt_fd_ = -1;
+    char *flt_map_ = nullptr;  // the GridFloat file, mapped
+    size_t flt_bytes_ = 0;
+    size_t flt_off_ = 0;  // bytes of it written
 };
 
 // ---- plain engine: the reference's binRun + outputASC ----------------------
@@ -755,7 +785,19 @@ void write_band(RasterOut &w, const Layout &L, int row, const std::vector<const 
 {
     const int j_hi = std::min(L.nrrows, (row + 1) * L.wp_nr) - 1, j_lo = row * L.wp_nr;
     const int nb = j_hi - j_lo + 1;
-    band.assign((size_t)std::max(nb, 0) * L.nrcols, -9999.0f);
+    if (nb <= 0) return;
+    if (!w.text()) {
+        // GridFloat only: straight from the packages' rasters
+        w.write_rows_direct(nb, [&](int jj, std::vector<iovec> &iov) {
+            const int lj = j_hi - jj - j_lo;
+            for (int col = 0; col < L.cols; ++col) {
+                const int i0 = col * L.wp_nc, n = std::min(L.wp_nc, L.nrcols - i0);
+                if (n > 0) iov.push_back({(void *)(h[(size_t)col] + (size_t)lj * L.wp_nc), (size_t)n * 4});
+            }
+        });
+        return;
+    }
+    band.assign((size_t)nb * L.nrcols, -9999.0f);
 #pragma omp parallel for schedule(static)
     for (int jj = 0; jj < nb; ++jj) {
         const int j = j_hi - jj;       // global row
@@ -766,7 +808,7 @@ void write_band(RasterOut &w, const Layout &L, int row, const std::vector<const 
             if (n > 0) std::memcpy(dst + i0, h[(size_t)col] + (size_t)lj * L.wp_nc, (size_t)n * 4);
         }
     }
-    if (nb > 0) w.write_rows(band.data(), nb);
+    w.write_rows(band.data(), nb);
 }
 
 std::vector<RawPoint> read_package(const Options &o, int row, int col)
@@ -917,7 +959,18 @@ void run_batched(tnni_ctx *ctx, const Options &o, const Layout &L, const nni::Se
             }
             return fl;
         };
+        // A writer thread puts each finished row of packages into the
+        // raster while the next rows are read, interpolated and waited for.
         std::vector<float> band;
+        std::thread writer;
+        std::vector<std::unique_ptr<Flight>> writing;  // the row the writer has
+        double blocked = 0;  // the main thread waiting for the writer
+        auto join_writer = [&] {
+            if (!writer.joinable()) return;
+            const double a = wall();
+            writer.join();
+            blocked += wall() - a;
+        };
         const double ts = wall();
         auto pending = submit_row(L.rows - 1);
         const int rstep = std::max(1, L.rows / 10);
@@ -933,16 +986,23 @@ void run_batched(tnni_ctx *ctx, const Options &o, const Layout &L, const nni::Se
             for (int col = 0; col < L.cols; ++col)
                 if (tnni_wait(ctx, pending[(size_t)col]->ticket, nullptr) != TNNI_OK)
                     die(std::string("tile: ") + tnni_last_error(ctx));
-            const double tw = wall();
-            std::vector<const float *> hp;
-            for (auto &f : pending) hp.push_back(f->h.data());
-            write_band(w, L, row, hp, band);
-            out_s += wall() - tw;
+            join_writer();  // bands go into the raster in order
+            writing = std::move(pending);
+            writer = std::thread([&w, &L, &writing, &band, &out_s, row] {
+                const double tw = wall();
+                std::vector<const float *> hp;
+                for (auto &f : writing) hp.push_back(f->h.data());
+                write_band(w, L, row, hp, band);
+                out_s += wall() - tw;
+            });
             pending = std::move(next);
         }
-        nni_s += wall() - ts;
+        join_writer();
+        nni_s += wall() - ts - blocked;
     }
-    t_nni = nni_s - out_s;
+    // output overlaps the interpolation: t_out is the writer's time, t_nni
+    // the rest of the loop's (reading, interpolating, waiting for results)
+    t_nni = nni_s;
     t_out = out_s;
 }
 
