@@ -960,9 +960,10 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
     const int RW = g.rk1 - g.rk0 + 1, RH = g.rl1 - g.rl0 + 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     WaveStage &ws = reinterpret_cast<WaveStage *>(smem)[wave];
-    uint32_t *s_d2 = smem + kWaves * (sizeof(WaveStage) / 4);
-    int32_t *s_val = reinterpret_cast<int32_t *>(s_d2 + RW * RH);
-    uint16_t *s_pend = reinterpret_cast<uint16_t *>(s_val + RW * RH);  // pixels still searching
+    // per labelled pixel: its squared distance and site value side by side
+    // (one 8-byte LDS read per pixel in the gather)
+    uint2 *s_dv = reinterpret_cast<uint2 *>(smem + kWaves * (sizeof(WaveStage) / 4));
+    uint16_t *s_pend = reinterpret_cast<uint16_t *>(s_dv + RW * RH);  // pixels still searching
 
     // 1. discrete Voronoi of the block's pixels, 8x8 pixels per wave; the
     //    waves take blocks as they finish (an LDS counter), so one wave on
@@ -989,15 +990,13 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
         if (active && pending) {
             // what rings 0-1 found: d2 and, until the search ends, the index
             const int p = (l - g.rl0) * RW + (k - g.rk0);
-            s_d2[p] = (uint32_t)(best >> 32);
-            s_val[p] = (int32_t)(uint32_t)best;
+            s_dv[p] = make_uint2((uint32_t)(best >> 32), (uint32_t)best);
             s_pend[base + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull))] = (uint16_t)p;
         } else if (active) {
             // a space-time winner's value, through its index
             if (TIME) bval = best == ~0ull ? 0 : site_value(t, t.Z[(uint32_t)best]);
             const uint32_t d2 = (uint32_t)(best >> 32);
-            s_d2[(l - g.rl0) * RW + (k - g.rk0)] = d2;
-            s_val[(l - g.rl0) * RW + (k - g.rk0)] = bval;
+            s_dv[(l - g.rl0) * RW + (k - g.rk0)] = make_uint2(d2, (uint32_t)bval);
             if (k >= g.ok0 && k <= g.ok1 && l >= g.ol0 && l <= g.ol1) {
                 const int64_t o = (int64_t)l * t.W + k;
                 if (t.out_labels) t.out_labels[o] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
@@ -1027,7 +1026,8 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
     for (uint32_t q = threadIdx.x / G; q < npend; q += ngrp) {
         const int p = s_pend[q];
         const int k = g.rk0 + p % RW, l = g.rl0 + p / RW;
-        uint64_t best = ((uint64_t)s_d2[p] << 32) | (uint32_t)s_val[p];
+        const uint2 dv = s_dv[p];
+        uint64_t best = ((uint64_t)dv.x << 32) | dv.y;
         int bval = (!TIME && G == 1 && best != ~0ull) ? site_value(t, t.Z[(uint32_t)best]) : 0;
         finish_pixel<ALG, TIME>(t, offs, edist, sorted, k, l, sub, G, best, bval);
         if (G > 1) {
@@ -1042,8 +1042,7 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
         }
         if (TIME || G > 1) bval = best == ~0ull ? 0 : site_value(t, t.Z[(uint32_t)best]);
         const uint32_t d2 = (uint32_t)(best >> 32);
-        s_d2[p] = d2;
-        s_val[p] = bval;
+        s_dv[p] = make_uint2(d2, (uint32_t)bval);
         if (k >= g.ok0 && k <= g.ok1 && l >= g.ol0 && l <= g.ol1) {
             const int64_t o = (int64_t)l * t.W + k;
             if (t.out_labels) t.out_labels[o] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
@@ -1101,7 +1100,8 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
                     const int q = p0 + n * RW + m;
-                    acc.add_if(dqt[n * 3 + m] < s_d2[q] + 1u, s_val[q]);
+                    const uint2 dv = s_dv[q];
+                    acc.add_if(dqt[n * 3 + m] < dv.x + 1u, (int32_t)dv.y);
                 }
             if (!TIME || t.acc_final) t.out_h[cell] = acc.result(t);
             else t.acc[cell] = acc.save();
@@ -1115,10 +1115,11 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
                 // the query region is in the plane; a space-time query adds
                 // its time offset to the distance only
                 const uint32_t dq = ux * ux + uy * uy;
-                const uint32_t d2 = s_d2[row + m];
+                const uint2 dv = s_dv[row + m];
+                const uint32_t d2 = dv.x;
                 bool steal = d2 != kNoD2 && dq + (TIME ? t.jj2 : 0u) <= d2;
                 if (ALG == TNNI_ALG_CONES) steal = steal && dq <= t.Rq2;
-                acc.add_if(steal, s_val[row + m]);
+                acc.add_if(steal, (int32_t)dv.y);
             }
         }
         if (!TIME || t.acc_final) t.out_h[cell] = acc.result(t);
