@@ -816,8 +816,15 @@ void scheduler(tnni_ctx *ctx)
             any_busy = true;
         }
         if (launched) continue;
-        if (any_busy || !ctx->pending.empty()) {
-            // batches in flight (poll their events) or a batch window open
+        if (any_busy) {
+            // batches in flight: poll their events, yielding the core in
+            // between (a sleep would add its granularity, tens of
+            // microseconds, to every request's latency)
+            lk.unlock();
+            std::this_thread::yield();
+            lk.lock();
+        } else if (!ctx->pending.empty()) {
+            // a batch window open
             lk.unlock();
             std::this_thread::sleep_for(std::chrono::microseconds(20));
             lk.lock();
