@@ -406,6 +406,9 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     bool wg_any = false;  // some tile has no tuned cap
     bool any_label_only = false;
     std::vector<Chunk> chunks;
+    int64_t all_pts = 0;
+    for (const Prepared &pr : tiles) all_pts += pr.t.n;
+    const int cpts = chunk_points(all_pts, ctx->n_cu);
     for (size_t k = 0; k < tiles.size(); ++k) {
         TileDev &t = tiles[k].t;
         if (t.algorithm != alg || t.mode != mode || (t.T != nullptr) != time)
@@ -432,12 +435,12 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
             if (!t.out_d2) split_d2_pix += (int64_t)t.W * t.H;
         }
         // every tile has at least one (possibly empty) chunk
-        for (int64_t b = 0; b < std::max<int64_t>(t.n, 1); b += kChunkPts) {
+        for (int64_t b = 0; b < std::max<int64_t>(t.n, 1); b += cpts) {
             Chunk c;
             c.tile = (int)k;
             c.pad = 0;
             c.begin = b;
-            c.end = std::min<int64_t>(t.n, b + kChunkPts);
+            c.end = std::min<int64_t>(t.n, b + cpts);
             chunks.push_back(c);
         }
         t.nchunks = (int)chunks.size() - t.chunk_base;
@@ -484,6 +487,7 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     std::memset(&bd, 0, sizeof bd);
     bd.n_tiles = (int)tiles.size();
     bd.n_chunks = (int)chunks.size();
+    bd.chunk_pts = (uint32_t)cpts;
     bd.n_scan = (uint32_t)(hents + 1);
     bd.n_items = items;
     bd.n_rows = rows;
@@ -604,7 +608,7 @@ uint64_t request_bytes(const TileDev &t, bool labels, bool d2)
     if (labels) b += pix * 4;
     if (d2) b += pix * 4;
     if (t.label_only) b += pix * 8;  // split form: value and squared-distance rasters
-    const uint64_t chunks = (uint64_t)t.n / kChunkPts + 1;
+    const uint64_t chunks = (uint64_t)t.n / kChunkMin + 1;  // chunks are at least kChunkMin points
     // the band histograms: one entry per (band, chunk)
     return b + sizeof(TileDev) + chunks * sizeof(Chunk) + ((uint64_t)t.nbands * chunks + 1) * 4;
 }
@@ -1080,7 +1084,8 @@ tnni_status tnni_tile_run_dev_noalloc(tnni_ctx *ctx, const tnni_tile_desc *desc,
     t.out_d2 = d_out_d2;
     // the layout of a batch of this one tile (run_prepared's, with every
     // base at 0): at least one chunk, a histogram entry per (band, chunk)
-    const int64_t nch = std::max<int64_t>(1, (t.n + kChunkPts - 1) / kChunkPts);
+    const int cpts = chunk_points(t.n, ctx->n_cu);
+    const int64_t nch = std::max<int64_t>(1, (t.n + cpts - 1) / cpts);
     const int64_t bks = (int64_t)t.nbx * t.nby, hents = (int64_t)t.nbands * nch;
     t.nbx4 = (t.nbx + 3) & ~3;
     const int64_t ebytes = (int64_t)t.nbx4 * t.nby;
@@ -1131,6 +1136,7 @@ tnni_status tnni_tile_run_dev_noalloc(tnni_ctx *ctx, const tnni_tile_desc *desc,
     std::memset(&bd, 0, sizeof bd);
     bd.n_tiles = 1;
     bd.n_chunks = (int)nch;
+    bd.chunk_pts = (uint32_t)cpts;
     bd.n_scan = (uint32_t)(hents + 1);
     bd.n_items = (int64_t)t.ntx * t.nty;
     bd.n_rows = t.nby;

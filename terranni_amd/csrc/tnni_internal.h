@@ -16,7 +16,8 @@ namespace tnni {
 
 constexpr int kBlock = 256;            // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
-constexpr int kChunkPts = 16384;       // points per chunk of the sort passes
+constexpr int kChunkPts = 16384;       // points per chunk of the sort passes (at most)
+constexpr int kChunkMin = 1024;        // points per chunk of the sort passes (at least)
 constexpr int kSortBlock = 1024;       // threads of a sort-pass workgroup
 constexpr int kSortCap = 7680;         // records a band sorts in LDS (larger bands: two passes)
 constexpr uint32_t kNoD2 = 0xFFFFFFFFu;
@@ -126,9 +127,22 @@ struct BatchDev {
     int64_t n_rows;        // bucket rows of the whole batch
     int64_t n_bands;       // bands of the whole batch
     uint32_t scan_chunk;   // elements per scan block
-    uint32_t pad;
+    uint32_t chunk_pts;    // points per chunk of the sort passes
     int64_t n_gather_items;  // cell blocks the split form's gather pass accumulates
 };
+
+// Points per chunk of the sort passes for a batch of pts points on n_cu
+// CUs: the least power of two from kChunkMin up to kChunkPts that gives no
+// more chunks than CUs, so that a small tile's count and scatter passes
+// spread over the chip (100k points: 98 chunks of 1024 instead of 7 of
+// 16384) while large batches keep kChunkPts (longer runs per band).
+__host__ __device__ inline int chunk_points(int64_t pts, int n_cu)
+{
+    const int64_t per = pts / (n_cu > 0 ? n_cu : 1);
+    int c = kChunkMin;
+    while (c < kChunkPts && c < per) c <<= 1;
+    return c;
+}
 
 __host__ __device__ inline int floordiv_i(int a, int b)  // b > 0
 {

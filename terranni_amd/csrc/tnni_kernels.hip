@@ -1534,7 +1534,7 @@ hipError_t launch_gather_slice_t(const TileDev &t, const int32_t *val, const uin
 
 // The descriptors of a one-tile batch written on the device from kernel
 // arguments (no host staging): the tile, the batch header, and the tile's
-// chunks of kChunkPts points (at least one, possibly empty).
+// chunks of bd.chunk_pts points (at least one, possibly empty).
 __global__ void __launch_bounds__(kBlock)
 k_single_tile_setup(const TileDev t, const BatchDev bd, TileDev *__restrict__ tiles,
                     Chunk *__restrict__ chunks, BatchDev *__restrict__ batch)
@@ -1547,8 +1547,8 @@ k_single_tile_setup(const TileDev t, const BatchDev bd, TileDev *__restrict__ ti
         Chunk ch;
         ch.tile = 0;
         ch.pad = 0;
-        ch.begin = (int64_t)c * kChunkPts;
-        ch.end = min(t.n, ch.begin + kChunkPts);
+        ch.begin = (int64_t)c * bd.chunk_pts;
+        ch.end = min(t.n, ch.begin + (int64_t)bd.chunk_pts);
         chunks[c] = ch;
     }
 }
