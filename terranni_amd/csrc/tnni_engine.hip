@@ -90,13 +90,13 @@ struct HostBuf {
 struct GraphKey {
     const void *ptrs[10];
     size_t lds;
-    int alg, mode, main_grid, gather_grid, label_only;
+    int alg, mode, main_grid, gather_grid, label_only, dense;
     size_t gather_lds;
     bool operator==(const GraphKey &o) const
     {
         return std::memcmp(ptrs, o.ptrs, sizeof ptrs) == 0 && lds == o.lds && alg == o.alg &&
                mode == o.mode && main_grid == o.main_grid && gather_grid == o.gather_grid &&
-               gather_lds == o.gather_lds && label_only == o.label_only;
+               gather_lds == o.gather_lds && label_only == o.label_only && dense == o.dense;
     }
 };
 
@@ -518,6 +518,8 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
     cfg.mode = mode;
     cfg.time = time ? 1 : 0;
     cfg.label_only = any_label_only ? 1 : 0;
+    cfg.dense = 0;
+    for (const Prepared &pr : tiles) cfg.dense |= pr.t.dense ? 1 : 0;
     DevBuffers db;
     db.batch = (BatchDev *)a.batch.p;
     db.tiles = (const TileDev *)a.tiles.p;
@@ -557,6 +559,7 @@ tnni_status run_prepared(tnni_ctx *ctx, Arena &a, std::vector<Prepared> &tiles, 
         key.gather_grid = cfg.gather_grid;
         key.gather_lds = cfg.gather_lds;
         key.label_only = cfg.label_only;
+        key.dense = cfg.dense;
         if (!a.gexec || !(key == a.gkey)) {
             if (a.gexec) HIP_TRY(ctx, hipGraphExecDestroy(a.gexec));
             a.gexec = nullptr;
@@ -1156,6 +1159,7 @@ tnni_status tnni_tile_run_dev_noalloc(tnni_ctx *ctx, const tnni_tile_desc *desc,
     cfg.mode = t.mode;
     cfg.time = 0;
     cfg.label_only = t.label_only;
+    cfg.dense = t.dense ? 1 : 0;
     cfg.sort_grid = (int)std::min<int64_t>(4096, nch);
     cfg.band_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, t.nbands));
     cfg.row_grid = (int)std::max<int64_t>(1, std::min<int64_t>(8192, t.nby));

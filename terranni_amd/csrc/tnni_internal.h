@@ -209,6 +209,7 @@ struct LaunchCfg {
     int mode;           // TNNI_ACC_* of every tile in the batch
     int time;           // space-time slices (every tile has T)
     int label_only;     // some tile of the batch is a label-only tile
+    int dense;          // some tile of the batch is dense (ring-by-ring searches)
     int gather_grid;    // blocks of the split form's gather pass (0: none)
     size_t gather_lds;  // its dynamic LDS: the largest pixel window of a cell block
 };

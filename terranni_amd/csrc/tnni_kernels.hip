@@ -614,6 +614,41 @@ __device__ __forceinline__ void square_global_part(const TileDev &t, const uint3
     }
 }
 
+// ring_global's sites shared by the G lanes of a group (lane sub visits
+// every G-th site of each range).
+template <int ALG, bool TIME>
+__device__ __forceinline__ void ring_global_part(const TileDev &t, const uint32_t *__restrict__ offs,
+                                                 const int4 *__restrict__ sorted, int bx, int by, int r,
+                                                 int px, int py, int R, uint32_t sub, uint32_t G,
+                                                 uint64_t &best, int &bval)
+{
+    const uint32_t *base = offs + t.bk_base;
+    for (int dy = -r; dy <= r; ++dy) {
+        const int row = by + dy;
+        if (row < 0 || row >= t.nby) continue;
+        const uint32_t *rs = base + (int64_t)row * t.nbx;
+        uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+        if (dy == -r || dy == r) {
+            const int c0 = max(bx - r, 0), c1 = min(bx + r, t.nbx - 1);
+            if (c0 <= c1) {
+                a0 = rs[c0];
+                a1 = rs[c1 + 1];
+            }
+        } else {
+            if (bx - r >= 0) {
+                a0 = rs[bx - r];
+                a1 = rs[bx - r + 1];
+            }
+            if (bx + r < t.nbx) {
+                b0 = rs[bx + r];
+                b1 = rs[bx + r + 1];
+            }
+        }
+        scan_range_part<ALG, TIME>(a0, a1, sub, G, sorted, px, py, R, t.Rs2, best, bval);
+        scan_range_part<ALG, TIME>(b0, b1, sub, G, sorted, px, py, R, t.Rs2, best, bval);
+    }
+}
+
 // k / bm for a pixel index 0 <= k < 2^20: multiply by ceil(2^32 / bm)
 // (exact since k * (ceil(2^32/bm) * bm - 2^32) < 2^32); bm == 1 has no
 // 32-bit multiplier and is taken as is.
@@ -689,7 +724,7 @@ __device__ __forceinline__ void square_staged(const TileDev &t, const WaveStage 
 // site is provably farther than the best so far (strictly, so an equal
 // distance still reaches the lower index) or outside the site region, and
 // otherwise goes on later (pending) from ring max(r0, 2).
-template <int ALG, bool TIME>
+template <int ALG, bool TIME, bool DENSE>
 __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__restrict__ offs,
                                             const uint8_t *__restrict__ edist,
                                             const int4 *__restrict__ sorted, WaveStage &ws,
@@ -759,7 +794,7 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
     const int eb = py - (t.gy0 + by * bs) + 1, et = t.gy0 + (by + 1) * bs - py;
     const int e0 = min(min(el, er), min(eb, et));
     const bool cover = t.ring1_covered;
-    if (t.dense || !staged) {
+    if ((DENSE && t.dense) || !staged) {
         // ring by ring: ring 1 only where it may still hold a covering site
         // nearer than the best (gap e0).  Also where the block's window
         // overflows the stage (a cluster core in a sparser tile): there the
@@ -793,7 +828,7 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
 // these long searches (sparse stretches) run side by side on full waves
 // instead of one or two lanes of a wave each.  best / bval hold what rings
 // 0-1 found.
-template <int ALG, bool TIME>
+template <int ALG, bool TIME, bool DENSE>
 __device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *__restrict__ offs,
                                              const uint8_t *__restrict__ edist,
                                              const int4 *__restrict__ sorted, int k, int l,
@@ -807,18 +842,26 @@ __device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *_
     const int el = px - (t.gx0 + bx * bs) + 1, er = t.gx0 + (bx + 1) * bs - px;
     const int eb = py - (t.gy0 + by * bs) + 1, et = t.gy0 + (by + 1) * bs - py;
     const int e0 = min(min(el, er), min(eb, et));
-    if (t.dense) {
+    if (DENSE && t.dense) {
         // ring by ring from max(r0, 2), stopping as soon as the gap passes
         // the best so far; a sparse stretch (r0 >= 2) opens with rings r0
-        // and r0 + 1 as one square (every ring inside r0 is empty)
+        // and r0 + 1 as one square (every ring inside r0 is empty).  The G
+        // lanes of a group visit every G-th site of each range and take the
+        // group's least key after every ring, so they all stop together
         for (int r = max(r0, 2);; ++r) {
             const int gap = e0 + (r - 1) * bs;
             if (gap > t.Rsb || (uint64_t)gap * (uint64_t)gap > (best >> 32)) break;
             if (r == r0) {
-                square_global<ALG, TIME>(t, offs, sorted, bx, by, r + 1, px, py, R, best, bval);
+                square_global_part<ALG, TIME>(t, offs, sorted, bx, by, r + 1, px, py, R, sub, G, best, bval);
                 ++r;
             } else {
-                ring_global<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, best, bval);
+                ring_global_part<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, sub, G, best, bval);
+            }
+            for (uint32_t m = G >> 1; m; m >>= 1) {
+                const uint32_t ohi = (uint32_t)__shfl_xor((int)(uint32_t)(best >> 32), (int)m, 64);
+                const uint32_t olo = (uint32_t)__shfl_xor((int)(uint32_t)best, (int)m, 64);
+                const uint64_t o = ((uint64_t)ohi << 32) | olo;
+                best = o < best ? o : best;
             }
         }
         return;
@@ -949,7 +992,7 @@ struct Acc {
 // LABEL_ONLY: the batch holds label-only tiles of the split form (checked
 // per tile); false for batches of fused tiles only, which then carry none
 // of that code.
-template <int ALG, int MODE, bool TIME, bool LABEL_ONLY>
+template <int ALG, int MODE, bool TIME, bool LABEL_ONLY, bool DENSE>
 __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, uint32_t *smem,
                                               uint32_t *s_next, uint32_t *s_npend, uint32_t share,
                                               const uint32_t *__restrict__ offs,
@@ -977,7 +1020,7 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
         uint64_t best;
         int bval;
         bool pending;
-        label_block<ALG, TIME>(t, offs, edist, sorted, ws, kA, lA, kB, lB, k, l, active, best, bval,
+        label_block<ALG, TIME, DENSE>(t, offs, edist, sorted, ws, kA, lA, kB, lB, k, l, active, best, bval,
                                pending);
         // the pixels that still search: one list slot each, one LDS atomic
         // per wave
@@ -1016,12 +1059,11 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
     //     share each pixel's square and the group takes the least key of
     //     its lanes: the least (d2, index) is the same whichever lane met
     //     it.  Groups cover up to share times the workgroup's lanes in
-    //     pixels.  Ring-by-ring searches (dense tiles) keep one pixel a
-    //     thread
+    //     pixels.  Ring-by-ring searches (dense tiles) are shared the
+    //     same way, the group taking its least key after every ring
     const uint32_t npend = *s_npend;
     uint32_t G = 1;
-    if (!t.dense)
-        while (G < 64 && npend * (2 * G) <= share * (uint32_t)kBlock) G *= 2;
+    while (G < 64 && npend * (2 * G) <= share * (uint32_t)kBlock) G *= 2;
     const uint32_t sub = threadIdx.x & (G - 1), ngrp = kBlock / G;
     for (uint32_t q = threadIdx.x / G; q < npend; q += ngrp) {
         const int p = s_pend[q];
@@ -1029,7 +1071,7 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
         const uint2 dv = s_dv[p];
         uint64_t best = ((uint64_t)dv.x << 32) | dv.y;
         int bval = (!TIME && G == 1 && best != ~0ull) ? site_value(t, t.Z[(uint32_t)best]) : 0;
-        finish_pixel<ALG, TIME>(t, offs, edist, sorted, k, l, sub, G, best, bval);
+        finish_pixel<ALG, TIME, DENSE>(t, offs, edist, sorted, k, l, sub, G, best, bval);
         if (G > 1) {
             // every lane of the group is here (same q, same trip count)
             for (uint32_t m = G >> 1; m; m >>= 1) {
@@ -1127,7 +1169,10 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
     }
 }
 
-template <int ALG, int MODE, bool TIME, bool LABEL_ONLY>
+// DENSE: the batch may hold dense tiles (ring-by-ring searches); false
+// compiles those paths out (every tile then searches the sparse way, which
+// is as exact), so that batches of sparse tiles run without their code.
+template <int ALG, int MODE, bool TIME, bool LABEL_ONLY, bool DENSE = true>
 __global__ void __launch_bounds__(kBlock)
 k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
         const uint8_t *__restrict__ edist, const int4 *__restrict__ sorted)
@@ -1165,7 +1210,8 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
         const TileDev t = tiles[lo];
         const int64_t local = item - t.item_base;
         const int tj = (int)(local / t.ntx), ti = (int)(local % t.ntx);
-        process_block<ALG, MODE, TIME, LABEL_ONLY>(t, ti, tj, smem, &s_next, &s_npend, share, offs, edist, sorted);
+        process_block<ALG, MODE, TIME, LABEL_ONLY, DENSE>(t, ti, tj, smem, &s_next, &s_npend, share, offs, edist,
+                                                         sorted);
         __syncthreads();
     }
 }
@@ -1429,11 +1475,11 @@ k_gather_split(const BatchDev *b, const TileDev *tiles)
     }
 }
 
-template <int ALG, int MODE, bool TIME, bool LABEL_ONLY>
+template <int ALG, int MODE, bool TIME, bool LABEL_ONLY, bool DENSE = true>
 hipError_t launch_tiles(const DevBuffers &d, const LaunchCfg &cfg, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_tiles<ALG, MODE, TIME, LABEL_ONLY>), dim3(cfg.main_grid), dim3(kBlock), cfg.main_lds, s,
-                       d.batch, d.tiles, d.offs, d.edist, d.sorted);
+    hipLaunchKernelGGL((k_tiles<ALG, MODE, TIME, LABEL_ONLY, DENSE>), dim3(cfg.main_grid), dim3(kBlock), cfg.main_lds,
+                       s, d.batch, d.tiles, d.offs, d.edist, d.sorted);
     if (!TIME && cfg.gather_grid > 0)
         hipLaunchKernelGGL((k_gather_split<ALG, MODE>), dim3(cfg.gather_grid), dim3(kBlock), cfg.gather_lds, s,
                            d.batch, d.tiles);
@@ -1614,8 +1660,15 @@ hipError_t enqueue_pipeline(const DevBuffers &d, const LaunchCfg &cfg, hipStream
     if (ev0 && (e = hipEventRecord(ev0, s)) != hipSuccess) return e;
     // space-time slices never take the split form
     if (cfg.time && cfg.label_only) return hipErrorInvalidValue;
-    const int sel = (cfg.time ? 6 : cfg.label_only ? 12 : 0) + cfg.algorithm * 3 + cfg.mode;
+    // fused 2-D batches without a dense tile: the kernel without the dense paths
+    const int sel = (cfg.time ? 6 : cfg.label_only ? 12 : cfg.dense ? 0 : 18) + cfg.algorithm * 3 + cfg.mode;
     switch (sel) {
+    case 18: e = launch_tiles<0, 0, false, false, false>(d, cfg, s); break;
+    case 19: e = launch_tiles<0, 1, false, false, false>(d, cfg, s); break;
+    case 20: e = launch_tiles<0, 2, false, false, false>(d, cfg, s); break;
+    case 21: e = launch_tiles<1, 0, false, false, false>(d, cfg, s); break;
+    case 22: e = launch_tiles<1, 1, false, false, false>(d, cfg, s); break;
+    case 23: e = launch_tiles<1, 2, false, false, false>(d, cfg, s); break;
     case 0: e = launch_tiles<0, 0, false, false>(d, cfg, s); break;
     case 1: e = launch_tiles<0, 1, false, false>(d, cfg, s); break;
     case 2: e = launch_tiles<0, 2, false, false>(d, cfg, s); break;
