@@ -794,7 +794,7 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
     const int eb = py - (t.gy0 + by * bs) + 1, et = t.gy0 + (by + 1) * bs - py;
     const int e0 = min(min(el, er), min(eb, et));
     const bool cover = t.ring1_covered;
-    if ((DENSE && t.dense) || !staged) {
+    if (t.dense || !staged) {
         // ring by ring: ring 1 only where it may still hold a covering site
         // nearer than the best (gap e0).  Also where the block's window
         // overflows the stage (a cluster core in a sparser tile): there the
@@ -842,6 +842,20 @@ __device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *_
     const int el = px - (t.gx0 + bx * bs) + 1, er = t.gx0 + (bx + 1) * bs - px;
     const int eb = py - (t.gy0 + by * bs) + 1, et = t.gy0 + (by + 1) * bs - py;
     const int e0 = min(min(el, er), min(eb, et));
+    if (!DENSE && t.dense) {
+        // (a batch of sparse tiles: never taken, one pixel a thread)
+        for (int r = max(r0, 2);; ++r) {
+            const int gap = e0 + (r - 1) * bs;
+            if (gap > t.Rsb || (uint64_t)gap * (uint64_t)gap > (best >> 32)) break;
+            if (r == r0) {
+                square_global<ALG, TIME>(t, offs, sorted, bx, by, r + 1, px, py, R, best, bval);
+                ++r;
+            } else {
+                ring_global<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, best, bval);
+            }
+        }
+        return;
+    }
     if (DENSE && t.dense) {
         // ring by ring from max(r0, 2), stopping as soon as the gap passes
         // the best so far; a sparse stretch (r0 >= 2) opens with rings r0
@@ -1063,7 +1077,8 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
     //     same way, the group taking its least key after every ring
     const uint32_t npend = *s_npend;
     uint32_t G = 1;
-    while (G < 64 && npend * (2 * G) <= share * (uint32_t)kBlock) G *= 2;
+    if (DENSE || !t.dense)
+        while (G < 64 && npend * (2 * G) <= share * (uint32_t)kBlock) G *= 2;
     const uint32_t sub = threadIdx.x & (G - 1), ngrp = kBlock / G;
     for (uint32_t q = threadIdx.x / G; q < npend; q += ngrp) {
         const int p = s_pend[q];
@@ -1169,9 +1184,10 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
     }
 }
 
-// DENSE: the batch may hold dense tiles (ring-by-ring searches); false
-// compiles those paths out (every tile then searches the sparse way, which
-// is as exact), so that batches of sparse tiles run without their code.
+// DENSE: the batch may hold dense tiles, whose long searches (ring by
+// ring) are shared by lane groups; false builds the kernel for batches of
+// sparse tiles, whose register allocation the shared ring searches would
+// disturb (a dense tile there would search one pixel a thread, as exactly).
 template <int ALG, int MODE, bool TIME, bool LABEL_ONLY, bool DENSE = true>
 __global__ void __launch_bounds__(kBlock)
 k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
