@@ -43,11 +43,10 @@ def compact(x, sig: int = 4):
     return x
 
 
-def write_full_record(out: dict, name: str):
+def write_full_record(out: dict, name: str, d: str):
     """The whole record, unrounded and with the per-class autotune tables,
-    to gpurun_out/<name> (next to the runs' other outputs)."""
+    to <d>/<name>."""
     try:
-        d = os.path.join(ROOT, "gpurun_out")
         os.makedirs(d, exist_ok=True)
         with open(os.path.join(d, name), "w") as f:
             json.dump(out, f, indent=1)
@@ -765,6 +764,9 @@ def main():
                     help="offered tiles/s, host-buffer requests")
     ap.add_argument("--served-requests", type=int, default=2000,
                     help="requests per arrival rate")
+    ap.add_argument("--record-dir", default=os.path.join(ROOT, "bench_records"),
+                    help="where the whole unrounded record (bench_full.json, with the autotune "
+                         "tables) is written")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters against known traffic)")
@@ -929,7 +931,7 @@ def main():
     full = dict(out)
     if choices is not None:
         full["autotune_choices"] = choices
-    write_full_record(full, "bench_full.json")
+    write_full_record(full, "bench_full.json", args.record_dir)
     print(json.dumps(compact(out)))
 
 
