@@ -1190,7 +1190,7 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
 // disturb (a dense tile there would search one pixel a thread, as exactly).
 template <int ALG, int MODE, bool TIME, bool LABEL_ONLY, bool DENSE = true>
 __global__ void __launch_bounds__(kBlock)
-k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
+k_tiles(BatchDev *b, const TileDev *__restrict__ tiles, const uint32_t *__restrict__ offs,
         const uint8_t *__restrict__ edist, const int4 *__restrict__ sorted)
 {
     extern __shared__ __align__(16) uint32_t smem[];
@@ -1221,9 +1221,10 @@ k_tiles(BatchDev *b, const TileDev *tiles, const uint32_t *__restrict__ offs,
             if (tiles[mid].item_base <= item) lo = mid;
             else hi = mid - 1;
         }
-        // a private copy: loaded once per item, never re-read after the
-        // kernel's own stores
-        const TileDev t = tiles[lo];
+        // the descriptor read in place: the kernel never writes the tile
+        // array (restrict), so its fields load where they are used rather
+        // than all at once into registers for the whole item
+        const TileDev &t = tiles[lo];
         const int64_t local = item - t.item_base;
         const int tj = (int)(local / t.ntx), ti = (int)(local % t.ntx);
         process_block<ALG, MODE, TIME, LABEL_ONLY, DENSE>(t, ti, tj, smem, &s_next, &s_npend, share, offs, edist,
