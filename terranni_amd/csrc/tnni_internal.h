@@ -100,7 +100,7 @@ struct TileDev {
 // buckets a side; woff holds each window row's bucket offsets plus the end
 // of its last bucket (kWinMax + 1 a row).
 constexpr int kWinMax = 10;      // window side in buckets (8 pixels + 2)
-constexpr int kStageCap = 384;   // staged sites per wave
+constexpr int kStageCap = 256;   // staged sites per wave
 template <int CAP>
 struct WaveStageT {
     uint32_t woff[kWinMax * (kWinMax + 1)];  // bucket offsets of the window rows
@@ -271,7 +271,7 @@ size_t tile_kernel_stage_bytes();
 // LDS of one workgroup: the per-wave stages plus the label tile, under the
 // 64 KB per-group limit (static LDS takes a few more bytes)
 constexpr size_t kLdsBudget = 60 * 1024;
-constexpr size_t kLdsBudget4 = 40 * 1024;  // four workgroups per CU (160 KB LDS)
+constexpr size_t kLdsBudget4 = 32 * 1024;  // five workgroups per CU (160 KB LDS)
 constexpr size_t kGatherLds = 80 * 1024;   // a gather block's pixel window (two workgroups per CU)
 
 // The pixel window (value and squared distance, 8 bytes a pixel) of an
