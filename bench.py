@@ -679,7 +679,7 @@ def bench_spacetime(T, S, args):
     print(json.dumps(out))
 
 
-def load_traffic(kernel="k_tiles<0, 0, false, false>"):
+def load_traffic(kernel="k_tiles<0, 0, false, false, false>"):
     """HBM bytes per k_tiles launch on this workload, from the committed
     FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_single_4m.json)."""
     path = os.path.join(ROOT, "profiles", "traffic_single_4m.json")
@@ -694,7 +694,7 @@ def load_traffic(kernel="k_tiles<0, 0, false, false>"):
 VALU_PEAK = 256 * 2.4e9  # wave64 VALU instructions per second: 1 per CU per cycle
 
 
-def load_valu(kernel="k_tiles<0, 0, false, false>"):
+def load_valu(kernel="k_tiles<0, 0, false, false, false>"):
     """VALU wave-instructions per k_tiles launch on this workload, from the
     committed SQ_INSTS_VALU summary (profiles/sq_counters_single_4m.json)."""
     path = os.path.join(ROOT, "profiles", "sq_counters_single_4m.json")
@@ -706,7 +706,7 @@ def load_valu(kernel="k_tiles<0, 0, false, false>"):
     return None if k is None else k.get("SQ_INSTS_VALU")
 
 
-def load_lds(kernel="k_tiles<0, 0, false, false>"):
+def load_lds(kernel="k_tiles<0, 0, false, false, false>"):
     """LDS activity of a k_tiles launch on this workload, from the committed
     LDS counter summary (profiles/lds_counters_single_4m.json): the share of
     CU-cycles the LDS is busy (peak 128 B per CU per cycle) and the share of
