@@ -587,19 +587,18 @@ __device__ __forceinline__ void scan_range_part(uint32_t s, uint32_t e, uint32_t
     if (i < e) visit_site<ALG, TIME>(sites[i], px, py, R, R2, best, bval);
 }
 
-// square_global's sites shared by the G lanes of a group: lane sub visits
-// every G-th site of each bucket row's range.  The row offsets come nine
-// rows at a time (the square of a long search at the built-in bucket side
-// is 9 rows: one round trip instead of three).
+// The buckets [c0, c1] x [r0, r1] (inside the grid), their sites shared by
+// the G lanes of a group: lane sub visits every G-th site of each bucket
+// row's range.  The row offsets come nine rows at a time (the square of a
+// long search at the built-in bucket side is 9 rows or fewer: one round
+// trip instead of three).
 template <int ALG, bool TIME>
-__device__ __forceinline__ void square_global_part(const TileDev &t, const uint32_t *__restrict__ offs,
-                                                   const int4 *__restrict__ sorted, int bx, int by, int R,
-                                                   int px, int py, int Rr, uint32_t sub, uint32_t G,
-                                                   uint64_t &best, int &bval)
+__device__ __forceinline__ void square_global_box(const TileDev &t, const uint32_t *__restrict__ offs,
+                                                  const int4 *__restrict__ sorted, int c0, int c1, int r0,
+                                                  int r1, int px, int py, int Rr, uint32_t sub, uint32_t G,
+                                                  uint64_t &best, int &bval)
 {
     const uint32_t *base = offs + t.bk_base;
-    const int c0 = max(bx - R, 0), c1 = min(bx + R, t.nbx - 1);
-    const int r0 = max(by - R, 0), r1 = min(by + R, t.nby - 1);
     for (int row = r0; row <= r1; row += 9) {
         uint32_t a[9], e[9];
 #pragma unroll
@@ -612,6 +611,18 @@ __device__ __forceinline__ void square_global_part(const TileDev &t, const uint3
         for (int q = 0; q < 9; ++q)
             scan_range_part<ALG, TIME>(a[q], e[q], sub, G, sorted, px, py, Rr, t.Rs2, best, bval);
     }
+}
+
+// The square of buckets within R of (bx, by) shared by the G lanes of a
+// group.
+template <int ALG, bool TIME>
+__device__ __forceinline__ void square_global_part(const TileDev &t, const uint32_t *__restrict__ offs,
+                                                   const int4 *__restrict__ sorted, int bx, int by, int R,
+                                                   int px, int py, int Rr, uint32_t sub, uint32_t G,
+                                                   uint64_t &best, int &bval)
+{
+    square_global_box<ALG, TIME>(t, offs, sorted, max(bx - R, 0), min(bx + R, t.nbx - 1), max(by - R, 0),
+                                 min(by + R, t.nby - 1), px, py, Rr, sub, G, best, bval);
 }
 
 // ring_global's sites shared by the G lanes of a group (lane sub visits
@@ -894,8 +905,15 @@ __device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *_
     // bucket row, the row offsets loaded four rows at a time): the rings
     // inside it are empty (below r0) or already visited (0 and 1, a few
     // sites at this density), and a site seen twice changes no minimum
-    if (rl >= max(r0, 2))
-        square_global_part<ALG, TIME>(t, offs, sorted, bx, by, rl, px, py, R, sub, G, best, bval);
+    // and within the buckets the site region reaches: a site covers the
+    // pixel only within R of it in x and in y
+    if (rl >= max(r0, 2)) {
+        const int c0 = max(max(bx - rl, div_magic(max(px - R - t.gx0, 0), bs, t.bs_magic)), 0);
+        const int c1 = min(min(bx + rl, div_magic(px + R - t.gx0, bs, t.bs_magic)), t.nbx - 1);
+        const int q0 = max(max(by - rl, div_magic(max(py - R - t.gy0, 0), bs, t.bs_magic)), 0);
+        const int q1 = min(min(by + rl, div_magic(py + R - t.gy0, bs, t.bs_magic)), t.nby - 1);
+        square_global_box<ALG, TIME>(t, offs, sorted, c0, c1, q0, q1, px, py, R, sub, G, best, bval);
+    }
 }
 
 template <int MODE>

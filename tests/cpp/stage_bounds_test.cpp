@@ -111,6 +111,21 @@ void check_form(const tnni_tile_desc &d, uint64_t n)
                         if (row - cy0 < 0 || row - cy0 >= wh) violation("staged row", d);
                     const int c0 = std::max(bx - 1, 0) - cx0, c1 = std::min(bx + 1, t.nbx - 1) + 1 - cx0;
                     if (c0 < 0 || c1 > ww) violation("staged column", d);
+                    {
+                        // finish_pixel's long-search box: the square of radius
+                        // rl clipped to the buckets the site region reaches
+                        const int px = t.x0 + k * t.ps_u, py = t.y0 + l * t.ps_u;
+                        const int R = t.algorithm == TNNI_ALG_PLANES ? t.Rs : t.Rsb;
+                        for (int rl = 2; rl <= 6; ++rl) {
+                            const int c0 = std::max(std::max(bx - rl, std::max(px - R - t.gx0, 0) / t.bs), 0);
+                            const int c1 = std::min(std::min(bx + rl, (px + R - t.gx0) / t.bs), t.nbx - 1);
+                            const int q0 = std::max(std::max(by - rl, std::max(py - R - t.gy0, 0) / t.bs), 0);
+                            const int q1 = std::min(std::min(by + rl, (py + R - t.gy0) / t.bs), t.nby - 1);
+                            if (c0 > bx || c1 < bx || q0 > by || q1 < by) violation("long-search box misses the pixel's bucket", d);
+                            if (c0 < 0 || c1 >= t.nbx || q0 < 0 || q1 >= t.nby) violation("long-search box off the grid", d);
+                            if ((int64_t)q1 * t.nbx + c1 + 1 >= n_offs) violation("long-search box offs index", d);
+                        }
+                    }
                     const int p = (l - g.rl0) * RW + (k - g.rk0);
                     if (p < 0 || p >= RW * RH) violation("label tile index", d);
                     if (t.label_only) {
