@@ -905,13 +905,22 @@ __device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *_
     // bucket row, the row offsets loaded four rows at a time): the rings
     // inside it are empty (below r0) or already visited (0 and 1, a few
     // sites at this density), and a site seen twice changes no minimum
-    // and within the buckets the site region reaches: a site covers the
-    // pixel only within R of it in x and in y
+    // and within the buckets that can hold a site both covering the pixel
+    // and at least as near as the best so far: |dx| and |dy| at most D =
+    // min(R, floor(sqrt(best d2))) (a site at exactly the best distance may
+    // still win on its index)
     if (rl >= max(r0, 2)) {
-        const int c0 = max(max(bx - rl, div_magic(max(px - R - t.gx0, 0), bs, t.bs_magic)), 0);
-        const int c1 = min(min(bx + rl, div_magic(px + R - t.gx0, bs, t.bs_magic)), t.nbx - 1);
-        const int q0 = max(max(by - rl, div_magic(max(py - R - t.gy0, 0), bs, t.bs_magic)), 0);
-        const int q1 = min(min(by + rl, div_magic(py + R - t.gy0, bs, t.bs_magic)), t.nby - 1);
+        int D = R;
+        if (bd != 0xFFFFFFFFu) {
+            int q = (int)sqrtf((float)bd);
+            while ((uint64_t)q * (uint64_t)q > bd) --q;
+            while ((uint64_t)(q + 1) * (uint64_t)(q + 1) <= bd) ++q;
+            D = min(D, q);
+        }
+        const int c0 = max(max(bx - rl, div_magic(max(px - D - t.gx0, 0), bs, t.bs_magic)), 0);
+        const int c1 = min(min(bx + rl, div_magic(px + D - t.gx0, bs, t.bs_magic)), t.nbx - 1);
+        const int q0 = max(max(by - rl, div_magic(max(py - D - t.gy0, 0), bs, t.bs_magic)), 0);
+        const int q1 = min(min(by + rl, div_magic(py + D - t.gy0, bs, t.bs_magic)), t.nby - 1);
         square_global_box<ALG, TIME>(t, offs, sorted, c0, c1, q0, q1, px, py, R, sub, G, best, bval);
     }
 }
