@@ -631,26 +631,27 @@ template <int ALG, bool TIME>
 __device__ __forceinline__ void ring_global_part(const TileDev &t, const uint32_t *__restrict__ offs,
                                                  const int4 *__restrict__ sorted, int bx, int by, int r,
                                                  int px, int py, int R, uint32_t sub, uint32_t G,
-                                                 uint64_t &best, int &bval)
+                                                 uint64_t &best, int &bval, int xc0, int xc1, int xq0, int xq1)
 {
+    // only the ring's buckets within the columns [xc0, xc1] and rows
+    // [xq0, xq1] (inside the grid) that can still hold a winner
     const uint32_t *base = offs + t.bk_base;
-    for (int dy = -r; dy <= r; ++dy) {
+    for (int dy = max(-r, xq0 - by); dy <= min(r, xq1 - by); ++dy) {
         const int row = by + dy;
-        if (row < 0 || row >= t.nby) continue;
         const uint32_t *rs = base + (int64_t)row * t.nbx;
         uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
         if (dy == -r || dy == r) {
-            const int c0 = max(bx - r, 0), c1 = min(bx + r, t.nbx - 1);
+            const int c0 = max(bx - r, xc0), c1 = min(bx + r, xc1);
             if (c0 <= c1) {
                 a0 = rs[c0];
                 a1 = rs[c1 + 1];
             }
         } else {
-            if (bx - r >= 0) {
+            if (bx - r >= xc0) {
                 a0 = rs[bx - r];
                 a1 = rs[bx - r + 1];
             }
-            if (bx + r < t.nbx) {
+            if (bx + r <= xc1) {
                 b0 = rs[bx + r];
                 b1 = rs[bx + r + 1];
             }
@@ -834,6 +835,26 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
     pending = gap <= t.Rsb && (uint64_t)gap * (uint64_t)gap <= (best >> 32);
 }
 
+// The bucket columns and rows that can hold a site both covering the pixel
+// centre (px, py) and at least as near as the best so far: a site within R
+// of it in x and in y (the site region), and within floor(sqrt(bd)) (a
+// site at exactly the best distance may still win on its index).
+__device__ __forceinline__ void site_reach(const TileDev &t, int px, int py, int R, uint64_t bd, int &c0,
+                                           int &c1, int &q0, int &q1)
+{
+    int D = R;
+    if (bd != 0xFFFFFFFFu) {
+        int q = (int)sqrtf((float)bd);
+        while ((uint64_t)q * (uint64_t)q > bd) --q;
+        while ((uint64_t)(q + 1) * (uint64_t)(q + 1) <= bd) ++q;
+        D = min(D, q);
+    }
+    c0 = max(div_magic(max(px - D - t.gx0, 0), t.bs, t.bs_magic), 0);
+    c1 = min(div_magic(px + D - t.gx0, t.bs, t.bs_magic), t.nbx - 1);
+    q0 = max(div_magic(max(py - D - t.gy0, 0), t.bs, t.bs_magic), 0);
+    q1 = min(div_magic(py + D - t.gy0, t.bs, t.bs_magic), t.nby - 1);
+}
+
 // The rest of a pixel's search from ring max(r0, 2) on, from the global
 // arrays: the pixels of an item that get there are gathered first, so that
 // these long searches (sparse stretches) run side by side on full waves
@@ -876,11 +897,17 @@ __device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *_
         for (int r = max(r0, 2);; ++r) {
             const int gap = e0 + (r - 1) * bs;
             if (gap > t.Rsb || (uint64_t)gap * (uint64_t)gap > (best >> 32)) break;
+            // the ring's part that can still hold a winner (the group's best)
+            int c0, c1, q0, q1;
+            site_reach(t, px, py, R, best >> 32, c0, c1, q0, q1);
             if (r == r0) {
-                square_global_part<ALG, TIME>(t, offs, sorted, bx, by, r + 1, px, py, R, sub, G, best, bval);
+                square_global_box<ALG, TIME>(t, offs, sorted, max(c0, bx - r - 1), min(c1, bx + r + 1),
+                                             max(q0, by - r - 1), min(q1, by + r + 1), px, py, R, sub, G, best,
+                                             bval);
                 ++r;
             } else {
-                ring_global_part<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, sub, G, best, bval);
+                ring_global_part<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, sub, G, best, bval, c0, c1, q0,
+                                            q1);
             }
             for (uint32_t m = G >> 1; m; m >>= 1) {
                 const uint32_t ohi = (uint32_t)__shfl_xor((int)(uint32_t)(best >> 32), (int)m, 64);
@@ -910,18 +937,10 @@ __device__ __forceinline__ void finish_pixel(const TileDev &t, const uint32_t *_
     // min(R, floor(sqrt(best d2))) (a site at exactly the best distance may
     // still win on its index)
     if (rl >= max(r0, 2)) {
-        int D = R;
-        if (bd != 0xFFFFFFFFu) {
-            int q = (int)sqrtf((float)bd);
-            while ((uint64_t)q * (uint64_t)q > bd) --q;
-            while ((uint64_t)(q + 1) * (uint64_t)(q + 1) <= bd) ++q;
-            D = min(D, q);
-        }
-        const int c0 = max(max(bx - rl, div_magic(max(px - D - t.gx0, 0), bs, t.bs_magic)), 0);
-        const int c1 = min(min(bx + rl, div_magic(px + D - t.gx0, bs, t.bs_magic)), t.nbx - 1);
-        const int q0 = max(max(by - rl, div_magic(max(py - D - t.gy0, 0), bs, t.bs_magic)), 0);
-        const int q1 = min(min(by + rl, div_magic(py + D - t.gy0, bs, t.bs_magic)), t.nby - 1);
-        square_global_box<ALG, TIME>(t, offs, sorted, c0, c1, q0, q1, px, py, R, sub, G, best, bval);
+        int c0, c1, q0, q1;
+        site_reach(t, px, py, R, bd, c0, c1, q0, q1);
+        square_global_box<ALG, TIME>(t, offs, sorted, max(c0, bx - rl), min(c1, bx + rl), max(q0, by - rl),
+                                     min(q1, by + rl), px, py, R, sub, G, best, bval);
     }
 }
 
