@@ -512,6 +512,28 @@ __device__ __forceinline__ void scan_range(uint32_t s, uint32_t e, const int4 *_
     if (q < qe) visit_site<ALG, TIME>(*q, px, py, R, R2, best, bval);
 }
 
+// The bucket columns and rows that can hold a site both covering the pixel
+// centre (px, py) and at least as near as the best so far: a site within R
+// of it in x and in y (the site region), and within floor(sqrt(bd)) (a
+// site at exactly the best distance may still win on its index).
+__device__ __forceinline__ void site_reach(const TileDev &t, int px, int py, int R, uint64_t bd, int &c0,
+                                           int &c1, int &q0, int &q1)
+{
+    int D = R;
+    if (bd != 0xFFFFFFFFu) {
+        // floor(sqrt(bd)): the float root of a value below 2^32 is within
+        // 0.01 of the true root, so truncating it is off by at most one
+        int q = (int)sqrtf((float)bd);
+        q = (uint64_t)q * (uint64_t)q > bd ? q - 1 : q;
+        q = (uint64_t)(q + 1) * (uint64_t)(q + 1) <= bd ? q + 1 : q;
+        D = min(D, q);
+    }
+    c0 = max(div_magic(max(px - D - t.gx0, 0), t.bs, t.bs_magic), 0);
+    c1 = min(div_magic(px + D - t.gx0, t.bs, t.bs_magic), t.nbx - 1);
+    q0 = max(div_magic(max(py - D - t.gy0, 0), t.bs, t.bs_magic), 0);
+    q1 = min(div_magic(py + D - t.gy0, t.bs, t.bs_magic), t.nby - 1);
+}
+
 // Ring r of the bucket square around (bx, by), from the global arrays.
 template <int ALG, bool TIME>
 __device__ __forceinline__ void ring_global(const TileDev &t, const uint32_t *__restrict__ offs,
@@ -726,6 +748,50 @@ __device__ __forceinline__ void square_staged(const TileDev &t, const WaveStage 
     }
 }
 
+// Ring 1 around (bx, by) within the columns [xc0, xc1] and rows [xq0,
+// xq1] (inside the grid; site_reach), from the global arrays (stage ==
+// nullptr) or from a wave's stage.
+template <int ALG, bool TIME>
+__device__ __forceinline__ void ring1_clipped(const TileDev &t, const uint32_t *__restrict__ offs,
+                                              const int4 *__restrict__ sorted, const WaveStage *stage,
+                                              int cx0, int cy0, int wp, int bx, int by, int px, int py, int R,
+                                              int xc0, int xc1, int xq0, int xq1, uint64_t &best, int &bval)
+{
+    for (int row = max(by - 1, xq0); row <= min(by + 1, xq1); ++row) {
+        // the row's bucket offsets and their staged slots (rel = 0: global)
+        const uint32_t *wr;
+        uint32_t rel = 0;
+        int cb = 0;
+        if (stage) {
+            wr = stage->woff + (row - cy0) * wp;
+            rel = stage->rowpre[row - cy0] - wr[0];
+            cb = cx0;
+        } else {
+            wr = offs + t.bk_base + (int64_t)row * t.nbx;
+        }
+        const int4 *sites = stage ? stage->sites : sorted;
+        uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+        if (row != by) {
+            const int c0 = max(bx - 1, xc0), c1 = min(bx + 1, xc1);
+            if (c0 <= c1) {
+                a0 = wr[c0 - cb] + rel;
+                a1 = wr[c1 + 1 - cb] + rel;
+            }
+        } else {
+            if (bx - 1 >= xc0) {
+                a0 = wr[bx - 1 - cb] + rel;
+                a1 = wr[bx - cb] + rel;
+            }
+            if (bx + 1 <= xc1) {
+                b0 = wr[bx + 1 - cb] + rel;
+                b1 = wr[bx + 2 - cb] + rel;
+            }
+        }
+        scan_range<ALG, TIME>(a0, a1, sites, px, py, R, t.Rs2, best, bval);
+        scan_range<ALG, TIME>(b0, b1, sites, px, py, R, t.Rs2, best, bval);
+    }
+}
+
 // Nearest covering site of every pixel of one 8x8 block (one wave, one
 // pixel per lane; inactive lanes still take part in the staging).  Each
 // lane searches the bucket grid in rings around its pixel's bucket: rings
@@ -812,11 +878,21 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
         // overflows the stage (a cluster core in a sparser tile): there the
         // 3x3 square would read hundreds of sites per pixel from global
         // memory, and ring 0 nearly always settles the pixel
-        for (int r = r0; r <= 1; ++r) {
-            if (r > 0 && (e0 > t.Rsb || (uint64_t)e0 * (uint64_t)e0 > (best >> 32))) break;
-            if (!staged) ring_global<ALG, TIME>(t, offs, sorted, bx, by, r, px, py, R, best, bval);
-            else if (cover) ring_staged<kAllCover, TIME>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
-            else ring_staged<ALG, TIME>(t, ws, cx0, cy0, wp, bx, by, r, px, py, R, best, bval);
+        if (r0 == 0) {
+            if (!staged) ring_global<ALG, TIME>(t, offs, sorted, bx, by, 0, px, py, R, best, bval);
+            else if (cover) ring_staged<kAllCover, TIME>(t, ws, cx0, cy0, wp, bx, by, 0, px, py, R, best, bval);
+            else ring_staged<ALG, TIME>(t, ws, cx0, cy0, wp, bx, by, 0, px, py, R, best, bval);
+        }
+        // ring 1, only its buckets that can still hold a winner
+        if (r0 <= 1 && e0 <= t.Rsb && (uint64_t)e0 * (uint64_t)e0 <= (best >> 32)) {
+            int xc0, xc1, xq0, xq1;
+            site_reach(t, px, py, R, best >> 32, xc0, xc1, xq0, xq1);
+            if (cover && staged)
+                ring1_clipped<kAllCover, TIME>(t, offs, sorted, &ws, cx0, cy0, wp, bx, by, px, py, R, xc0, xc1, xq0,
+                                               xq1, best, bval);
+            else
+                ring1_clipped<ALG, TIME>(t, offs, sorted, staged ? &ws : nullptr, cx0, cy0, wp, bx, by, px, py, R,
+                                         xc0, xc1, xq0, xq1, best, bval);
         }
     } else if (r0 <= 1) {
         // rings 0 and 1 as one 3x3 square of buckets (three row ranges).
@@ -833,26 +909,6 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
     // pixels, on full waves
     const int gap = e0 + (max(r0, 2) - 1) * bs;
     pending = gap <= t.Rsb && (uint64_t)gap * (uint64_t)gap <= (best >> 32);
-}
-
-// The bucket columns and rows that can hold a site both covering the pixel
-// centre (px, py) and at least as near as the best so far: a site within R
-// of it in x and in y (the site region), and within floor(sqrt(bd)) (a
-// site at exactly the best distance may still win on its index).
-__device__ __forceinline__ void site_reach(const TileDev &t, int px, int py, int R, uint64_t bd, int &c0,
-                                           int &c1, int &q0, int &q1)
-{
-    int D = R;
-    if (bd != 0xFFFFFFFFu) {
-        int q = (int)sqrtf((float)bd);
-        while ((uint64_t)q * (uint64_t)q > bd) --q;
-        while ((uint64_t)(q + 1) * (uint64_t)(q + 1) <= bd) ++q;
-        D = min(D, q);
-    }
-    c0 = max(div_magic(max(px - D - t.gx0, 0), t.bs, t.bs_magic), 0);
-    c1 = min(div_magic(px + D - t.gx0, t.bs, t.bs_magic), t.nbx - 1);
-    q0 = max(div_magic(max(py - D - t.gy0, 0), t.bs, t.bs_magic), 0);
-    q1 = min(div_magic(py + D - t.gy0, t.bs, t.bs_magic), t.nby - 1);
 }
 
 // The rest of a pixel's search from ring max(r0, 2) on, from the global
