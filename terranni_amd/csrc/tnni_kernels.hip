@@ -748,6 +748,42 @@ __device__ __forceinline__ void square_staged(const TileDev &t, const WaveStage 
     }
 }
 
+// Rings 0 and 1 of a pixel (the 3x3 square of buckets around its own,
+// (bx, by)) from a wave's stage, pruned by the distances to its bucket's
+// edges: every site of the row above is at least eb from the pixel in y,
+// of the left column at least el in x, and so on.  The middle row first;
+// then the row above only if eb^2 <= the best d2 so far, and its corner
+// buckets only if el^2 + eb^2 (er^2 + eb^2) <= it; then the row below the
+// same way.  A site farther than the best cannot win, nor tie on d2 (a
+// tie at exactly the best distance is still visited), so the minimum is
+// the one the whole square gives.
+template <int ALG, bool TIME>
+__device__ __forceinline__ void square3_staged(const TileDev &t, const WaveStage &ws, int cx0, int cy0, int wp,
+                                               int bx, int by, int px, int py, int R, int el, int er, int eb,
+                                               int et, uint64_t &best, int &bval)
+{
+    const int c0 = max(bx - 1, 0), c1 = min(bx + 1, t.nbx - 1);
+    {
+        const uint32_t *wr = ws.woff + (by - cy0) * wp;
+        const uint32_t rel = ws.rowpre[by - cy0] - wr[0];
+        scan_range<ALG, TIME>(wr[c0 - cx0] + rel, wr[c1 + 1 - cx0] + rel, ws.sites, px, py, R, t.Rs2, best, bval);
+    }
+    const uint64_t el2 = (uint64_t)el * (uint64_t)el, er2 = (uint64_t)er * (uint64_t)er;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int row = side == 0 ? by - 1 : by + 1;
+        const int e = side == 0 ? eb : et;
+        const uint64_t bd = best >> 32;
+        const uint64_t e2 = (uint64_t)e * (uint64_t)e;
+        if (row < 0 || row >= t.nby || e2 > bd) continue;
+        const int a = e2 + el2 <= bd ? c0 : bx;
+        const int b = e2 + er2 <= bd ? c1 : bx;
+        const uint32_t *wr = ws.woff + (row - cy0) * wp;
+        const uint32_t rel = ws.rowpre[row - cy0] - wr[0];
+        scan_range<ALG, TIME>(wr[a - cx0] + rel, wr[b + 1 - cx0] + rel, ws.sites, px, py, R, t.Rs2, best, bval);
+    }
+}
+
 // Ring 1 around (bx, by) within the columns [xc0, xc1] and rows [xq0,
 // xq1] (inside the grid; site_reach), from the global arrays (stage ==
 // nullptr) or from a wave's stage.
@@ -900,8 +936,9 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
         // covers the pixel) or e0^2 > d2 after ring 0 (none of it is nearer),
         // so visiting it anyway leaves the minimum as it is.
         if (!staged) square_global<ALG, TIME>(t, offs, sorted, bx, by, 1, px, py, R, best, bval);
-        else if (cover) square_staged<kAllCover, TIME>(t, ws, cx0, cy0, wp, bx, by, 1, px, py, R, best, bval);
-        else square_staged<ALG, TIME>(t, ws, cx0, cy0, wp, bx, by, 1, px, py, R, best, bval);
+        else if (cover)
+            square3_staged<kAllCover, TIME>(t, ws, cx0, cy0, wp, bx, by, px, py, R, el, er, eb, et, best, bval);
+        else square3_staged<ALG, TIME>(t, ws, cx0, cy0, wp, bx, by, px, py, R, el, er, eb, et, best, bval);
     }
     // every site not yet visited lies in ring max(r0, 2) or beyond, at least
     // gap away: if one may still cover the pixel and be nearer, the pixel
