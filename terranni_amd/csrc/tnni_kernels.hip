@@ -635,18 +635,6 @@ __device__ __forceinline__ void square_global_box(const TileDev &t, const uint32
     }
 }
 
-// The square of buckets within R of (bx, by) shared by the G lanes of a
-// group.
-template <int ALG, bool TIME>
-__device__ __forceinline__ void square_global_part(const TileDev &t, const uint32_t *__restrict__ offs,
-                                                   const int4 *__restrict__ sorted, int bx, int by, int R,
-                                                   int px, int py, int Rr, uint32_t sub, uint32_t G,
-                                                   uint64_t &best, int &bval)
-{
-    square_global_box<ALG, TIME>(t, offs, sorted, max(bx - R, 0), min(bx + R, t.nbx - 1), max(by - R, 0),
-                                 min(by + R, t.nby - 1), px, py, Rr, sub, G, best, bval);
-}
-
 // ring_global's sites shared by the G lanes of a group (lane sub visits
 // every G-th site of each range).
 template <int ALG, bool TIME>
@@ -729,22 +717,6 @@ __device__ __forceinline__ void ring_staged(const TileDev &t, const WaveStage &w
         }
         scan_range<ALG, TIME>(a0, a1, ws.sites, px, py, R, t.Rs2, best, bval);
         scan_range<ALG, TIME>(b0, b1, ws.sites, px, py, R, t.Rs2, best, bval);
-    }
-}
-
-// The square of buckets of radius r (0: the pixel's bucket, 1: the 3x3
-// square of rings 0 and 1) around (bx, by), from the stage.
-template <int ALG, bool TIME>
-__device__ __forceinline__ void square_staged(const TileDev &t, const WaveStage &ws, int cx0, int cy0,
-                                              int wp, int bx, int by, int r, int px, int py, int R,
-                                              uint64_t &best, int &bval)
-{
-    const int c0 = max(bx - r, 0) - cx0, c1 = min(bx + r, t.nbx - 1) + 1 - cx0;
-    for (int row = max(by - r, 0); row <= min(by + r, t.nby - 1); ++row) {
-        const int j = row - cy0;
-        const uint32_t *wr = ws.woff + j * wp;
-        const uint32_t rel = ws.rowpre[j] - wr[0];  // staged slot = offset + rel
-        scan_range<ALG, TIME>(wr[c0] + rel, wr[c1] + rel, ws.sites, px, py, R, t.Rs2, best, bval);
     }
 }
 
