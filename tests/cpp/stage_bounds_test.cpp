@@ -82,6 +82,35 @@ void check_form(const tnni_tile_desc &d, uint64_t n)
             if (stage_bytes + ((size_t)RW * RH * 10 + 3) / 4 * 4 > info.lds) violation("label tile past the LDS", d);
             if (RW * RH > 65536) violation("list slot exceeds 16 bits", d);
             const int nbx8 = (RW + 7) >> 3, nby8 = (RH + 7) >> 3;
+            // finish_pixel's long-search box: the square of radius rl about
+            // the pixel's bucket clipped to the buckets its site region
+            // reaches. The box is a product of a column range set by the
+            // pixel's column and a row range set by its row, each bound
+            // monotone in rl: the item's columns and rows at the smallest and
+            // largest radii cover every pixel and radius.
+            {
+                const int R = t.algorithm == TNNI_ALG_PLANES ? t.Rs : t.Rsb;
+                int cmax = -1, qmax = -1;
+                for (int rl = 2; rl <= 6; rl += 4) {
+                    for (int k = g.rk0; k <= g.rk1; ++k) {
+                        const int bx = t.bK + div_bm(t, k), px = t.x0 + k * t.ps_u;
+                        const int c0 = std::max(std::max(bx - rl, std::max(px - R - t.gx0, 0) / t.bs), 0);
+                        const int c1 = std::min(std::min(bx + rl, (px + R - t.gx0) / t.bs), t.nbx - 1);
+                        if (c0 > bx || c1 < bx) violation("long-search box misses the pixel's bucket column", d);
+                        if (c0 < 0 || c1 >= t.nbx) violation("long-search box off the grid's columns", d);
+                        cmax = std::max(cmax, c1);
+                    }
+                    for (int l = g.rl0; l <= g.rl1; ++l) {
+                        const int by = t.bK + div_bm(t, l), py = t.y0 + l * t.ps_u;
+                        const int q0 = std::max(std::max(by - rl, std::max(py - R - t.gy0, 0) / t.bs), 0);
+                        const int q1 = std::min(std::min(by + rl, (py + R - t.gy0) / t.bs), t.nby - 1);
+                        if (q0 > by || q1 < by) violation("long-search box misses the pixel's bucket row", d);
+                        if (q0 < 0 || q1 >= t.nby) violation("long-search box off the grid's rows", d);
+                        qmax = std::max(qmax, q1);
+                    }
+                }
+                if ((int64_t)qmax * t.nbx + cmax + 1 >= n_offs) violation("long-search box offs index", d);
+            }
             for (int blk = 0; blk < nbx8 * nby8; ++blk) {
                 ++g_blocks;
                 const int kA = g.rk0 + (blk % nbx8) * 8, lA = g.rl0 + (blk / nbx8) * 8;
@@ -111,21 +140,6 @@ void check_form(const tnni_tile_desc &d, uint64_t n)
                         if (row - cy0 < 0 || row - cy0 >= wh) violation("staged row", d);
                     const int c0 = std::max(bx - 1, 0) - cx0, c1 = std::min(bx + 1, t.nbx - 1) + 1 - cx0;
                     if (c0 < 0 || c1 > ww) violation("staged column", d);
-                    {
-                        // finish_pixel's long-search box: the square of radius
-                        // rl clipped to the buckets the site region reaches
-                        const int px = t.x0 + k * t.ps_u, py = t.y0 + l * t.ps_u;
-                        const int R = t.algorithm == TNNI_ALG_PLANES ? t.Rs : t.Rsb;
-                        for (int rl = 2; rl <= 6; ++rl) {
-                            const int c0 = std::max(std::max(bx - rl, std::max(px - R - t.gx0, 0) / t.bs), 0);
-                            const int c1 = std::min(std::min(bx + rl, (px + R - t.gx0) / t.bs), t.nbx - 1);
-                            const int q0 = std::max(std::max(by - rl, std::max(py - R - t.gy0, 0) / t.bs), 0);
-                            const int q1 = std::min(std::min(by + rl, (py + R - t.gy0) / t.bs), t.nby - 1);
-                            if (c0 > bx || c1 < bx || q0 > by || q1 < by) violation("long-search box misses the pixel's bucket", d);
-                            if (c0 < 0 || c1 >= t.nbx || q0 < 0 || q1 >= t.nby) violation("long-search box off the grid", d);
-                            if ((int64_t)q1 * t.nbx + c1 + 1 >= n_offs) violation("long-search box offs index", d);
-                        }
-                    }
                     const int p = (l - g.rl0) * RW + (k - g.rk0);
                     if (p < 0 || p >= RW * RH) violation("label tile index", d);
                     if (t.label_only) {
