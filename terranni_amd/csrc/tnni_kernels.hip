@@ -320,7 +320,10 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
 #pragma unroll
             for (int q = 0; q < kSortRecs; ++q) {
                 const uint32_t i = threadIdx.x + (uint32_t)q * kSortBlock;
-                rec[q] = i < n ? banded[s0 + i] : make_int4(0, t.gx0, t.gy0, 0);
+                // (a plain guarded load: a select against other values
+                // would be made a select of addresses, one dword at a time)
+                rec[q] = make_int4(0, 0, 0, 0);
+                if (i < n) rec[q] = banded[s0 + i];
             }
 #pragma unroll
             for (int q = 0; q < kSortRecs; ++q) {
@@ -339,7 +342,8 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
 #pragma unroll
                 for (int q = 0; q < kPtsPerStep; ++q) {
                     const uint32_t p = p0 + (uint32_t)q * kSortBlock;
-                    r4[q] = p < s1 ? banded[p] : make_int4(0, t.gx0, t.gy0, 0);
+                    r4[q] = make_int4(0, 0, 0, 0);
+                    if (p < s1) r4[q] = banded[p];
                 }
 #pragma unroll
                 for (int q = 0; q < kPtsPerStep; ++q) {
@@ -393,7 +397,8 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
 #pragma unroll
                 for (int q = 0; q < kPtsPerStep; ++q) {
                     const uint32_t p = p0 + (uint32_t)q * kSortBlock;
-                    r4[q] = p < s1 ? banded[p] : make_int4(0, t.gx0, t.gy0, 0);
+                    r4[q] = make_int4(0, 0, 0, 0);
+                    if (p < s1) r4[q] = banded[p];
                 }
 #pragma unroll
                 for (int q = 0; q < kPtsPerStep; ++q) {
