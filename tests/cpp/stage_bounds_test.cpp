@@ -16,7 +16,6 @@
 //
 //   stage_bounds_test [n_random_geometries]
 #include <algorithm>
-#include <cmath>
 #include <cstdio>
 #include <vector>
 #include <cstdlib>
@@ -266,19 +265,6 @@ int main(int argc, char **argv)
         if (uni(0, 3) == 0) d.pixel_offset = uni(0, 5);
         const uint64_t n = (uint64_t)uni(0, 200000);
         check_tile(d, n);
-    }
-    // the tile kernel's window slot -> row: floor((i + 0.5) * rcp(wp)) for
-    // slots i < 128 of windows of wp <= kWinMax + 1 columns, with the
-    // hardware reciprocal (1 ulp) taken anywhere within 2 ulp of 1/wp
-    {
-        tnni_tile_desc d = base_desc(1, 1, 1.0f, 1);
-        for (int wp = 1; wp <= kWinMax + 1; ++wp)
-            for (int u = -2; u <= 2; ++u) {
-                float r = 1.0f / (float)wp;
-                for (int s = 0; s < (u < 0 ? -u : u); ++s) r = std::nextafter(r, u < 0 ? 0.0f : 2.0f);
-                for (int i = 0; i < 128; ++i)
-                    if ((int)(((float)i + 0.5f) * r) != i / wp) violation("window slot row by the reciprocal", d);
-            }
     }
     // stage and workgroup LDS at 256 and 384 sites per wave
     const size_t s256 = sizeof(WaveStageT<256>), s384 = sizeof(WaveStageT<384>);
