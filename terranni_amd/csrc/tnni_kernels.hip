@@ -23,6 +23,21 @@
 namespace tnni {
 namespace {
 
+// The descriptor's arrays are device allocations.  A pointer read from
+// memory is generic, and generic (flat) accesses also count against the
+// LDS wait counter, so the compiler waits for all of them before any LDS
+// access: these casts make them global-memory instructions.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *glob(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *glob(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 // LAS int32 -> tile-local fixed-point units, in FP64.  Mirrors the LAS
 // de-scale (las_reader.inl:59) and drawPlane's shift by the work-package
 // origin (nni.cpp:189-190, 546).
@@ -106,14 +121,18 @@ k_band_count(BatchDev *b, const TileDev *tiles, const Chunk *chunks, uint32_t *h
         const int nb = t.nbands;
         for (int i = threadIdx.x; i < nb; i += kSortBlock) h[i] = 0u;
         __syncthreads();
+        const auto gX = glob(t.X), gY = glob(t.Y), gT = glob(t.T);
+        const bool st = t.T != nullptr;
         for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kSortBlock * kChunkStep) {
             int32_t X[kChunkStep], Y[kChunkStep], Tm[kChunkStep];
+            // every load issued before any is used: past the chunk's end a
+            // lane reads its last point again, and the pass skips it
 #pragma unroll
             for (int q = 0; q < kChunkStep; ++q) {
-                const int64_t i = i0 + (int64_t)q * kSortBlock;
-                X[q] = i < ch.end ? t.X[i] : 0;
-                Y[q] = i < ch.end ? t.Y[i] : 0;
-                Tm[q] = (t.T && i < ch.end) ? t.T[i] : 0;
+                const int64_t i = min(i0 + (int64_t)q * kSortBlock, ch.end - 1);
+                X[q] = gX[i];
+                Y[q] = gY[i];
+                Tm[q] = st ? gT[i] : 0;
             }
 #pragma unroll
             for (int q = 0; q < kChunkStep; ++q) {
@@ -148,14 +167,16 @@ k_band_scatter(const BatchDev *b, const TileDev *tiles, const Chunk *chunks,
         for (int i = threadIdx.x; i < nb; i += kSortBlock)
             cur[i] = hist[t.hist_base + (int64_t)i * t.nchunks + col];
         __syncthreads();
+        const auto gX = glob(t.X), gY = glob(t.Y), gW = glob(t.T ? t.T : t.Z);
         for (int64_t i0 = ch.begin + threadIdx.x; i0 < ch.end; i0 += kSortBlock * kChunkStep) {
             int32_t X[kChunkStep], Y[kChunkStep], W[kChunkStep];
+            // every load issued before any is used (as in the count pass)
 #pragma unroll
             for (int q = 0; q < kChunkStep; ++q) {
-                const int64_t i = i0 + (int64_t)q * kSortBlock;
-                X[q] = i < ch.end ? t.X[i] : 0;
-                Y[q] = i < ch.end ? t.Y[i] : 0;
-                W[q] = i < ch.end ? (t.T ? t.T[i] : t.Z[i]) : 0;
+                const int64_t i = min(i0 + (int64_t)q * kSortBlock, ch.end - 1);
+                X[q] = gX[i];
+                Y[q] = gY[i];
+                W[q] = gW[i];
             }
             // a site record {index, x, y, w}: the index first (the low half
             // of the search key); w the site's value, or for a space-time
@@ -764,25 +785,27 @@ __device__ __forceinline__ void square3_staged(const TileDev &t, const WaveStage
 // Ring 1 around (bx, by) within the columns [xc0, xc1] and rows [xq0,
 // xq1] (inside the grid; site_reach), from the global arrays (stage ==
 // nullptr) or from a wave's stage.
-template <int ALG, bool TIME>
+template <int ALG, bool TIME, bool STAGED>
 __device__ __forceinline__ void ring1_clipped(const TileDev &t, const uint32_t *__restrict__ offs,
-                                              const int4 *__restrict__ sorted, const WaveStage *stage,
+                                              const int4 *__restrict__ sorted, const WaveStage &ws,
                                               int cx0, int cy0, int wp, int bx, int by, int px, int py, int R,
                                               int xc0, int xc1, int xq0, int xq1, uint64_t &best, int &bval)
 {
+    // (STAGED a template argument: a pointer that may be the stage or the
+    // global arrays would make every read a flat one)
     for (int row = max(by - 1, xq0); row <= min(by + 1, xq1); ++row) {
         // the row's bucket offsets and their staged slots (rel = 0: global)
         const uint32_t *wr;
         uint32_t rel = 0;
         int cb = 0;
-        if (stage) {
-            wr = stage->woff + (row - cy0) * wp;
-            rel = stage->rowpre[row - cy0] - wr[0];
+        if (STAGED) {
+            wr = ws.woff + (row - cy0) * wp;
+            rel = ws.rowpre[row - cy0] - wr[0];
             cb = cx0;
         } else {
             wr = offs + t.bk_base + (int64_t)row * t.nbx;
         }
-        const int4 *sites = stage ? stage->sites : sorted;
+        const int4 *sites = STAGED ? ws.sites : sorted;
         uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
         if (row != by) {
             const int c0 = max(bx - 1, xc0), c1 = min(bx + 1, xc1);
@@ -900,12 +923,15 @@ __device__ __forceinline__ void label_block(const TileDev &t, const uint32_t *__
         if (r0 <= 1 && e0 <= t.Rsb && (uint64_t)e0 * (uint64_t)e0 <= (best >> 32)) {
             int xc0, xc1, xq0, xq1;
             site_reach(t, px, py, R, best >> 32, xc0, xc1, xq0, xq1);
-            if (cover && staged)
-                ring1_clipped<kAllCover, TIME>(t, offs, sorted, &ws, cx0, cy0, wp, bx, by, px, py, R, xc0, xc1, xq0,
-                                               xq1, best, bval);
+            if (!staged)
+                ring1_clipped<ALG, TIME, false>(t, offs, sorted, ws, cx0, cy0, wp, bx, by, px, py, R, xc0, xc1, xq0,
+                                                xq1, best, bval);
+            else if (cover)
+                ring1_clipped<kAllCover, TIME, true>(t, offs, sorted, ws, cx0, cy0, wp, bx, by, px, py, R, xc0, xc1,
+                                                     xq0, xq1, best, bval);
             else
-                ring1_clipped<ALG, TIME>(t, offs, sorted, staged ? &ws : nullptr, cx0, cy0, wp, bx, by, px, py, R,
-                                         xc0, xc1, xq0, xq1, best, bval);
+                ring1_clipped<ALG, TIME, true>(t, offs, sorted, ws, cx0, cy0, wp, bx, by, px, py, R, xc0, xc1, xq0,
+                                               xq1, best, bval);
         }
     } else if (r0 <= 1) {
         // rings 0 and 1 as one 3x3 square of buckets (three row ranges).
@@ -1167,14 +1193,14 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
             s_pend[base + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull))] = (uint16_t)p;
         } else if (active) {
             // a space-time winner's value, through its index
-            if (TIME) bval = best == ~0ull ? 0 : site_value(t, t.Z[(uint32_t)best]);
+            if (TIME) bval = best == ~0ull ? 0 : site_value(t, glob(t.Z)[(uint32_t)best]);
             const uint32_t d2 = (uint32_t)(best >> 32);
             s_dv[(l - g.rl0) * RW + (k - g.rk0)] = make_uint2(d2, (uint32_t)bval);
             if (k >= g.ok0 && k <= g.ok1 && l >= g.ol0 && l <= g.ol1) {
                 const int64_t o = (int64_t)l * t.W + k;
-                if (t.out_labels) t.out_labels[o] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
-                if (t.out_d2) t.out_d2[o] = d2;
-                if (LABEL_ONLY && t.out_val) t.out_val[o] = bval;
+                if (t.out_labels) glob(t.out_labels)[o] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
+                if (t.out_d2) glob(t.out_d2)[o] = d2;
+                if (LABEL_ONLY && t.out_val) glob(t.out_val)[o] = bval;
             }
         }
         uint32_t nxt = 0;
@@ -1201,7 +1227,7 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
         const int k = g.rk0 + p % RW, l = g.rl0 + p / RW;
         const uint2 dv = s_dv[p];
         uint64_t best = ((uint64_t)dv.x << 32) | dv.y;
-        int bval = (!TIME && G == 1 && best != ~0ull) ? site_value(t, t.Z[(uint32_t)best]) : 0;
+        int bval = (!TIME && G == 1 && best != ~0ull) ? site_value(t, glob(t.Z)[(uint32_t)best]) : 0;
         finish_pixel<ALG, TIME, DENSE>(t, offs, edist, sorted, k, l, sub, G, best, bval);
         if (G > 1) {
             // every lane of the group is here (same q, same trip count)
@@ -1213,14 +1239,14 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
             }
             if (sub != 0) continue;
         }
-        if (TIME || G > 1) bval = best == ~0ull ? 0 : site_value(t, t.Z[(uint32_t)best]);
+        if (TIME || G > 1) bval = best == ~0ull ? 0 : site_value(t, glob(t.Z)[(uint32_t)best]);
         const uint32_t d2 = (uint32_t)(best >> 32);
         s_dv[p] = make_uint2(d2, (uint32_t)bval);
         if (k >= g.ok0 && k <= g.ok1 && l >= g.ol0 && l <= g.ol1) {
             const int64_t o = (int64_t)l * t.W + k;
-            if (t.out_labels) t.out_labels[o] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
-            if (t.out_d2) t.out_d2[o] = d2;
-            if (LABEL_ONLY && t.out_val) t.out_val[o] = bval;
+            if (t.out_labels) glob(t.out_labels)[o] = (best == ~0ull) ? -1 : (int32_t)(uint32_t)best;
+            if (t.out_d2) glob(t.out_d2)[o] = d2;
+            if (LABEL_ONLY && t.out_val) glob(t.out_val)[o] = bval;
         }
     }
     // a label-only item has written its pixels' values and squared
@@ -1276,7 +1302,7 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
                     const uint2 dv = s_dv[q];
                     acc.add_if(dqt[n * 3 + m] < dv.x + 1u, (int32_t)dv.y);
                 }
-            if (!TIME || t.acc_final) t.out_h[cell] = acc.result(t);
+            if (!TIME || t.acc_final) glob(t.out_h)[cell] = acc.result(t);
             else t.acc[cell] = acc.save();
             continue;
         }
@@ -1295,7 +1321,7 @@ __device__ __forceinline__ void process_block(const TileDev &t, int ti, int tj, 
                 acc.add_if(steal, (int32_t)dv.y);
             }
         }
-        if (!TIME || t.acc_final) t.out_h[cell] = acc.result(t);
+        if (!TIME || t.acc_final) glob(t.out_h)[cell] = acc.result(t);
         else t.acc[cell] = acc.save();
     }
 }
@@ -1485,7 +1511,7 @@ __device__ __forceinline__ void scatter_block(const TileDev &t, int kx0, int ky0
         acc.s64 = (int64_t)s_sum[k];
         acc.numer = (uint32_t)s_sum[k];
         acc.cnt = s_cnt[k];
-        t.out_h[(int64_t)j * t.ncols + i] = acc.result(t);
+        glob(t.out_h)[(int64_t)j * t.ncols + i] = acc.result(t);
     }
 }
 
@@ -1603,7 +1629,7 @@ k_gather_split(const BatchDev *b, const TileDev *tiles)
                 }
             }
         }
-        if (part == 0 && mine) t.out_h[(int64_t)j * t.ncols + i] = acc.result(t);
+        if (part == 0 && mine) glob(t.out_h)[(int64_t)j * t.ncols + i] = acc.result(t);
         __syncthreads();
     }
 }
@@ -1649,7 +1675,7 @@ k_gather_slice(const TileDev t, const int32_t *__restrict__ val, const uint32_t 
                 acc.add_if(steal, val[row + m]);
             }
         }
-        if (t.acc_final) t.out_h[cell] = acc.result(t);
+        if (t.acc_final) glob(t.out_h)[cell] = acc.result(t);
         else t.acc[cell] = acc.save();
     }
 }
