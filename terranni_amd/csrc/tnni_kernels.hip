@@ -238,15 +238,6 @@ k_scan_reduce(const BatchDev *b, const uint32_t *counts, uint32_t *partials)
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-__global__ void __launch_bounds__(kScanBlocks) k_scan_partials(uint32_t *partials)
-{
-    __shared__ uint32_t s_w[kScanBlocks / 64];
-    const uint32_t v = partials[threadIdx.x];
-    uint32_t total;
-    const uint32_t ex = block_excl_scan<kScanBlocks>(v, s_w, total);
-    partials[threadIdx.x] = ex;
-}
-
 __global__ void __launch_bounds__(kBlock)
 k_scan_down(const BatchDev *b, uint32_t *counts, const uint32_t *partials)
 {
@@ -254,7 +245,13 @@ k_scan_down(const BatchDev *b, uint32_t *counts, const uint32_t *partials)
     const uint32_t n = b->n_scan, chunk = b->scan_chunk;
     const uint64_t beg = (uint64_t)blockIdx.x * chunk;
     const uint64_t end = min((uint64_t)n, beg + chunk);
-    uint32_t run = partials[blockIdx.x];
+    // the block's first output: the totals of the blocks before it (at
+    // most kScanBlocks of them, summed here rather than by a launch of
+    // their own)
+    uint32_t before = 0;
+    for (int i = threadIdx.x; i < (int)blockIdx.x; i += kBlock) before += partials[i];
+    uint32_t run;
+    block_excl_scan<kBlock>(before, s_w, run);
     constexpr int STEP = kBlock * kScanItems;
     for (uint64_t base = beg; base < end; base += STEP) {
         uint32_t v[kScanItems];
@@ -1806,7 +1803,6 @@ hipError_t enqueue_pipeline(const DevBuffers &d, const LaunchCfg &cfg, hipStream
                        d.chunks, d.hist);
     hipLaunchKernelGGL(k_scan_reduce, dim3(kScanBlocks), dim3(kBlock), 0, s, d.batch, d.hist,
                        d.partials);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanBlocks), 0, s, d.partials);
     hipLaunchKernelGGL(k_scan_down, dim3(kScanBlocks), dim3(kBlock), 0, s, d.batch, d.hist,
                        d.partials);
     hipLaunchKernelGGL(k_band_scatter, dim3(cfg.sort_grid), dim3(kSortBlock), 0, s, d.batch, d.tiles,
