@@ -373,22 +373,57 @@ k_band_sort(const BatchDev *b, const TileDev *tiles, const uint32_t *__restrict_
             }
         }
         __syncthreads();
-        // row distances to a bucket holding a site, capped at ecap
+        // row distances to a bucket holding a site, capped at ecap.  Each
+        // row is cut into chunks of 64 columns and each chunk's non-empty
+        // buckets become one 64-bit mask (a wave ballot); a bucket's
+        // nearest non-empty bucket on either side is the nearest set bit in
+        // its own chunk's mask or, walking outward, in the next non-zero
+        // one, no farther than cap.  The masks go to the stage's memory,
+        // which holds no records yet.
         const int cap = t.ecap;
-        for (int i = threadIdx.x; i < nloc; i += kSortBlock) {
-            const int rr = i / nbx, x = i - rr * nbx;
-            int d = cap;
-            for (int q = 0; q < cap; ++q)
-                if ((x - q >= 0 && cnt[i - q]) || (x + q < nbx && cnt[i + q])) {
-                    d = q;
-                    break;
+        {
+            const int nck = (nbx + 63) >> 6, nrow = r1 - r0;
+            // at most one chunk per bucket of the band
+            static_assert(kBandBuckets * sizeof(uint64_t) <= kSortCap * sizeof(int4), "masks in the stage");
+            uint64_t *masks = reinterpret_cast<uint64_t *>(stage);
+            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+            for (int c = wave; c < nrow * nck; c += kSortBlock / 64) {
+                const int rr = c / nck, x = ((c - rr * nck) << 6) + lane;
+                const uint64_t m = __ballot(x < nbx && cnt[rr * nbx + x] != 0u);
+                if (lane == 0) masks[c] = m;
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < nloc; i += kSortBlock) {
+                const int rr = i / nbx, x = i - rr * nbx, k = x >> 6, bit = x & 63;
+                const uint64_t *row = masks + rr * nck;
+                int d = cap;
+                // left: set bits at or below the bucket's own
+                uint64_t m = row[k] & (bit == 63 ? ~0ull : ((2ull << bit) - 1ull));
+                for (int kk = k;;) {
+                    if (m) {
+                        d = min(d, x - ((kk << 6) + 63 - __clzll((long long)m)));
+                        break;
+                    }
+                    if (--kk < 0 || x - ((kk << 6) + 63) >= d) break;
+                    m = row[kk];
                 }
-            hdist[t.e_base + (int64_t)(r0 + rr) * t.nbx4 + x] = (uint8_t)d;
+                // right: set bits at or above it
+                m = row[k] & (~0ull << bit);
+                for (int kk = k;;) {
+                    if (m) {
+                        d = min(d, (kk << 6) + __ffsll((long long)m) - 1 - x);
+                        break;
+                    }
+                    if (++kk >= nck || (kk << 6) - x >= d) break;
+                    m = row[kk];
+                }
+                hdist[t.e_base + (int64_t)(r0 + rr) * t.nbx4 + x] = (uint8_t)d;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        // exclusive scan of the counters: each thread a contiguous segment
         const int seg = (nloc + kSortBlock - 1) / kSortBlock;
         const int sb = min(nloc, (int)threadIdx.x * seg), se = min(nloc, sb + seg);
+        // exclusive scan of the counters: each thread a contiguous segment
         uint32_t sum = 0;
         for (int i = sb; i < se; ++i) sum += cnt[i];
         uint32_t total;
