@@ -928,6 +928,25 @@ def main():
                            "stolen_pairs": ref["S"], "candidate_pairs": ref["K"],
                            "stolen_pairs_per_s": ref["S"] / (elapsed / args.steps)}
     ctx.close()
+    # the secondary figures once more, short, as the line's last key: a log
+    # that keeps only the line's last characters still holds them
+    summ = {"k_tiles_ms": out["roofline"]["kernel_ms"]}
+    if "valu" in out["roofline"]:
+        summ["k_tiles_valu_frac"] = out["roofline"]["valu"]["frac"]
+    if "uniform_100k" in out:
+        summ["ref100k_ms"] = out["uniform_100k"]["gpu_ms"]
+    bvp = out.get("batched_vs_plain")
+    if bvp:
+        summ["batched_speedup"] = bvp["speedup"]
+        summ["batched_cells_per_s"] = bvp["batched_cells_per_s"]
+        summ["plain_cells_per_s"] = bvp["plain_cells_per_s"]
+    srv = out.get("served")
+    if srv:
+        for kind in ("hbm_resident", "host_buffers"):
+            summ[f"served_{kind}_p50_p99_ms"] = {
+                ("saturated" if r["rate"] == 0 else str(int(r["rate"]))): [r["p50_ms"], r["p99_ms"]]
+                for r in srv.get(kind, [])}
+    out["summary"] = summ
     full = dict(out)
     if choices is not None:
         full["autotune_choices"] = choices
